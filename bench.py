@@ -1,0 +1,166 @@
+"""Headline benchmark: CTM grad-evals/s (SOCS forward + adjoint) at 2048^2, K = 24.
+
+Workload (the single-GPU configuration the metric is quoted on): one
+2048 x 2048 tile at 4 nm/px, lambda 193 nm, NA 1.35, annular source
+sigma 0.6-0.9, K = 24 SOCS kernels from the device precompute (219 x 219
+window), seeded synthetic Manhattan target, sigmoid resist alpha 50 /
+threshold 0.225, L-BFGS history 10, 50 iterations, gradient tolerance 1e-2.
+One step = one full CTM L-BFGS solve; a grad-eval is one objective
+evaluation (line-search trials included).  Inputs are resident in HBM before
+the timed region.  FP32 fields, FP64 loss / norm / L-BFGS scalars.
+
+The CPU baseline is the FP64 numpy/scipy oracle (the reference's full-FFT
+dataflow, 3K+1 N x N FFTs per grad-eval) timed on a bounded sample of
+grad-evals of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+N, K, W = 2048, 24, 219
+OPT = dict(pixel_nm=4.0, wavelength_nm=193.0, na=1.35, sigma_in=0.6, sigma_out=0.9)
+THR, ALPHA = 0.225, 50.0
+HBM_PEAK = 8.0e12
+
+
+def _load(name, path):
+    import importlib.util
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def algorithmic_bytes_per_eval(n, k, w):
+    """16 ((K+1) N^2 + K W^2): theta/target/gradient traffic, K complex64 fields
+    written and read once, kernel windows read twice."""
+    return 16 * ((k + 1) * n * n + k * w * w)
+
+
+def column_kernel_bytes(n, k):
+    """Algorithmic bytes of the fused column kernel per launch: the K complex64
+    fields the reference writes and re-reads (16 K N^2) plus the target read."""
+    return 16 * k * n * n + 4 * n * n
+
+
+def cpu_baseline(kernels, target, seconds=20.0):
+    from oracle import imaging as im
+    from oracle import ctm as oc
+    cores = len(os.sched_getaffinity(0))
+    im.set_threads(cores)
+    cache = im.Cache.from_socs(kernels)
+    theta = oc.initialize_theta(target.astype(np.float64))
+    t64 = target.astype(np.float64)
+    im.ctm_evaluate(cache, theta, t64, THR, ALPHA)            # warm-up (plans, page-in)
+    evals, t0 = 0, time.perf_counter()
+    while True:
+        im.ctm_evaluate(cache, theta, t64, THR, ALPHA)
+        evals += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or evals >= 50:
+            break
+    return {"value": evals / el, "unit": "grad-evals/s", "cores": cores, "kind": "port",
+            "sample": f"{evals} grad-evals of the 2048^2 K=24 workload (FP64 numpy/scipy oracle, "
+                      f"full N x N FFTs, {cores} threads) in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--iterations", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(tiles would shard across GPUs; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    la = _load("litho_amd", ROOT / "c-version-lithography_amd" / "litho_amd.py")
+    tg = _load("litho_targets", ROOT / "c-version-lithography_amd" / "targets.py")
+    from oracle import optics as op
+
+    ctx = la.Context(0)
+    t0 = time.perf_counter()
+    dk = ctx.socs_precompute(N, OPT["pixel_nm"], OPT["wavelength_nm"], OPT["na"], OPT["sigma_in"],
+                             OPT["sigma_out"], K)
+    t_pre = time.perf_counter() - t0
+    sk = op.SocsKernels(n=N, win_y0=dk["y0"], win_x0=dk["x0"], kernels=dk["kernels"], weights=dk["weights"],
+                        weight_sum=dk["weight_sum"])
+    plan = la.Plan.from_socs(ctx, sk, capacity=1)
+    target = tg.manhattan_target(N, 2048)
+    d_target = torch.from_numpy(target).cuda()
+    out = torch.zeros_like(d_target)
+    torch.cuda.synchronize()
+    print(f"setup: precompute {t_pre:.2f} s, window {dk['rows']}x{dk['cols']}, plan {plan.device_bytes()/2**20:.0f} MiB",
+          file=sys.stderr, flush=True)
+
+    def solve():
+        return plan.ctm_optimize(d_target, out, method="lbfgs", iterations=args.iterations, history=10, tol=1e-2,
+                                 threshold=THR, alpha=ALPHA)
+
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    evals = 0
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = solve()
+        evals += r["evaluations"]
+        infos.append(r)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = evals / elapsed
+    print(f"timed: {args.steps} solves, {evals} grad-evals in {elapsed:.3f} s; final PE {infos[-1]['final_pe']:.3f} "
+          f"status {infos[-1]['status']}", file=sys.stderr, flush=True)
+
+    # per-stage HIP-event timing on the library stream
+    theta0 = torch.zeros_like(d_target)
+    la.initialize_theta(ctx, d_target, theta0)
+    st = plan.profile_eval(theta0, d_target, threshold=THR, alpha=ALPHA, repeats=20)
+    col_s = st[1] * 1e-3
+    col_bytes = column_kernel_bytes(N, K)
+    eval_bytes = algorithmic_bytes_per_eval(N, K, W)
+    print(f"stages (ms): forward {st[0]:.3f} column {st[1]:.3f} adjoint-tail {st[2]:.3f} eval {st[3]:.3f}",
+          file=sys.stderr, flush=True)
+    line = {
+        "metric": "CTM grad-evals/s (SOCS fwd+adjoint) 2048^2, K=24",
+        "value": round(value, 3), "unit": "grad-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (fp64 reductions)", "data": "synthetic",
+        "config": {"workload": "single-tile CTM L-BFGS, 2048x2048 @4nm, K=24 (219x219 window), m=10, "
+                               "50 iterations, tol 1e-2, seeded Manhattan target",
+                   "n": N, "k": K, "window": W, "evals_per_step": round(evals / args.steps, 2)},
+        "roofline": {"bound": "hbm", "kernel": "k_col_image_adj<2048>", "achieved": round(col_bytes / col_s / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(col_bytes / col_s / HBM_PEAK, 4),
+                     "traffic": None, "launch_ms": round(st[1], 4)},
+        "eval_roofline": {"algorithmic_bytes": eval_bytes, "eval_ms": round(st[3], 4),
+                          "achieved_GBps": round(eval_bytes / (st[3] * 1e-3) / 1e9, 1),
+                          "frac": round(eval_bytes / (st[3] * 1e-3) / HBM_PEAK, 4)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(sk, target, args.cpu_seconds)
+    plan.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,290 @@
+// C ABI entry points (include/litho_amd.h).  Every call converts exceptions
+// into status codes and a thread-local message.
+#include "../../include/litho_amd.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctm_driver.hpp"
+#include "optics.hpp"
+#include "plan.hpp"
+
+struct litho_ctx {
+  litho::Context c;
+};
+
+struct litho_plan {
+  litho::Plan* p;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return LITHO_OK;
+  } catch (const litho::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "out of host memory";
+    return LITHO_ENOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return LITHO_ERUNTIME;
+  }
+}
+
+void require(bool c, const char* msg) {
+  if (!c) throw litho::Error(LITHO_EINVAL, msg);
+}
+
+litho::Plan& plan_of(litho_plan* p) {
+  require(p != nullptr && p->p != nullptr, "null plan");
+  return *p->p;
+}
+
+void set_device(const litho::Plan& p) { LITHO_HIP(hipSetDevice(p.ctx->device)); }
+
+}  // namespace
+
+extern "C" {
+
+const char* litho_last_error(void) { return g_last_error.c_str(); }
+
+int litho_supported_size(int n) { return litho::socs_ops_for(n) != nullptr ? 1 : 0; }
+
+int litho_ctx_create(int device, litho_ctx** out) {
+  return guarded([&] {
+    require(out != nullptr, "null output handle");
+    int count = 0;
+    LITHO_HIP(hipGetDeviceCount(&count));
+    require(device >= 0 && device < count, "device index out of range");
+    LITHO_HIP(hipSetDevice(device));
+    auto* c = new litho_ctx();
+    c->c.device = device;
+    if (hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      throw litho::Error(LITHO_EDEVICE, "hipStreamCreate failed");
+    }
+    *out = c;
+  });
+}
+
+int litho_ctx_destroy(litho_ctx* ctx) {
+  return guarded([&] {
+    if (ctx == nullptr) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+  });
+}
+
+int litho_ctx_synchronize(litho_ctx* ctx) {
+  return guarded([&] {
+    require(ctx != nullptr, "null context");
+    LITHO_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+void* litho_ctx_stream(litho_ctx* ctx) { return ctx != nullptr ? (void*)ctx->c.stream : nullptr; }
+
+int litho_socs_window(int n, double pixel_nm, double wavelength_nm, double na, double sigma_in, double sigma_out,
+                      int* win_rows, int* win_cols, int* win_y0, int* win_x0, int* num_source_points,
+                      double* weight_sum) {
+  return guarded([&] {
+    litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out};
+    const litho::SourceSet src = litho::make_source_set(op);
+    const litho::Window w = litho::socs_window(op, src);
+    if (win_rows) *win_rows = w.rows;
+    if (win_cols) *win_cols = w.cols;
+    if (win_y0) *win_y0 = w.y0;
+    if (win_x0) *win_x0 = w.x0;
+    if (num_source_points) *num_source_points = (int)src.fs.size();
+    if (weight_sum) *weight_sum = src.weight_sum;
+  });
+}
+
+int litho_socs_precompute(litho_ctx* ctx, int n, double pixel_nm, double wavelength_nm, double na, double sigma_in,
+                          double sigma_out, int k, double* kernels_out, double* weights_out,
+                          double* singular_values_out) {
+  return guarded([&] {
+    require(ctx != nullptr, "null context");
+    require(kernels_out != nullptr && weights_out != nullptr, "null output arrays");
+    LITHO_HIP(hipSetDevice(ctx->c.device));
+    litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out};
+    litho::socs_precompute_device(op, k, ctx->c.stream, kernels_out, weights_out, singular_values_out);
+  });
+}
+
+int litho_plan_create(litho_ctx* ctx, int n, int k, int win_rows, int win_cols, int win_y0, int win_x0,
+                      const double* kernels, const double* weights, double weight_sum, int batch_capacity,
+                      litho_plan** out) {
+  return guarded([&] {
+    require(ctx != nullptr && out != nullptr, "null context or output handle");
+    litho::Plan* p = litho::make_plan(&ctx->c, n, k, win_rows, win_cols, win_y0, win_x0, kernels, weights,
+                                      weight_sum, batch_capacity);
+    *out = new litho_plan{p};
+  });
+}
+
+int litho_plan_destroy(litho_plan* plan) {
+  return guarded([&] {
+    if (plan == nullptr) return;
+    if (plan->p) {
+      (void)hipSetDevice(plan->p->ctx->device);
+      (void)hipStreamSynchronize(plan->p->ctx->stream);
+      delete plan->p;
+    }
+    delete plan;
+  });
+}
+
+size_t litho_plan_device_bytes(const litho_plan* plan) {
+  return plan != nullptr && plan->p != nullptr ? plan->p->device_bytes : 0;
+}
+
+int litho_socs_forward(litho_plan* plan, int batch, const float* mask, float* aerial, float* wafer, double threshold,
+                       double alpha) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(mask != nullptr, "Imaging: mask is required");
+    require(std::isfinite(threshold) && std::isfinite(alpha), "Imaging: mask, threshold, and alpha must be finite");
+    set_device(p);
+    litho::EvalRequest rq;
+    rq.mode = 0;
+    rq.base = mask;
+    rq.adjoint = false;
+    rq.im.thr = (float)threshold;
+    rq.im.alpha = (float)alpha;
+    rq.im.aerial = aerial;
+    rq.im.wafer = wafer;
+    litho::run_eval(p, batch, rq);
+    LITHO_HIP(hipStreamSynchronize(p.ctx->stream));
+  });
+}
+
+int litho_pe_gradient(litho_plan* plan, int batch, const float* mask, const float* target, const float* pe_window,
+                      const float* penalty_window, double pe_weight, double penalty, double penalty_threshold,
+                      double threshold, double alpha, float* aerial, float* wafer, float* grad_mask, double* pe_loss,
+                      double* bg_loss) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(mask != nullptr && target != nullptr, "Gradient: mask and target are required");
+    require(std::isfinite(threshold) && std::isfinite(alpha), "Imaging: mask, threshold, and alpha must be finite");
+    set_device(p);
+    litho::EvalRequest rq;
+    rq.mode = 0;
+    rq.base = mask;
+    rq.im.thr = (float)threshold;
+    rq.im.alpha = (float)alpha;
+    rq.im.aerial = aerial;
+    rq.im.wafer = wafer;
+    rq.im.target = target;
+    rq.im.pe_win = pe_window;
+    rq.im.pen_win = penalty_window;
+    rq.im.pe_weight = (float)pe_weight;
+    rq.im.penalty = (float)penalty;
+    rq.im.pen_thr = (float)penalty_threshold;
+    rq.ga.scale = (float)(4.0 * alpha);
+    rq.ga.grad_mask = grad_mask;
+    litho::run_eval(p, batch, rq);
+    const double* red = litho::fetch_reductions(p, batch);
+    for (int b = 0; b < batch; ++b) {
+      if (pe_loss) pe_loss[b] = red[4 * b + 0];
+      if (bg_loss) bg_loss[b] = red[4 * b + 1];
+    }
+  });
+}
+
+int litho_ctm_eval(litho_plan* plan, int batch, const float* theta, const float* target, double threshold,
+                   double alpha, float* mask, float* grad_theta, double* pe, double* grad_norm2) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(theta != nullptr && target != nullptr, "CTM: theta and target are required");
+    set_device(p);
+    litho::EvalRequest rq;
+    rq.mode = 1;
+    rq.base = theta;
+    rq.mask_out = mask;
+    rq.im.thr = (float)threshold;
+    rq.im.alpha = (float)alpha;
+    rq.im.target = target;
+    rq.im.pe_weight = 1.f;
+    rq.ga.scale = (float)(4.0 * alpha);
+    rq.ga.theta = theta;
+    rq.ga.grad_theta = grad_theta;
+    litho::run_eval(p, batch, rq);
+    const double* red = litho::fetch_reductions(p, batch);
+    for (int b = 0; b < batch; ++b) {
+      if (pe) pe[b] = red[4 * b + 0];
+      if (grad_norm2) grad_norm2[b] = red[4 * b + 2];
+    }
+  });
+}
+
+int litho_ctm_optimize(litho_plan* plan, const float* target, const float* theta0, const litho_ctm_options* options,
+                       float* out_mask, float* out_theta, double* history, int history_capacity,
+                       litho_ctm_info* info) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(options != nullptr, "null options");
+    set_device(p);
+    litho::CtmOptions o;
+    o.method = options->method;
+    o.iterations = options->iterations;
+    o.learning_rate = options->learning_rate;
+    o.history = options->lbfgs_history;
+    o.tolerance = options->gradient_tolerance;
+    o.threshold = options->threshold;
+    o.alpha = options->alpha;
+    auto res = litho::run_ctm(p, 1, target, theta0, o, out_mask, out_theta);
+    const auto& r = res[0];
+    if (history != nullptr)
+      for (int i = 0; i < (int)r.history.size() && i < history_capacity; ++i) history[i] = r.history[i];
+    if (info != nullptr) {
+      info->evaluations = r.evaluations;
+      info->iterations = r.iterations;
+      info->final_pe = r.final_pe;
+      info->final_grad_norm = r.final_grad_norm;
+      info->status = r.status;
+    }
+  });
+}
+
+int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, const float* target, double threshold,
+                           double alpha, int repeats, double* stage_ms) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(theta != nullptr && target != nullptr && stage_ms != nullptr, "null argument");
+    set_device(p);
+    litho::EvalRequest rq;
+    rq.mode = 1;
+    rq.base = theta;
+    rq.im.thr = (float)threshold;
+    rq.im.alpha = (float)alpha;
+    rq.im.target = target;
+    rq.im.pe_weight = 1.f;
+    rq.ga.scale = (float)(4.0 * alpha);
+    rq.ga.theta = theta;
+    litho::profile_eval(p, batch, rq, repeats, stage_ms);
+  });
+}
+
+int litho_initialize_theta(litho_ctx* ctx, const float* pattern, float* theta, size_t count) {
+  return guarded([&] {
+    require(ctx != nullptr && pattern != nullptr && theta != nullptr, "null argument");
+    LITHO_HIP(hipSetDevice(ctx->c.device));
+    litho::launch_init_theta(pattern, theta, count, ctx->c.stream);
+    LITHO_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+}  // extern "C"

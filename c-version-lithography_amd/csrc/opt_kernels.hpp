@@ -1,0 +1,29 @@
+// Optimizer kernels (opt_kernels.hip): theta init, L-BFGS pair / direction,
+// FP64 fixed-order reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace litho {
+
+constexpr int kOptThreads = 256;
+constexpr int kOptBlocks = 512;          // grid-stride blocks per tile
+constexpr int kMaxHist = 12;             // largest supported L-BFGS history m
+constexpr int kPairDots = 4 * (kMaxHist + 1);
+constexpr int kMetaStride = 2 + kMaxHist;   // [m, new_slot, slots[kMaxHist]]
+constexpr int kCoefStride = 1 + 2 * kMaxHist;
+
+void launch_init_theta(const float* pattern, float* theta, size_t count, hipStream_t s);
+void launch_scale(const float* x, float* y, const float* coef, const int* active, size_t n, int batch,
+                  hipStream_t s);
+void launch_axpy(const float* x, float* y, const float* coef, const int* active, size_t n, int batch,
+                 hipStream_t s);
+void launch_mask_from_theta(const float* theta, float* mask, size_t count, hipStream_t s);
+void launch_lbfgs_pair(const float* x, const float* xp, const float* g, const float* gp, float* hs, float* hy,
+                       const int* meta, size_t n, int batch, double* partials, hipStream_t s);
+void launch_lbfgs_direction(const float* g, const float* hs, const float* hy, const int* meta, const double* coef,
+                            size_t n, float* d, int batch, double* partials, hipStream_t s);
+void launch_reduce_vec(const double* partials, int nblk, int nv, double* out, int batch, hipStream_t s);
+
+}  // namespace litho

@@ -1,0 +1,38 @@
+// Optical model (FP64 host code) and the one-time SOCS kernel precompute.
+//
+// Restates Grid (grid.cpp:4-48), Pupil without aberrations (pupil.cpp:14-46),
+// Source::compute_source_map (source.cpp:114-158),
+// LithoPrepare::_extract_source_points (litho_prepare.cpp:65-102) and the
+// compact SOCS window (litho_prepare.cpp:238-256) with the same IEEE operation
+// order, so the window and the source-point set match the reference exactly.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace litho {
+
+struct OpticsParams {
+  int n;
+  double pixel_nm, wavelength_nm, na, sigma_in, sigma_out;
+};
+
+struct SourceSet {
+  std::vector<double> f_norm;          // centred frequency axis / (NA / lambda), length n
+  std::vector<double> fs, gs, ws;      // source points (fx, fy, weight)
+  double weight_sum = 0.0;
+};
+
+struct Window {
+  int y0, x0, rows, cols;
+};
+
+SourceSet make_source_set(const OpticsParams& op);
+Window socs_window(const OpticsParams& op, const SourceSet& src);
+
+// Device precompute (rocBLAS zherk + rocSOLVER zheevdx); outputs as in
+// litho_socs_precompute (include/litho_amd.h).
+void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, double* kernels_out,
+                            double* weights_out, double* singular_values_out);
+
+}  // namespace litho

@@ -1,0 +1,150 @@
+// Plan construction and the batched evaluation sequence.
+#include "plan.hpp"
+
+#include <cmath>
+#include <complex>
+
+namespace litho {
+
+Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, const double* kernels,
+                const double* weights, double weight_sum, int capacity) {
+  // validation mirrors Imaging::compute (imaging.cpp:68-107)
+  if (n <= 0) throw Error(1, "Imaging: cache.N must be positive");
+  if (k <= 0) throw Error(1, "Imaging: no optical kernels available");
+  if (kernels == nullptr || weights == nullptr) throw Error(1, "plan: kernels and weights are required");
+  if (wr <= 0 || wc <= 0 || wr > n || wc > n || y0 < 0 || x0 < 0 || y0 + wr > n || x0 + wc > n)
+    throw Error(1, "plan: kernel window does not fit the N x N grid");
+  if (!std::isfinite(weight_sum) || weight_sum <= 0.0)
+    throw Error(1, "Imaging: source weights must be finite, nonnegative, and have a positive sum");
+  for (int i = 0; i < k; ++i)
+    if (!std::isfinite(weights[i]) || weights[i] < 0.0)
+      throw Error(1, "Imaging: SOCS values must be finite and nonnegative");
+  if (capacity <= 0) throw Error(1, "plan: batch capacity must be positive");
+  const SocsOps* ops = socs_ops_for(n);
+  if (ops == nullptr) throw Error(4, "plan: no compiled FFT for tile size " + std::to_string(n));
+
+  Plan* p = new Plan();
+  try {
+    p->ctx = ctx;
+    p->n = n; p->k = k; p->wr = wr; p->wc = wc; p->win_y0 = y0; p->win_x0 = x0;
+    p->capacity = capacity; p->weight_sum = weight_sum; p->ops = ops;
+    p->col_blocks = ops->col_blocks();
+    LITHO_HIP(hipSetDevice(ctx->device));
+
+    // scaled kernels H~_k = H_k sqrt(lambda_k / sum w) / N^2
+    std::vector<float2> hk((size_t)k * wr * wc);
+    const double inv_n2 = 1.0 / ((double)n * (double)n);
+    for (int kk = 0; kk < k; ++kk) {
+      const double c = std::sqrt(weights[kk] / weight_sum) * inv_n2;
+      for (size_t e = 0; e < (size_t)wr * wc; ++e) {
+        const size_t idx = (size_t)kk * wr * wc + e;
+        const double re = kernels[2 * idx], im = kernels[2 * idx + 1];
+        if (!std::isfinite(re) || !std::isfinite(im)) throw Error(1, "plan: kernels must be finite");
+        hk[idx] = make_float2((float)(re * c), (float)(im * c));
+      }
+    }
+    std::vector<float2> tw(n);
+    for (int m = 0; m < n; ++m) {
+      const double ang = -2.0 * 3.14159265358979323846 * (double)m / (double)n;
+      tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    // DC-corner frequency index of centred index c: (c - N/2) mod N
+    std::vector<int> rf(wr), cf(wc);
+    for (int i = 0; i < wr; ++i) rf[i] = (((y0 + i) - n / 2) % n + n) % n;
+    for (int j = 0; j < wc; ++j) cf[j] = (((x0 + j) - n / 2) % n + n) % n;
+
+    p->h.alloc(hk.size() * sizeof(float2));
+    p->tw.alloc(tw.size() * sizeof(float2));
+    p->row_freq.alloc(rf.size() * sizeof(int));
+    p->col_freq.alloc(cf.size() * sizeof(int));
+    LITHO_HIP(hipMemcpy(p->h.ptr, hk.data(), p->h.bytes, hipMemcpyHostToDevice));
+    LITHO_HIP(hipMemcpy(p->tw.ptr, tw.data(), p->tw.bytes, hipMemcpyHostToDevice));
+    LITHO_HIP(hipMemcpy(p->row_freq.ptr, rf.data(), p->row_freq.bytes, hipMemcpyHostToDevice));
+    LITHO_HIP(hipMemcpy(p->col_freq.ptr, cf.data(), p->col_freq.bytes, hipMemcpyHostToDevice));
+
+    const size_t c = (size_t)capacity, N = (size_t)n;
+    p->rt.alloc(c * wc * N * sizeof(float2));
+    p->mhat.alloc(c * wr * wc * sizeof(float2));
+    p->t.alloc(c * k * wr * N * sizeof(float2));
+    p->v.alloc(c * k * wr * N * sizeof(float2));
+    p->y.alloc(c * k * wr * wc * sizeof(float2));
+    p->p.alloc(c * wr * N * sizeof(float2));
+    p->img_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
+    p->grad_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
+    p->red.alloc(c * 4 * sizeof(double));
+    p->coef.alloc(c * sizeof(float));
+    p->host_red = new HostPinned(c * 4 * sizeof(double));
+    p->host_coef = new HostPinned(c * sizeof(float));
+    p->device_bytes = p->h.bytes + p->tw.bytes + p->row_freq.bytes + p->col_freq.bytes + p->rt.bytes +
+                      p->mhat.bytes + p->t.bytes + p->v.bytes + p->y.bytes + p->p.bytes + p->img_partials.bytes +
+                      p->grad_partials.bytes + p->red.bytes + p->coef.bytes;
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  return p;
+}
+
+void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
+  if (batch <= 0 || batch > plan.capacity) throw Error(1, "batch exceeds the plan capacity");
+  hipStream_t s = plan.ctx->stream;
+  SocsArgs a = plan.args();
+  a.active = rq.active;
+  plan.ops->forward_half(a, batch, rq.mode, rq.base, rq.dir, rq.coef, rq.theta_out, rq.mask_out, s);
+  ImageArgs im = rq.im;
+  im.partials = plan.img_partials.as<double>();
+  SocsArgs a2 = a;
+  if (!rq.adjoint) a2.v = nullptr;
+  plan.ops->image_adjoint(a2, im, batch, s);
+  reduce_pairs(plan.img_partials.as<double>(), plan.col_blocks, plan.red.as<double>(), 4, batch, s);
+  if (rq.adjoint && rq.im.target != nullptr) {
+    GradArgs ga = rq.ga;
+    ga.partials = plan.grad_partials.as<double>();
+    plan.ops->adjoint_tail(a, ga, batch, s);
+    reduce_pairs(plan.grad_partials.as<double>(), plan.col_blocks, plan.red.as<double>() + 2, 4, batch, s);
+  }
+  LITHO_HIP(hipGetLastError());
+}
+
+void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, double* stage_ms) {
+  if (repeats <= 0) throw Error(1, "profile: repeats must be positive");
+  hipStream_t s = plan.ctx->stream;
+  hipEvent_t ev[4];
+  for (auto& e : ev) LITHO_HIP(hipEventCreate(&e));
+  double acc[4] = {0, 0, 0, 0};
+  SocsArgs a = plan.args();
+  a.active = rq.active;
+  ImageArgs im = rq.im;
+  im.partials = plan.img_partials.as<double>();
+  GradArgs ga = rq.ga;
+  ga.partials = plan.grad_partials.as<double>();
+  for (int r = 0; r < repeats; ++r) {
+    LITHO_HIP(hipEventRecord(ev[0], s));
+    plan.ops->forward_half(a, batch, rq.mode, rq.base, rq.dir, rq.coef, rq.theta_out, rq.mask_out, s);
+    LITHO_HIP(hipEventRecord(ev[1], s));
+    plan.ops->image_adjoint(a, im, batch, s);
+    LITHO_HIP(hipEventRecord(ev[2], s));
+    plan.ops->adjoint_tail(a, ga, batch, s);
+    LITHO_HIP(hipEventRecord(ev[3], s));
+    LITHO_HIP(hipEventSynchronize(ev[3]));
+    float ms = 0.f;
+    for (int i = 0; i < 3; ++i) {
+      LITHO_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      acc[i] += ms;
+    }
+    LITHO_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
+    acc[3] += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int i = 0; i < 4; ++i) stage_ms[i] = acc[i] / repeats;
+}
+
+const double* fetch_reductions(Plan& plan, int batch) {
+  hipStream_t s = plan.ctx->stream;
+  LITHO_HIP(hipMemcpyAsync(plan.host_red->ptr, plan.red.ptr, (size_t)batch * 4 * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+  LITHO_HIP(hipStreamSynchronize(s));
+  return plan.host_red->as<double>();
+}
+
+}  // namespace litho

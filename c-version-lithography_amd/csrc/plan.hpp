@@ -1,0 +1,143 @@
+// Host-side context / plan objects behind the C ABI and the device evaluation
+// helper shared by the single-tile optimizers, the batch driver and the server.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "opt_kernels.hpp"
+#include "socs_kernels.hpp"
+
+namespace litho {
+
+// Exceptions carry the C status code the ABI returns.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define LITHO_HIP(call)                                                                         \
+  do {                                                                                          \
+    hipError_t _e = (call);                                                                     \
+    if (_e != hipSuccess)                                                                       \
+      throw ::litho::Error(3, std::string(#call) + ": " + hipGetErrorString(_e));               \
+  } while (0)
+
+// Owning device allocation.
+struct DevMem {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  explicit DevMem(size_t b) { alloc(b); }
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  DevMem(DevMem&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevMem& operator=(DevMem&& o) noexcept {
+    if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevMem() { release(); }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) return;
+    if (hipMalloc(&ptr, b) != hipSuccess) {
+      ptr = nullptr;
+      throw Error(2, "hipMalloc of " + std::to_string(b) + " bytes failed");
+    }
+    bytes = b;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Pinned host staging.
+struct HostPinned {
+  void* ptr = nullptr;
+  explicit HostPinned(size_t b) {
+    if (hipHostMalloc(&ptr, b, hipHostMallocDefault) != hipSuccess) throw Error(2, "hipHostMalloc failed");
+  }
+  HostPinned(const HostPinned&) = delete;
+  HostPinned& operator=(const HostPinned&) = delete;
+  ~HostPinned() { if (ptr) (void)hipHostFree(ptr); }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+// Device state of one tile size: scaled kernels, tables and the workspace of
+// up to `capacity` tiles.
+struct Plan {
+  Context* ctx = nullptr;
+  int n = 0, k = 0, wr = 0, wc = 0, win_y0 = 0, win_x0 = 0, capacity = 0;
+  double weight_sum = 0.0;
+  const SocsOps* ops = nullptr;
+  int col_blocks = 0;
+  DevMem h, tw, row_freq, col_freq;
+  DevMem rt, mhat, t, v, y, p;
+  DevMem img_partials, grad_partials, red;   // red: [capacity][4] = pe, bg, |g|^2, g.d
+  DevMem coef;                               // [capacity] float, K1 step coefficients
+  HostPinned* host_red = nullptr;            // [capacity][4]
+  HostPinned* host_coef = nullptr;           // [capacity] float
+  size_t device_bytes = 0;
+
+  ~Plan() {
+    delete host_red;
+    delete host_coef;
+  }
+  size_t tile_elems() const { return (size_t)n * n; }
+  SocsArgs args() const {
+    SocsArgs a;
+    a.k = k; a.wr = wr; a.wc = wc;
+    a.row_freq = row_freq.as<int>(); a.col_freq = col_freq.as<int>();
+    a.h = h.as<float2>(); a.tw = tw.as<float2>();
+    a.rt = rt.as<float2>(); a.mhat = mhat.as<float2>(); a.t = t.as<float2>();
+    a.v = v.as<float2>(); a.y = y.as<float2>(); a.p = p.as<float2>();
+    a.active = nullptr;
+    return a;
+  }
+};
+
+// Plan construction (validates like imaging.cpp:68-107).
+Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, const double* kernels,
+                const double* weights, double weight_sum, int capacity);
+
+// One batched evaluation.  K1 input: mode 0 = mask, 1 = theta (+ coef*dir);
+// need_adjoint adds the adjoint tail.  Reductions land in plan.red; the call
+// does not synchronise.
+struct EvalRequest {
+  int mode = 1;
+  const float* base = nullptr;
+  const float* dir = nullptr;
+  const float* coef = nullptr;       // device [batch]
+  float* theta_out = nullptr;
+  float* mask_out = nullptr;
+  ImageArgs im{};
+  GradArgs ga{};
+  bool adjoint = true;
+  const int* active = nullptr;       // device [batch] flags, null = all tiles
+};
+
+void run_eval(Plan& plan, int batch, const EvalRequest& rq);
+
+// Average HIP-event time (ms) of the three launch groups of one evaluation on
+// the plan's stream: [0] forward half (3 kernels), [1] fused column
+// image/adjoint kernel, [2] adjoint tail (3 kernels), [3] whole evaluation.
+void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, double* stage_ms);
+
+// Copy plan.red of `batch` tiles to the pinned host mirror and wait.
+const double* fetch_reductions(Plan& plan, int batch);
+
+}  // namespace litho

@@ -1,0 +1,511 @@
+// SOCS forward imaging, PE loss and adjoint gradient for one or a batch of
+// equal-size tiles, hand-written for gfx950.
+//
+// The reference (imaging.cpp:60-171, gradient.cpp:34-109) runs 3K+1 full N x N
+// FFTs per gradient evaluation and materialises K complex fields.  Here every
+// transform is pruned to the pupil's W_r x W_c frequency window (all SOCS
+// kernels vanish outside it, litho_prepare.cpp:238-256,330) and the fields are
+// never stored: they are rebuilt from the K row-transformed spectra T_k
+// (K x W_r x N) inside the fused column kernel.  The ifftshift / fftshift
+// pairs around the reference's transforms cancel exactly (circular
+// convolution commutes with the shift), so the device computes
+//   E_k  = IFFT(H_k . FFT(M))
+//   I    = sum_k |E~_k|^2,  E~_k = E_k sqrt(lambda_k / sum w)
+//   W    = 1 / (1 + exp(-alpha (I - thr)))
+//   dLdI = (pe_w (W - T) pe_win + pen [T < .5] max(W - tau, 0) pen_win) W (1 - W)
+//   G_M  = 4 alpha Re IFFT( sum_k conj(H~_k) . FFT(E~_k dLdI) )
+// with H~_k = H_k sqrt(lambda_k / sum w) / N^2 folded in by the host.
+//
+// Pipeline per evaluation (b = tile index = blockIdx.y):
+//   k_row_fwd        theta/mask rows -> row spectra at the W_c window columns
+//   k_col_fwd        window columns -> M^ (W_r x W_c)
+//   k_row_inv        (k, window row) -> T_k rows  (H~_k . M^, inverse along x)
+//   k_col_image_adj  columns: inverse along y for every k -> I, W, dLdI, PE;
+//                    then rebuild E~_k, multiply by dLdI, forward along y ->
+//                    V_k rows at the window rows
+//   k_row_adj        (k, window row): forward along x -> Y_k = conj(H~_k) V^
+//   k_final_row_inv  window rows: sum_k Y_k, inverse along x -> P
+//   k_final_col_inv  columns: inverse along y -> G_M, theta chain, norms
+//   k_reduce         fixed-order FP64 sums of the per-block partials
+#include <hip/hip_runtime.h>
+
+#include "fft_engine.hpp"
+#include "socs_kernels.hpp"
+
+namespace litho {
+
+using fft::cconj;
+using fft::cmul;
+using fft::cmulc;
+
+// ---------------------------------------------------------------- tunables
+template <int N>
+struct RowCfg {
+  static constexpr int LINES = N >= 1024 ? 4 : (N >= 256 ? 8 : 16);
+  static constexpr int NT = N >= 512 ? 256 : (N >= 128 ? 128 : 64);
+  static constexpr int LS = N;  // line stride (float2)
+};
+
+template <int N>
+struct ColCfg {
+  static constexpr int LINES = N >= 256 ? 8 : 16;
+  static constexpr int NT = N >= 1024 ? 512 : (N >= 256 ? 256 : 64);
+  static constexpr int LS = N + 1;                      // pad against bank conflicts
+  static constexpr int ELEMS = LINES * N;
+  static constexpr int PER = (ELEMS + NT - 1) / NT;     // owned elements per thread
+};
+
+// ---------------------------------------------------------------- K1
+// Rows of the mask -> W_c window columns of the row spectra, two real rows
+// packed per complex line.  theta = base + coef[b] * dir (dir optional);
+// mode 0: base is the mask itself; mode 1: base is theta, M = (1 + cos theta)/2.
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::NT)
+k_row_fwd(SocsArgs a, int mode, const float* base, const float* __restrict__ dir, const float* __restrict__ coef,
+          float* theta_out, float* __restrict__ mask_out) {
+  using C = RowCfg<N>;
+  constexpr int PAIRS = C::LINES;        // complex lines per block, 2 rows each
+  __shared__ float2 buf[PAIRS * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const size_t tile = (size_t)b * N * N;
+  const int row0 = blockIdx.x * 2 * PAIRS;
+  const float cf = (dir != nullptr) ? coef[b] : 0.f;
+  for (int e = threadIdx.x; e < PAIRS * N; e += C::NT) {
+    const int l = e / N, x = e - l * N;
+    float m2[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int y = row0 + 2 * l + h;
+      float m = 0.f;
+      if (y < N) {
+        const size_t g = tile + (size_t)y * N + x;
+        float v = base[g];
+        if (mode == 1) {
+          if (dir != nullptr) v = v + cf * dir[g];
+          if (theta_out != nullptr) theta_out[g] = v;
+          m = 0.5f * (1.0f + cosf(v));
+        } else {
+          m = v;
+        }
+        if (mask_out != nullptr) mask_out[g] = m;
+      }
+      m2[h] = m;
+    }
+    buf[l * C::LS + x] = make_float2(m2[0], m2[1]);
+  }
+  __syncthreads();
+  fft::fft_lines<N, PAIRS, C::LS, C::NT, -1>(buf, a.tw);
+  // unpack X_a[v] = (Z[v] + conj Z[-v]) / 2, X_b[v] = (Z[v] - conj Z[-v]) / (2i)
+  float2* rt = a.rt + (size_t)b * a.wc * N;
+  for (int e = threadIdx.x; e < PAIRS * a.wc; e += C::NT) {
+    const int l = e / a.wc, j = e - l * a.wc;
+    const int v = a.col_freq[j];
+    const int vn = v == 0 ? 0 : N - v;
+    const float2 z = buf[l * C::LS + v], zc = cconj(buf[l * C::LS + vn]);
+    const float2 xa = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+    const float2 xb = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+    const int y = row0 + 2 * l;
+    if (y < N) rt[(size_t)j * N + y] = xa;
+    if (y + 1 < N) rt[(size_t)j * N + y + 1] = xb;
+  }
+}
+
+// ---------------------------------------------------------------- K2
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::NT)
+k_col_fwd(SocsArgs a) {
+  using C = RowCfg<N>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const float2* rt = a.rt + (size_t)b * a.wc * N;
+  const int j0 = blockIdx.x * C::LINES;
+  for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
+    const int l = e / N, y = e - l * N;
+    const int j = j0 + l;
+    buf[l * C::LS + y] = j < a.wc ? rt[(size_t)j * N + y] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
+  float2* mh = a.mhat + (size_t)b * a.wr * a.wc;
+  for (int e = threadIdx.x; e < C::LINES * a.wr; e += C::NT) {
+    const int l = e / a.wr, i = e - l * a.wr;
+    const int j = j0 + l;
+    if (j < a.wc) mh[(size_t)i * a.wc + j] = buf[l * C::LS + a.row_freq[i]];
+  }
+}
+
+// ---------------------------------------------------------------- K3
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::NT)
+k_row_inv(SocsArgs a) {
+  using C = RowCfg<N>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int rows = a.k * a.wr;
+  const int q0 = blockIdx.x * C::LINES;
+  for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
+    const int l = e / N, x = e - l * N;
+    buf[l * C::LS + x] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const float2* mh = a.mhat + (size_t)b * a.wr * a.wc;
+  for (int e = threadIdx.x; e < C::LINES * a.wc; e += C::NT) {
+    const int l = e / a.wc, j = e - l * a.wc;
+    const int q = q0 + l;
+    if (q < rows) {
+      const int i = q % a.wr;
+      buf[l * C::LS + a.col_freq[j]] = cmul(a.h[(size_t)q * a.wc + j], mh[(size_t)i * a.wc + j]);
+    }
+  }
+  __syncthreads();
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+  float2* t = a.t + (size_t)b * rows * N;
+  for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
+    const int l = e / N, x = e - l * N;
+    const int q = q0 + l;
+    if (q < rows) t[(size_t)q * N + x] = buf[l * C::LS + x];
+  }
+}
+
+// ---------------------------------------------------------------- K4+K5
+// Column block [x0, x0 + LINES).  Element ownership: e = tid + s*NT,
+// column c = e % LINES, row y = e / LINES (coalesced LINES-wide segments).
+template <int N>
+__device__ __forceinline__ void load_window_columns(float2* buf, const float2* __restrict__ src, int wr,
+                                                    const int* __restrict__ row_freq, int x0) {
+  using C = ColCfg<N>;
+  for (int e = threadIdx.x; e < wr * C::LINES; e += C::NT) {
+    const int i = e / C::LINES, c = e - i * C::LINES;
+    const int x = x0 + c;
+    if (x < N) buf[c * C::LS + row_freq[i]] = src[(size_t)i * N + x];
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(ColCfg<N>::NT)
+k_col_image_adj(SocsArgs a, ImageArgs im) {
+  using C = ColCfg<N>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ double red[2 * C::NT / 64];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int x0 = blockIdx.x * C::LINES;
+  const size_t tile = (size_t)b * N * N;
+  const int rows = a.k * a.wr;
+  const float2* t = a.t + (size_t)b * rows * N;
+
+  float acc[C::PER];
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    acc[s] = 0.f;
+    const int e = threadIdx.x + s * C::NT;
+    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + e / C::LINES] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  // ---- pass 1: aerial image
+  for (int k = 0; k < a.k; ++k) {
+    load_window_columns<N>(buf, t + (size_t)k * a.wr * N, a.wr, a.row_freq, x0);
+    __syncthreads();
+    fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+#pragma unroll
+    for (int s = 0; s < C::PER; ++s) {
+      const int e = threadIdx.x + s * C::NT;
+      if (e < C::ELEMS) {
+        float2* p = buf + (e % C::LINES) * C::LS + e / C::LINES;
+        const float2 z = *p;
+        acc[s] = fmaf(z.x, z.x, fmaf(z.y, z.y, acc[s]));
+        *p = make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+  }
+  // ---- resist, loss, dLdI
+  float g[C::PER];
+  double pe = 0.0, bg = 0.0;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    g[s] = 0.f;
+    const int e = threadIdx.x + s * C::NT;
+    const int c = e % C::LINES, y = e / C::LINES;
+    const int x = x0 + c;
+    if (e < C::ELEMS && x < N) {
+      const size_t gi = tile + (size_t)y * N + x;
+      const float iv = acc[s];
+      const float w = 1.0f / (1.0f + expf(-im.alpha * (iv - im.thr)));
+      if (im.aerial != nullptr) im.aerial[gi] = iv;
+      if (im.wafer != nullptr) im.wafer[gi] = w;
+      if (im.target != nullptr) {
+        const float tv = im.target[gi];
+        const float r = w - tv;
+        const float pw = im.pe_win != nullptr ? im.pe_win[gi] : 1.0f;
+        float drive = im.pe_weight * r * pw;
+        pe += (double)r * (double)r * (double)pw;
+        if (im.pen_win != nullptr) {
+          const float bw = im.pen_win[gi];
+          const float ex = fmaxf(w - im.pen_thr, 0.f);
+          const float bgm = tv < 0.5f ? 1.f : 0.f;
+          bg += (double)bw * (double)bgm * (double)ex * (double)ex;
+          drive += im.penalty * bgm * ex * bw;
+        }
+        g[s] = drive * w * (1.0f - w);
+        if (im.dldi != nullptr) im.dldi[gi] = g[s];
+      }
+    }
+  }
+  // block reduction of the two FP64 partials (fixed order)
+  {
+    for (int off = 32; off > 0; off >>= 1) {
+      pe += __shfl_down(pe, off, 64);
+      bg += __shfl_down(bg, off, 64);
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+      red[2 * wave] = pe;
+      red[2 * wave + 1] = bg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && im.partials != nullptr) {
+      double sp = 0.0, sb = 0.0;
+      for (int w = 0; w < C::NT / 64; ++w) {
+        sp += red[2 * w];
+        sb += red[2 * w + 1];
+      }
+      double* out = im.partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
+      out[0] = sp;
+      out[1] = sb;
+    }
+  }
+  if (im.target == nullptr || a.v == nullptr) return;
+  // ---- pass 2: rebuild E~_k, multiply by dLdI, forward along y
+  float2* v = a.v + (size_t)b * rows * N;
+  for (int k = 0; k < a.k; ++k) {
+    load_window_columns<N>(buf, t + (size_t)k * a.wr * N, a.wr, a.row_freq, x0);
+    __syncthreads();
+    fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+#pragma unroll
+    for (int s = 0; s < C::PER; ++s) {
+      const int e = threadIdx.x + s * C::NT;
+      if (e < C::ELEMS) {
+        float2* p = buf + (e % C::LINES) * C::LS + e / C::LINES;
+        const float2 z = *p;
+        *p = make_float2(z.x * g[s], z.y * g[s]);
+      }
+    }
+    __syncthreads();
+    fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
+    float2* vk = v + (size_t)k * a.wr * N;
+    for (int e = threadIdx.x; e < a.wr * C::LINES; e += C::NT) {
+      const int i = e / C::LINES, c = e - i * C::LINES;
+      const int x = x0 + c;
+      if (x < N) vk[(size_t)i * N + x] = buf[c * C::LS + a.row_freq[i]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < C::PER; ++s) {
+      const int e = threadIdx.x + s * C::NT;
+      if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + e / C::LINES] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- K6
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::NT)
+k_row_adj(SocsArgs a) {
+  using C = RowCfg<N>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int rows = a.k * a.wr;
+  const int q0 = blockIdx.x * C::LINES;
+  const float2* v = a.v + (size_t)b * rows * N;
+  for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
+    const int l = e / N, x = e - l * N;
+    const int q = q0 + l;
+    buf[l * C::LS + x] = q < rows ? v[(size_t)q * N + x] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
+  float2* yk = a.y + (size_t)b * rows * a.wc;
+  for (int e = threadIdx.x; e < C::LINES * a.wc; e += C::NT) {
+    const int l = e / a.wc, j = e - l * a.wc;
+    const int q = q0 + l;
+    if (q < rows) yk[(size_t)q * a.wc + j] = cmulc(buf[l * C::LS + a.col_freq[j]], a.h[(size_t)q * a.wc + j]);
+  }
+}
+
+// ---------------------------------------------------------------- K7
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::NT)
+k_final_row_inv(SocsArgs a) {
+  using C = RowCfg<N>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int rows = a.k * a.wr;
+  const int i0 = blockIdx.x * C::LINES;
+  for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
+    const int l = e / N, x = e - l * N;
+    buf[l * C::LS + x] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const float2* yk = a.y + (size_t)b * rows * a.wc;
+  for (int e = threadIdx.x; e < C::LINES * a.wc; e += C::NT) {
+    const int l = e / a.wc, j = e - l * a.wc;
+    const int i = i0 + l;
+    if (i < a.wr) {
+      float2 s = make_float2(0.f, 0.f);
+      for (int k = 0; k < a.k; ++k) {
+        const float2 u = yk[((size_t)k * a.wr + i) * a.wc + j];
+        s.x += u.x;
+        s.y += u.y;
+      }
+      buf[l * C::LS + a.col_freq[j]] = s;
+    }
+  }
+  __syncthreads();
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+  float2* p = a.p + (size_t)b * a.wr * N;
+  for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
+    const int l = e / N, x = e - l * N;
+    const int i = i0 + l;
+    if (i < a.wr) p[(size_t)i * N + x] = buf[l * C::LS + x];
+  }
+}
+
+// ---------------------------------------------------------------- K8
+template <int N>
+__global__ void __launch_bounds__(ColCfg<N>::NT)
+k_final_col_inv(SocsArgs a, GradArgs ga) {
+  using C = ColCfg<N>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ double red[2 * C::NT / 64];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int x0 = blockIdx.x * C::LINES;
+  const size_t tile = (size_t)b * N * N;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + e / C::LINES] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  load_window_columns<N>(buf, a.p + (size_t)b * a.wr * N, a.wr, a.row_freq, x0);
+  __syncthreads();
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+  double nrm = 0.0, dot = 0.0;
+  const int* rect = ga.rect != nullptr ? ga.rect + 4 * b : nullptr;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int c = e % C::LINES, y = e / C::LINES;
+    const int x = x0 + c;
+    if (e < C::ELEMS && x < N) {
+      const size_t gi = tile + (size_t)y * N + x;
+      const float gm = ga.scale * buf[c * C::LS + y].x;
+      if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gm;
+      if (ga.theta != nullptr) {
+        float gt = -0.5f * gm * sinf(ga.theta[gi]);
+        if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
+        if (ga.grad_theta != nullptr) ga.grad_theta[gi] = gt;
+        nrm += (double)gt * (double)gt;
+        if (ga.dir != nullptr) dot += (double)gt * (double)ga.dir[gi];
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nrm += __shfl_down(nrm, off, 64);
+    dot += __shfl_down(dot, off, 64);
+  }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) {
+    red[2 * wave] = nrm;
+    red[2 * wave + 1] = dot;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && ga.partials != nullptr) {
+    double sn = 0.0, sd = 0.0;
+    for (int w = 0; w < C::NT / 64; ++w) {
+      sn += red[2 * w];
+      sd += red[2 * w + 1];
+    }
+    double* out = ga.partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
+    out[0] = sn;
+    out[1] = sd;
+  }
+}
+
+// ---------------------------------------------------------------- reduce
+// out[b*stride + j] = sum_blk partials[(b*nblk + blk)*2 + j], fixed order.
+__global__ void k_reduce_pairs(const double* __restrict__ partials, int nblk, double* __restrict__ out,
+                               int out_stride) {
+  const int b = blockIdx.x;
+  const int j = threadIdx.x;  // 0 or 1
+  if (j >= 2) return;
+  double s = 0.0;
+  for (int i = 0; i < nblk; ++i) s += partials[((size_t)b * nblk + i) * 2 + j];
+  out[(size_t)b * out_stride + j] = s;
+}
+
+// ---------------------------------------------------------------- launchers
+template <int N>
+struct SocsImpl {
+  static int col_blocks() { return (N + ColCfg<N>::LINES - 1) / ColCfg<N>::LINES; }
+
+  static void forward_half(const SocsArgs& a, int batch, int mode, const float* base, const float* dir,
+                           const float* coef, float* theta_out, float* mask_out, hipStream_t s) {
+    using R = RowCfg<N>;
+    dim3 g1((N + 2 * R::LINES - 1) / (2 * R::LINES), batch);
+    hipLaunchKernelGGL(k_row_fwd<N>, g1, dim3(R::NT), 0, s, a, mode, base, dir, coef, theta_out, mask_out);
+    dim3 g2((a.wc + R::LINES - 1) / R::LINES, batch);
+    hipLaunchKernelGGL(k_col_fwd<N>, g2, dim3(R::NT), 0, s, a);
+    dim3 g3((a.k * a.wr + R::LINES - 1) / R::LINES, batch);
+    hipLaunchKernelGGL(k_row_inv<N>, g3, dim3(R::NT), 0, s, a);
+  }
+
+  static void image_adjoint(const SocsArgs& a, const ImageArgs& im, int batch, hipStream_t s) {
+    using C = ColCfg<N>;
+    dim3 g(col_blocks(), batch);
+    hipLaunchKernelGGL(k_col_image_adj<N>, g, dim3(C::NT), 0, s, a, im);
+  }
+
+  static void adjoint_tail(const SocsArgs& a, const GradArgs& ga, int batch, hipStream_t s) {
+    using R = RowCfg<N>;
+    using C = ColCfg<N>;
+    dim3 g6((a.k * a.wr + R::LINES - 1) / R::LINES, batch);
+    hipLaunchKernelGGL(k_row_adj<N>, g6, dim3(R::NT), 0, s, a);
+    dim3 g7((a.wr + R::LINES - 1) / R::LINES, batch);
+    hipLaunchKernelGGL(k_final_row_inv<N>, g7, dim3(R::NT), 0, s, a);
+    dim3 g8(col_blocks(), batch);
+    hipLaunchKernelGGL(k_final_col_inv<N>, g8, dim3(C::NT), 0, s, a, ga);
+  }
+};
+
+void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_pairs, dim3(batch), dim3(64), 0, s, partials, nblk, out, out_stride);
+}
+
+template <int N>
+static const SocsOps* make_ops() {
+  static const SocsOps ops = {N, &SocsImpl<N>::col_blocks, &SocsImpl<N>::forward_half,
+                              &SocsImpl<N>::image_adjoint, &SocsImpl<N>::adjoint_tail};
+  return &ops;
+}
+
+const SocsOps* socs_ops_for(int n) {
+  switch (n) {
+#define LITHO_N(v) \
+  case v:          \
+    return make_ops<v>();
+    LITHO_SUPPORTED_SIZES(LITHO_N)
+#undef LITHO_N
+    default:
+      return nullptr;
+  }
+}
+
+}  // namespace litho

@@ -1,0 +1,65 @@
+// Kernel-side argument blocks and the per-size launcher table of the SOCS
+// forward / adjoint pipeline (socs_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace litho {
+
+// Tile sizes with compiled FFT instantiations: the CPU-test sizes, the Micro
+// tile sizes (core + 2*halo) and the single-tile sizes 512 / 2048.
+#define LITHO_SUPPORTED_SIZES(X)                                                            \
+  X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
+  X(320) X(384) X(512) X(640) X(768) X(896) X(1024) X(1152) X(1280) X(1536) X(2048) X(2176)
+
+// Plan-resident data shared by all kernels of one tile size (per-tile buffers
+// are indexed by blockIdx.y with the strides implied by the shapes).
+struct SocsArgs {
+  int k, wr, wc;                  // kernels, window rows, window columns
+  const int* row_freq;            // [wr] DC-corner frequency index of window row i
+  const int* col_freq;            // [wc]
+  const float2* h;                // [k][wr][wc] scaled kernels H~_k
+  const float2* tw;               // [N] twiddles e^{-2 pi i m / N}
+  float2* rt;                     // [b][wc][N]   row spectra at window columns
+  float2* mhat;                   // [b][wr][wc]  mask spectrum window
+  float2* t;                      // [b][k][wr][N] row-inverse spectra
+  float2* v;                      // [b][k][wr][N] adjoint column spectra
+  float2* y;                      // [b][k][wr][wc]
+  float2* p;                      // [b][wr][N]
+  const int* active;              // [b] nonzero = evaluate tile b; null = all tiles
+};
+
+struct ImageArgs {
+  float thr, alpha;
+  float* aerial;                  // [b][N][N] or null
+  float* wafer;                   // [b][N][N] or null
+  const float* target;            // null: forward only (no loss, no adjoint)
+  const float* pe_win;            // null = all ones
+  const float* pen_win;           // null = no background penalty
+  float pe_weight, penalty, pen_thr;
+  float* dldi;                    // optional debug output
+  double* partials;               // [b][col_blocks][2]: PE, background penalty
+};
+
+struct GradArgs {
+  float scale;                    // 4 alpha
+  float* grad_mask;               // dPE/dM, optional
+  const float* theta;             // optional: enables the theta chain
+  float* grad_theta;              // optional
+  const float* dir;               // optional: partial dot grad_theta . dir
+  const int* rect;                // optional [b][4] = y0,y1,x0,x1 update window
+  double* partials;               // [b][col_blocks][2]: |G_theta|^2, G_theta . dir
+};
+
+struct SocsOps {
+  int n;
+  int (*col_blocks)();
+  void (*forward_half)(const SocsArgs&, int batch, int mode, const float* base, const float* dir,
+                       const float* coef, float* theta_out, float* mask_out, hipStream_t);
+  void (*image_adjoint)(const SocsArgs&, const ImageArgs&, int batch, hipStream_t);
+  void (*adjoint_tail)(const SocsArgs&, const GradArgs&, int batch, hipStream_t);
+};
+
+const SocsOps* socs_ops_for(int n);
+void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s);
+
+}  // namespace litho

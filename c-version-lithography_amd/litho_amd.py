@@ -1,0 +1,233 @@
+"""ctypes binding of liblitho_amd.so (the C ABI declared in include/litho_amd.h).
+
+The library is the product; this module only declares its entry points and
+offers thin helpers that pass torch tensors' device pointers through.  It never
+falls back to anything: a missing library or a failing call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liblitho_amd.so"
+
+OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, ERUNTIME = 0, 1, 2, 3, 4, 5
+
+
+class LithoError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[litho status {code}] {msg}")
+        self.code = code
+
+
+class CtmOptions(C.Structure):
+    _fields_ = [("method", C.c_int), ("iterations", C.c_int), ("learning_rate", C.c_double),
+                ("lbfgs_history", C.c_int), ("gradient_tolerance", C.c_double),
+                ("threshold", C.c_double), ("alpha", C.c_double)]
+
+
+class CtmInfo(C.Structure):
+    _fields_ = [("evaluations", C.c_int), ("iterations", C.c_int), ("final_pe", C.c_double),
+                ("final_grad_norm", C.c_double), ("status", C.c_int)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); the declaration list of include/litho_amd.h
+SIGNATURES = {
+    "litho_last_error": (C.c_char_p, []),
+    "litho_supported_size": (_I, [_I]),
+    "litho_ctx_create": (_I, [_I, C.POINTER(_P)]),
+    "litho_ctx_destroy": (_I, [_P]),
+    "litho_ctx_synchronize": (_I, [_P]),
+    "litho_ctx_stream": (_P, [_P]),
+    "litho_socs_window": (_I, [_I, _D, _D, _D, _D, _D, _IP, _IP, _IP, _IP, _IP, _DP]),
+    "litho_socs_precompute": (_I, [_P, _I, _D, _D, _D, _D, _D, _I, _DP, _DP, _DP]),
+    "litho_plan_create": (_I, [_P, _I, _I, _I, _I, _I, _I, _DP, _DP, _D, _I, C.POINTER(_P)]),
+    "litho_plan_destroy": (_I, [_P]),
+    "litho_plan_device_bytes": (C.c_size_t, [_P]),
+    "litho_socs_forward": (_I, [_P, _I, _P, _P, _P, _D, _D]),
+    "litho_pe_gradient": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _D, _D, _P, _P, _P, _DP, _DP]),
+    "litho_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _P, _P, _DP, _DP]),
+    "litho_ctm_optimize": (_I, [_P, _P, _P, C.POINTER(CtmOptions), _P, _P, _DP, _I, C.POINTER(CtmInfo)]),
+    "litho_initialize_theta": (_I, [_P, _P, _P, C.c_size_t]),
+    "litho_profile_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _I, _DP]),
+}
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load the native library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != OK:
+        msg = load().litho_last_error().decode(errors="replace")
+        raise LithoError(code, msg)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+def _dp(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_DP)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = _P()
+        check(self.lib.litho_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self.lib.litho_ctx_destroy(self.h))
+            self.h = None
+
+    def synchronize(self):
+        check(self.lib.litho_ctx_synchronize(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def socs_precompute(self, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, k, singular_values=False):
+        """Device SOCS precompute; returns dict like oracle SocsKernels fields."""
+        win = socs_window(n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out)
+        kern = np.zeros((k, win["rows"], win["cols"], 2), dtype=np.float64)
+        w = np.zeros(k, dtype=np.float64)
+        sv = np.zeros(win["num_points"], dtype=np.float64) if singular_values else None
+        check(self.lib.litho_socs_precompute(self.h, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, k,
+                                             _dp(kern), _dp(w), _dp(sv)))
+        out = dict(win)
+        out.update(kernels=kern[..., 0] + 1j * kern[..., 1], weights=w, singular_values=sv)
+        return out
+
+
+def socs_window(n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out) -> dict:
+    lib = load()
+    r, c, y0, x0, npts = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    ws = C.c_double()
+    check(lib.litho_socs_window(n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, C.byref(r), C.byref(c),
+                                C.byref(y0), C.byref(x0), C.byref(npts), C.byref(ws)))
+    return dict(rows=r.value, cols=c.value, y0=y0.value, x0=x0.value, num_points=npts.value,
+                weight_sum=ws.value)
+
+
+class Plan:
+    """litho_plan: SOCS kernels + workspace for `capacity` tiles of size n."""
+
+    def __init__(self, ctx: Context, n: int, kernels: np.ndarray, weights, weight_sum: float,
+                 win_y0: int, win_x0: int, capacity: int = 1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        kern = np.ascontiguousarray(kernels, dtype=np.complex128)
+        if kern.ndim != 3:
+            raise ValueError("kernels must be (K, Wr, Wc)")
+        k, wr, wc = kern.shape
+        kd = np.ascontiguousarray(np.stack([kern.real, kern.imag], axis=-1))
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        h = _P()
+        check(self.lib.litho_plan_create(ctx.h, n, k, wr, wc, win_y0, win_x0, _dp(kd), _dp(w),
+                                         float(weight_sum), capacity, C.byref(h)))
+        self.h = h
+        self.n, self.k, self.capacity = n, k, capacity
+
+    @classmethod
+    def from_socs(cls, ctx, sk, capacity=1):
+        """From an oracle-style object with n, kernels, weights, weight_sum, win_y0, win_x0."""
+        return cls(ctx, sk.n, sk.kernels, sk.weights, sk.weight_sum, sk.win_y0, sk.win_x0, capacity)
+
+    def close(self):
+        if self.h:
+            check(self.lib.litho_plan_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self) -> int:
+        return int(self.lib.litho_plan_device_bytes(self.h))
+
+    def forward(self, mask, aerial=None, wafer=None, threshold=0.25, alpha=50.0, batch=1):
+        check(self.lib.litho_socs_forward(self.h, batch, ptr(mask), ptr(aerial), ptr(wafer), threshold, alpha))
+
+    def pe_gradient(self, mask, target, grad_mask=None, aerial=None, wafer=None, pe_window=None,
+                    penalty_window=None, pe_weight=1.0, penalty=0.0, penalty_threshold=0.01,
+                    threshold=0.25, alpha=50.0, batch=1):
+        pe = np.zeros(batch)
+        bg = np.zeros(batch)
+        check(self.lib.litho_pe_gradient(self.h, batch, ptr(mask), ptr(target), ptr(pe_window),
+                                         ptr(penalty_window), pe_weight, penalty, penalty_threshold,
+                                         threshold, alpha, ptr(aerial), ptr(wafer), ptr(grad_mask),
+                                         _dp(pe), _dp(bg)))
+        return pe, bg
+
+    def ctm_eval(self, theta, target, mask=None, grad_theta=None, threshold=0.25, alpha=50.0, batch=1):
+        pe = np.zeros(batch)
+        gn = np.zeros(batch)
+        check(self.lib.litho_ctm_eval(self.h, batch, ptr(theta), ptr(target), threshold, alpha, ptr(mask),
+                                      ptr(grad_theta), _dp(pe), _dp(gn)))
+        return pe, gn
+
+    def profile_eval(self, theta, target, threshold=0.25, alpha=50.0, repeats=10, batch=1):
+        """HIP-event stage times (ms): forward half, fused column kernel, adjoint tail, total."""
+        out = np.zeros(4)
+        check(self.lib.litho_profile_ctm_eval(self.h, batch, ptr(theta), ptr(target), threshold, alpha, repeats,
+                                              _dp(out)))
+        return out
+
+    def ctm_optimize(self, target, out_mask, theta0=None, out_theta=None, method="lbfgs", iterations=50,
+                     learning_rate=0.9, history=10, tol=0.0, threshold=0.25, alpha=50.0,
+                     history_capacity=4096):
+        o = CtmOptions(1 if method == "lbfgs" else 0, iterations, learning_rate, history, tol, threshold, alpha)
+        info = CtmInfo()
+        hist = np.zeros(history_capacity)
+        check(self.lib.litho_ctm_optimize(self.h, ptr(target), ptr(theta0), C.byref(o), ptr(out_mask),
+                                          ptr(out_theta), _dp(hist), history_capacity, C.byref(info)))
+        return dict(evaluations=info.evaluations, iterations=info.iterations, final_pe=info.final_pe,
+                    final_grad_norm=info.final_grad_norm, status=info.status,
+                    history=hist[: min(info.evaluations, history_capacity)].copy())
+
+
+def supported_size(n: int) -> bool:
+    return bool(load().litho_supported_size(n))
+
+
+def initialize_theta(ctx: Context, pattern, theta) -> None:
+    check(ctx.lib.litho_initialize_theta(ctx.h, ptr(pattern), ptr(theta), pattern.numel()))

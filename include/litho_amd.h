@@ -1,0 +1,156 @@
+/*
+ * litho_amd.h — C ABI of the MI355X-native inverse-lithography hot path.
+ *
+ * Drop-in replacement for the FFTW/OpenMP path of wyhcoder/C-version-Lithography
+ * ("Litho_cpp").  Every entry point names the reference interface it replaces.
+ * Plain C: opaque handles, plain pointers and sizes.  Unless stated otherwise,
+ * image buffers are DEVICE pointers to row-major N x N float32 arrays
+ * (row = y, column = x, element (y, x) at y*N + x); a batch of B tiles is B
+ * such arrays back to back.  Scalars that the reference keeps in FP64 (losses,
+ * norms, optimizer scalars) are double.
+ *
+ * Status codes: 0 on success, otherwise one of LITHO_E*; litho_last_error()
+ * returns the calling thread's message.  Argument checks mirror the
+ * std::invalid_argument checks of imaging.cpp:68-107, gradient.cpp:47-54 and
+ * CTM_Optimizer.cpp:33-43.
+ *
+ * Threading: one host thread per context at a time (the reference's "one caller
+ * per Imaging/Gradient object").  Plans are bound to the context that made them.
+ */
+#ifndef LITHO_AMD_H
+#define LITHO_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LITHO_OK 0
+#define LITHO_EINVAL 1        /* std::invalid_argument in the reference */
+#define LITHO_ENOMEM 2        /* allocation failure / budget exceeded */
+#define LITHO_EDEVICE 3       /* HIP runtime error */
+#define LITHO_EUNSUPPORTED 4  /* tile size without a compiled FFT */
+#define LITHO_ERUNTIME 5      /* std::runtime_error in the reference */
+
+typedef struct litho_ctx litho_ctx;
+typedef struct litho_plan litho_plan;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* litho_last_error(void);
+
+/* 1 if tile size n has a compiled device FFT, else 0. */
+int litho_supported_size(int n);
+
+/* Device context: one HIP device and one stream (default priority). */
+int litho_ctx_create(int device, litho_ctx** out);
+int litho_ctx_destroy(litho_ctx* ctx);
+int litho_ctx_synchronize(litho_ctx* ctx);
+/* The context's hipStream_t, for callers that order their own work on it. */
+void* litho_ctx_stream(litho_ctx* ctx);
+
+/* ---------------------------------------------------------------- optics
+ * Compact SOCS frequency window of the reference optics
+ * (Grid grid.cpp:4-48, Pupil pupil.cpp:14-46, Source source.cpp:114-158,
+ *  LithoPrepare::_extract_source_points litho_prepare.cpp:65-102 and the
+ *  compact window litho_prepare.cpp:238-256).  Host only, FP64. */
+int litho_socs_window(int n, double pixel_nm, double wavelength_nm, double na, double sigma_in,
+                      double sigma_out, int* win_rows, int* win_cols, int* win_y0, int* win_x0,
+                      int* num_source_points, double* weight_sum);
+
+/* One-time SOCS kernel precompute, LithoPrepare(grid, pupil, source, use_socs=true,
+ * compact_socs=true) with K as a parameter (litho_prepare.cpp:16-54,225-385).
+ * FP64 on the device: A (window pixels x source points) is built by a kernel,
+ * G = A^H A by rocBLAS, the top-K eigenpairs of G by rocSOLVER, U_k = A v_k / s_k.
+ * kernels_out: host, k*win_rows*win_cols complex128 (re, im interleaved),
+ * element (k, r, c) at centred frequency index (win_y0 + r, win_x0 + c).
+ * weights_out: host, k values lambda_k = s_k^2.  singular_values_out (nullable):
+ * host, num_source_points values, descending. */
+int litho_socs_precompute(litho_ctx* ctx, int n, double pixel_nm, double wavelength_nm, double na,
+                          double sigma_in, double sigma_out, int k, double* kernels_out,
+                          double* weights_out, double* singular_values_out);
+
+/* ---------------------------------------------------------------- plan
+ * Replaces ImagingCache (litho_prepare.h:18-47) plus the FFTW plans of Imaging
+ * (imaging.cpp:20-51) and Gradient (gradient.cpp:14-25).  kernels/weights are
+ * host arrays as produced by litho_socs_precompute (or any cache: Abbe kernels
+ * with weights = source weights).  weight_sum = sum of source weights.
+ * batch_capacity = maximum number of equal-size tiles per call. */
+int litho_plan_create(litho_ctx* ctx, int n, int k, int win_rows, int win_cols, int win_y0, int win_x0,
+                      const double* kernels, const double* weights, double weight_sum,
+                      int batch_capacity, litho_plan** out);
+int litho_plan_destroy(litho_plan* plan);
+/* Device bytes held by the plan (kernels, tables and workspace). */
+size_t litho_plan_device_bytes(const litho_plan* plan);
+
+/* Imaging::compute (imaging.cpp:60-171): aerial = sum_k w_k |E_k|^2 / sum w,
+ * wafer = sigmoid(alpha (aerial - threshold)).  aerial / wafer nullable. */
+int litho_socs_forward(litho_plan* plan, int batch, const float* mask, float* aerial, float* wafer,
+                       double threshold, double alpha);
+
+/* Imaging::compute followed by Gradient::pe_gradient / pe_gradient_with_penalty
+ * (gradient.cpp:121-196) and Loss::pe_loss / background_penalty_loss
+ * (loss.cpp:16-64):
+ *   dLdI = (pe_weight (W-T) pe_window + penalty [T<0.5] max(W-penalty_threshold,0)
+ *           penalty_window) W (1-W),  grad_mask = 4 alpha sum_k ... (adjoint).
+ * pe_window nullable (= ones); penalty_window nullable (= no penalty term).
+ * pe_loss[b] = sum (W-T)^2 pe_window, bg_loss[b] = sum penalty_window [T<.5]
+ * max(W-thr,0)^2 — host arrays of length batch (FP64).  aerial, wafer, grad_mask,
+ * pe_loss, bg_loss nullable. */
+int litho_pe_gradient(litho_plan* plan, int batch, const float* mask, const float* target,
+                      const float* pe_window, const float* penalty_window, double pe_weight,
+                      double penalty, double penalty_threshold, double threshold, double alpha,
+                      float* aerial, float* wafer, float* grad_mask, double* pe_loss, double* bg_loss);
+
+/* One CTM grad-eval, CTM_Optimizer::_evaluate_theta_objective
+ * (CTM_Optimizer.cpp:175-187): mask = (1+cos theta)/2, PE and
+ * grad_theta = -0.5 grad_mask sin(theta).  pe / grad_norm2 host arrays of length
+ * batch (nullable); mask, grad_theta device (nullable). */
+int litho_ctm_eval(litho_plan* plan, int batch, const float* theta, const float* target, double threshold,
+                   double alpha, float* mask, float* grad_theta, double* pe, double* grad_norm2);
+
+/* CTM_Optimizer::optimize (CTM_Optimizer.cpp:104-227) with the optimizer state on
+ * the device.  method: 0 = gradient_descent (fixed step, grad-norm stop, returns
+ * the best-PE mask), 1 = lbfgs (LBFGS++-style: two-loop-equivalent compact
+ * update, strong-Wolfe line search; returns the final mask). */
+typedef struct {
+  int method;
+  int iterations;
+  double learning_rate;
+  int lbfgs_history;
+  double gradient_tolerance;
+  double threshold;
+  double alpha;
+} litho_ctm_options;
+
+typedef struct {
+  int evaluations;       /* objective evaluations incl. line-search trials */
+  int iterations;        /* optimizer iterations completed */
+  double final_pe;       /* L-BFGS: PE at the returned point; GD: best PE */
+  double final_grad_norm;
+  int status;            /* 0 converged (norm <= tol), 1 max iterations, 2 line search failed */
+} litho_ctm_info;
+
+/* target, theta0 (nullable: initialize_theta(target)), out_mask, out_theta
+ * (nullable) are device N x N.  history (host, nullable) receives one PE per
+ * evaluation, at most history_capacity. */
+int litho_ctm_optimize(litho_plan* plan, const float* target, const float* theta0,
+                       const litho_ctm_options* options, float* out_mask, float* out_theta,
+                       double* history, int history_capacity, litho_ctm_info* info);
+
+/* Timing of one CTM grad-eval by HIP events on the context stream (no reference
+ * counterpart; the reference times with std::chrono, demo_CTM.cpp:72-98).
+ * stage_ms[4] = mean over `repeats` of: forward half (3 kernels), fused column
+ * image/adjoint kernel, adjoint tail (3 kernels), whole evaluation. */
+int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, const float* target, double threshold,
+                           double alpha, int repeats, double* stage_ms);
+
+/* CTM_Optimizer::initialize_theta (CTM_Optimizer.cpp:66-73): 0.3 pi where
+ * pattern > 0, else 0.7 pi.  Device arrays of n*n floats. */
+int litho_initialize_theta(litho_ctx* ctx, const float* pattern, float* theta, size_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LITHO_AMD_H */

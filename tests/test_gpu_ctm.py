@@ -1,0 +1,109 @@
+"""GPU CTM optimizers (device-resident gradient descent and L-BFGS) against the
+FP64 oracle and the reference's optimizer-control invariants
+(tests/test_ctm_optimizer.cpp:181-196).
+
+Tolerances: GD PE history relative <= 1e-4 per iteration for 10 iterations;
+L-BFGS final PE within 0.5 % of the oracle's (the line-search path itself is
+not pinned: LBFGS++ is not vendored).
+"""
+import numpy as np
+import pytest
+
+from conftest import litho_amd, targets
+from oracle import ctm as oc
+from oracle import imaging as im
+from oracle import optics as op
+from test_gpu_socs import OPT, dev, empty, host, socs
+
+pytestmark = pytest.mark.gpu
+
+
+def case(n, k):
+    s = socs(n, k)
+    cache = im.Cache.from_socs(s)
+    tgt = targets().manhattan_target(n, 3)
+    return s, cache, tgt
+
+
+def test_gd_trajectory(gpu_ctx):
+    print("case gd n=128 k=24 10 iterations", flush=True)
+    la = litho_amd()
+    n = 128
+    s, cache, tgt = case(n, 24)
+    plan = la.Plan.from_socs(gpu_ctx, s)
+    out, th = empty(n), empty(n)
+    r = plan.ctm_optimize(dev(tgt), out[0], out_theta=th[0], method="gradient_descent", iterations=10,
+                          learning_rate=0.9, tol=0.0, threshold=0.225, alpha=50.0)
+    best, hist, theta = oc.gradient_descent(cache, tgt.astype(np.float64), 0.225, 50.0, 10, 0.9, 0.0,
+                                            theta0=oc.initialize_theta(tgt).astype(np.float32).astype(np.float64))
+    assert r["evaluations"] == 10
+    h = r["history"]
+    assert np.max(np.abs(h - np.array(hist)) / np.array(hist)) <= 1e-4
+    assert np.argmin(h) == int(np.argmin(hist))
+    assert np.max(np.abs(host(out)[0] - best)) <= 1e-3
+    plan.close()
+
+
+def test_optimizer_control_invariants(gpu_ctx):
+    # tol = 1e100 stops both methods after exactly one evaluation with the same initial mask;
+    # a 1-iteration L-BFGS run evaluates >= 2 points and does not increase PE
+    print("case invariants n=64", flush=True)
+    la = litho_amd()
+    n = 64
+    s, cache, _ = case(n, 24)
+    tgt = np.zeros((n, n), np.float32)
+    tgt[23:42, 23:42] = 1.0
+    plan = la.Plan.from_socs(gpu_ctx, s)
+    m1, m2, m3 = empty(n), empty(n), empty(n)
+    gd = plan.ctm_optimize(dev(tgt), m1[0], method="gradient_descent", iterations=3, tol=1e100,
+                           threshold=0.25, alpha=85.0)
+    lb = plan.ctm_optimize(dev(tgt), m2[0], method="lbfgs", iterations=3, tol=1e100, threshold=0.25, alpha=85.0)
+    assert gd["evaluations"] == 1 and lb["evaluations"] == 1
+    assert np.max(np.abs(host(m1) - host(m2))) <= 1e-12
+    st = plan.ctm_optimize(dev(tgt), m3[0], method="lbfgs", iterations=1, tol=0.0, threshold=0.25, alpha=85.0)
+    assert st["evaluations"] >= 2
+    assert st["history"][-1] <= st["history"][0]
+    assert np.all(np.isfinite(host(m3)))
+    plan.close()
+
+
+@pytest.mark.parametrize("n,iters", [(128, 15), (512, 10)])
+def test_lbfgs_final_pe(gpu_ctx, n, iters):
+    print(f"case lbfgs n={n} iterations={iters}", flush=True)
+    la = litho_amd()
+    s, cache, tgt = case(n, 24)
+    plan = la.Plan.from_socs(gpu_ctx, s)
+    out = empty(n)
+    r = plan.ctm_optimize(dev(tgt), out[0], method="lbfgs", iterations=iters, history=10, tol=1e-2,
+                          threshold=0.225, alpha=50.0)
+    mask, hist, theta, info = oc.lbfgs(cache, tgt.astype(np.float64), 0.225, 50.0, iters, 10, 1e-2)
+    assert abs(r["final_pe"] - info["final_pe"]) <= 5e-3 * info["final_pe"]
+    assert r["history"][0] == pytest.approx(hist[0], rel=1e-5)
+    # the returned mask is the mask of the returned point
+    _, w_gpu, _ = im.forward(cache, host(out)[0], 0.225, 50.0)
+    assert abs(im.pe_loss(w_gpu, tgt) - r["final_pe"]) <= 1e-4 * r["final_pe"]
+    # binary mismatch of the thresholded masks (M >= 0.5), counted exactly and reported
+    mism = int(np.sum((host(out)[0] >= 0.5) != (mask >= 0.5)))
+    near = int(np.sum(np.abs(mask - 0.5) < 1e-4))
+    print(f"lbfgs n={n}: gpu PE {r['final_pe']:.6f} oracle PE {info['final_pe']:.6f} evals "
+          f"{r['evaluations']}/{len(hist)} binary mismatches {mism} (oracle pixels within 1e-4 of 0.5: {near})")
+    plan.close()
+
+
+@pytest.mark.parametrize("n,k", [(64, 8), (512, 24)])
+def test_precompute_matches_oracle(gpu_ctx, n, k):
+    print(f"case precompute n={n} k={k}", flush=True)
+    la = litho_amd()
+    dk = gpu_ctx.socs_precompute(n, 4.0, 193.0, 1.35, 0.6, 0.9, k, singular_values=True)
+    ref = socs(n, k)
+    assert (dk["rows"], dk["cols"], dk["y0"], dk["x0"]) == (ref.win_rows, ref.win_cols, ref.win_y0, ref.win_x0)
+    assert abs(dk["weight_sum"] - ref.weight_sum) <= 1e-12 * ref.weight_sum
+    assert np.allclose(dk["weights"], ref.weights, rtol=1e-9, atol=0)
+    assert np.allclose(dk["singular_values"], ref.singular_values, rtol=1e-6, atol=1e-7 * ref.singular_values[0])
+    # kernels agree up to phase: observables are identical
+    mine = op.SocsKernels(n=n, win_y0=dk["y0"], win_x0=dk["x0"], kernels=dk["kernels"], weights=dk["weights"],
+                          weight_sum=dk["weight_sum"])
+    tgt = targets().manhattan_target(n, 5).astype(np.float64)
+    a1, _, _ = im.forward(im.Cache.from_socs(mine), tgt, 0.225, 50.0)
+    a2, _, _ = im.forward(im.Cache.from_socs(ref), tgt, 0.225, 50.0)
+    assert np.max(np.abs(a1 - a2)) <= 1e-10 * np.max(np.abs(a2))
