@@ -4,7 +4,7 @@
 // The reference forms A = [sqrt(w_i) P(f + f_i)] over the compact window
 // (W^2 rows x Ns columns) and takes a thin BDCSVD.  Here A is built by a
 // kernel, G = A^H A (Ns x Ns Hermitian) by rocBLAS zherk, the K largest
-// eigenpairs (sigma_k^2, v_k) of G by rocSOLVER zheevdx, and the left singular
+// eigenpairs (sigma_k^2, v_k) of G by rocSOLVER zheevx, and the left singular
 // vectors U_k = A v_k / sigma_k by zgemm.  lambda_k = sigma_k^2 as in the
 // reference (litho_prepare.cpp:328).  Kernel phases are arbitrary, as they are
 // for an SVD; observables (the aerial image) do not depend on them.
@@ -13,6 +13,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <string>
 #include <cmath>
 #include <vector>
 
@@ -121,37 +122,62 @@ void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, d
   rb(rocblas_zherk(h, rocblas_fill_lower, rocblas_operation_conjugate_transpose, ns, (int)lda, &one,
                    d_a.as<rocblas_double_complex>(), (int)lda, &zero, d_g.as<rocblas_double_complex>(), ns),
      "rocblas_zherk");
-  // all eigenvalues (optional) on a copy, then the top-K eigenpairs
-  if (singular_values_out != nullptr) {
-    DevMem d_g2(d_g.bytes), d_w2((size_t)ns * sizeof(double)), d_e2((size_t)ns * sizeof(double)),
-        d_info2(sizeof(rocblas_int));
-    LITHO_HIP(hipMemcpyAsync(d_g2.ptr, d_g.ptr, d_g.bytes, hipMemcpyDeviceToDevice, stream));
-    rb(rocsolver_zheevd(h, rocblas_evect_none, rocblas_fill_lower, ns, d_g2.as<rocblas_double_complex>(), ns,
-                        d_w2.as<double>(), d_e2.as<double>(), d_info2.as<rocblas_int>()),
-       "rocsolver_zheevd");
-    std::vector<double> ev(ns);
-    LITHO_HIP(hipMemcpyAsync(ev.data(), d_w2.ptr, ns * sizeof(double), hipMemcpyDeviceToHost, stream));
+  // Top-K eigenpairs of G by rocSOLVER zheevx: Householder tridiagonalisation,
+  // bisection for the K selected eigenvalues (abstol 0 = most accurate),
+  // inverse iteration with cluster re-orthogonalisation for their vectors.  Z is
+  // sized for all n columns.  (The divide-and-conquer zheevd returned
+  // non-finite eigenvectors for n >= 228 on ROCm 7.2, so it is not used.)
+  DevMem d_w((size_t)ns * sizeof(double)), d_e((size_t)ns * sizeof(double)), d_info(sizeof(rocblas_int));
+  std::vector<double> ev(k), all;
+  auto read_info = [&](const char* what) {
+    rocblas_int info = 0;
+    LITHO_HIP(hipMemcpyAsync(&info, d_info.ptr, sizeof(info), hipMemcpyDeviceToHost, stream));
     LITHO_HIP(hipStreamSynchronize(stream));
-    for (int i = 0; i < ns; ++i) singular_values_out[i] = std::sqrt(std::max(ev[ns - 1 - i], 0.0));
+    if (info != 0) throw Error(5, std::string(what) + " did not converge (info " + std::to_string(info) + ")");
+  };
+  if (singular_values_out != nullptr) {
+    // all eigenvalues, no vectors, on a copy
+    DevMem d_g2(d_g.bytes);
+    LITHO_HIP(hipMemcpyAsync(d_g2.ptr, d_g.ptr, d_g.bytes, hipMemcpyDeviceToDevice, stream));
+    rb(rocsolver_zheev(h, rocblas_evect_none, rocblas_fill_lower, ns, d_g2.as<rocblas_double_complex>(), ns,
+                       d_w.as<double>(), d_e.as<double>(), d_info.as<rocblas_int>()),
+       "rocsolver_zheev");
+    read_info("rocsolver_zheev");
+    all.resize(ns);
+    LITHO_HIP(hipMemcpyAsync(all.data(), d_w.ptr, ns * sizeof(double), hipMemcpyDeviceToHost, stream));
+    LITHO_HIP(hipStreamSynchronize(stream));
   }
-  DevMem d_w((size_t)ns * sizeof(double)), d_z((size_t)ns * k * sizeof(double2)), d_nev(sizeof(rocblas_int)),
-      d_info(sizeof(rocblas_int));
-  rb(rocsolver_zheevdx(h, rocblas_evect_original, rocblas_erange_index, rocblas_fill_lower, ns,
-                       d_g.as<rocblas_double_complex>(), ns, 0.0, 0.0, ns - k + 1, ns, d_nev.as<rocblas_int>(),
-                       d_w.as<double>(), d_z.as<rocblas_double_complex>(), ns, d_info.as<rocblas_int>()),
-     "rocsolver_zheevdx");
-  rocblas_int nev = 0, info = 0;
-  std::vector<double> ev(k);
-  LITHO_HIP(hipMemcpyAsync(&nev, d_nev.ptr, sizeof(nev), hipMemcpyDeviceToHost, stream));
-  LITHO_HIP(hipMemcpyAsync(&info, d_info.ptr, sizeof(info), hipMemcpyDeviceToHost, stream));
-  LITHO_HIP(hipMemcpyAsync(ev.data(), d_w.ptr, k * sizeof(double), hipMemcpyDeviceToHost, stream));
-  LITHO_HIP(hipStreamSynchronize(stream));
-  if (info != 0 || nev != k) throw Error(5, "rocsolver_zheevdx did not converge (info " + std::to_string(info) + ")");
+  DevMem d_zx((size_t)ns * ns * sizeof(double2));
+  {
+    DevMem d_nev(sizeof(rocblas_int)), d_ifail((size_t)ns * sizeof(rocblas_int));
+    rb(rocsolver_zheevx(h, rocblas_evect_original, rocblas_erange_index, rocblas_fill_lower, ns,
+                        d_g.as<rocblas_double_complex>(), ns, 0.0, 0.0, ns - k + 1, ns, 0.0, d_nev.as<rocblas_int>(),
+                        d_w.as<double>(), d_zx.as<rocblas_double_complex>(), ns, d_ifail.as<rocblas_int>(),
+                        d_info.as<rocblas_int>()),
+       "rocsolver_zheevx");
+    read_info("rocsolver_zheevx");
+    rocblas_int nev = 0;
+    LITHO_HIP(hipMemcpyAsync(&nev, d_nev.ptr, sizeof(nev), hipMemcpyDeviceToHost, stream));
+    LITHO_HIP(hipMemcpyAsync(ev.data(), d_w.ptr, k * sizeof(double), hipMemcpyDeviceToHost, stream));
+    LITHO_HIP(hipStreamSynchronize(stream));
+    if (nev != k) throw Error(5, "rocsolver_zheevx returned " + std::to_string(nev) + " eigenpairs");
+  }
+  const rocblas_double_complex* d_z = d_zx.as<rocblas_double_complex>();
+  if (singular_values_out != nullptr)
+    for (int i = 0; i < ns; ++i) singular_values_out[i] = std::sqrt(std::max(all[ns - 1 - i], 0.0));
+  {
+    std::vector<double2> z((size_t)ns * k);
+    LITHO_HIP(hipMemcpyAsync(z.data(), d_z, z.size() * sizeof(double2), hipMemcpyDeviceToHost, stream));
+    LITHO_HIP(hipStreamSynchronize(stream));
+    size_t bad = 0;
+    for (const double2& v : z) bad += !(std::isfinite(v.x) && std::isfinite(v.y));
+    if (bad) throw Error(5, "SOCS precompute: " + std::to_string(bad) + " non-finite eigenvector entries from the eigensolver");
+  }
   // U = A Z, columns scaled by 1/sigma; eigenvalues come back ascending
   DevMem d_u(lda * k * sizeof(double2));
   const rocblas_double_complex c1(1.0, 0.0), c0(0.0, 0.0);
   rb(rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (int)lda, k, ns, &c1,
-                   d_a.as<rocblas_double_complex>(), (int)lda, d_z.as<rocblas_double_complex>(), ns, &c0,
+                   d_a.as<rocblas_double_complex>(), (int)lda, d_z, ns, &c0,
                    d_u.as<rocblas_double_complex>(), (int)lda),
      "rocblas_zgemm");
   std::vector<double> inv(k);
@@ -167,6 +193,11 @@ void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, d
   std::vector<double2> u(lda * k);
   LITHO_HIP(hipMemcpyAsync(u.data(), d_u.ptr, u.size() * sizeof(double2), hipMemcpyDeviceToHost, stream));
   LITHO_HIP(hipStreamSynchronize(stream));
+  {
+    size_t bad = 0;
+    for (const double2& v : u) bad += !(std::isfinite(v.x) && std::isfinite(v.y));
+    if (bad) throw Error(5, "SOCS precompute: " + std::to_string(bad) + " non-finite kernel entries after A v / s");
+  }
   // descending order: output kernel j = eigen column k-1-j
   for (int j = 0; j < k; ++j) {
     const int col = k - 1 - j;
