@@ -3,6 +3,8 @@
 
 #include <cmath>
 #include <complex>
+#include <cstdlib>
+#include <string>
 
 namespace litho {
 
@@ -52,6 +54,21 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     std::vector<int> rf(wr), cf(wc);
     for (int i = 0; i < wr; ++i) rf[i] = (((y0 + i) - n / 2) % n + n) % n;
     for (int j = 0; j < wc; ++j) cf[j] = (((x0 + j) - n / 2) % n + n) % n;
+
+    // frequency -> window row table and the pruning precondition: every window
+    // frequency lies in the first or the last 1/R_1 of the axis
+    std::vector<int> rou(n, -1);
+    for (int i = 0; i < wr; ++i) rou[rf[i]] = i;
+    const int r1 = first_radix(n);
+    bool prunable = r1 > 0;
+    if (prunable) {
+      const int t1 = n / r1;
+      for (int v : rf) prunable = prunable && (v < t1 || v >= n - t1);
+    }
+    const char* path = std::getenv("LITHO_COLUMN_PATH");
+    p->pruned = prunable && !(path != nullptr && std::string(path) == "generic") ? 1 : 0;
+    p->row_of_u.alloc(rou.size() * sizeof(int));
+    LITHO_HIP(hipMemcpy(p->row_of_u.ptr, rou.data(), p->row_of_u.bytes, hipMemcpyHostToDevice));
 
     p->h.alloc(hk.size() * sizeof(float2));
     p->tw.alloc(tw.size() * sizeof(float2));

@@ -85,7 +85,8 @@ struct Plan {
   double weight_sum = 0.0;
   const SocsOps* ops = nullptr;
   int col_blocks = 0;
-  DevMem h, tw, row_freq, col_freq;
+  DevMem h, tw, row_freq, col_freq, row_of_u;
+  int pruned = 0;                            // window-pruned column kernels in use
   DevMem rt, mhat, t, v, y, p;
   DevMem img_partials, grad_partials, red;   // red: [capacity][4] = pe, bg, |g|^2, g.d
   DevMem coef;                               // [capacity] float, K1 step coefficients
@@ -106,6 +107,8 @@ struct Plan {
     a.rt = rt.as<float2>(); a.mhat = mhat.as<float2>(); a.t = t.as<float2>();
     a.v = v.as<float2>(); a.y = y.as<float2>(); a.p = p.as<float2>();
     a.active = nullptr;
+    a.row_of_u = row_of_u.as<int>();
+    a.pruned = pruned;
     return a;
   }
 };

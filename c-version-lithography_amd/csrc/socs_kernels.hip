@@ -29,6 +29,7 @@
 //   k_reduce         fixed-order FP64 sums of the per-block partials
 #include <hip/hip_runtime.h>
 
+#include "fft_columns.hpp"
 #include "fft_engine.hpp"
 #include "socs_kernels.hpp"
 
@@ -439,6 +440,171 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
   }
 }
 
+// ---------------------------------------------------------------- K4+K5 pruned
+// Same contract as k_col_image_adj, with the window-pruned column transforms of
+// fft_columns.hpp.  Element ownership follows the last inverse stage:
+// column q % LINES, row i + r*TM with q = tid + s*NT, i = q / LINES.
+template <int N>
+__global__ void __launch_bounds__(ColCfg<N>::NT)
+k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
+  using C = ColCfg<N>;
+  using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ double red[2 * C::NT / 64];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int x0 = blockIdx.x * C::LINES;
+  const size_t tile = (size_t)b * N * N;
+  const int rows = a.k * a.wr;
+  const float2* t = a.t + (size_t)b * rows * N;
+  float acc[CF::PERM][CF::RM];
+  float2 v[CF::PERM][CF::RM];
+#pragma unroll
+  for (int s = 0; s < CF::PERM; ++s)
+#pragma unroll
+    for (int r = 0; r < CF::RM; ++r) acc[s][r] = 0.f;
+  // ---- pass 1: aerial image
+  for (int k = 0; k < a.k; ++k) {
+    CF::inverse_to_regs(buf, t + (size_t)k * a.wr * N, a.row_of_u, x0, a.tw, v);
+#pragma unroll
+    for (int s = 0; s < CF::PERM; ++s)
+#pragma unroll
+      for (int r = 0; r < CF::RM; ++r) acc[s][r] = fmaf(v[s][r].x, v[s][r].x, fmaf(v[s][r].y, v[s][r].y, acc[s][r]));
+    __syncthreads();
+  }
+  // ---- resist, loss, dLdI
+  float g[CF::PERM][CF::RM];
+  double pe = 0.0, bg = 0.0;
+#pragma unroll
+  for (int s = 0; s < CF::PERM; ++s) {
+    const int q = threadIdx.x + s * C::NT;
+    const int line = q % C::LINES, i = q / C::LINES;
+    const int x = x0 + line;
+    const bool ok = (CF::BM % C::NT == 0 || q < CF::BM) && x < N;
+#pragma unroll
+    for (int r = 0; r < CF::RM; ++r) {
+      g[s][r] = 0.f;
+      if (!ok) continue;
+      const int y = i + r * CF::TM;
+      const size_t gi = tile + (size_t)y * N + x;
+      const float iv = acc[s][r];
+      const float w = 1.0f / (1.0f + expf(-im.alpha * (iv - im.thr)));
+      if (im.aerial != nullptr) im.aerial[gi] = iv;
+      if (im.wafer != nullptr) im.wafer[gi] = w;
+      if (im.target != nullptr) {
+        const float tv = im.target[gi];
+        const float rr = w - tv;
+        const float pw = im.pe_win != nullptr ? im.pe_win[gi] : 1.0f;
+        float drive = im.pe_weight * rr * pw;
+        pe += (double)rr * (double)rr * (double)pw;
+        if (im.pen_win != nullptr) {
+          const float bw = im.pen_win[gi];
+          const float ex = fmaxf(w - im.pen_thr, 0.f);
+          const float bgm = tv < 0.5f ? 1.f : 0.f;
+          bg += (double)bw * (double)bgm * (double)ex * (double)ex;
+          drive += im.penalty * bgm * ex * bw;
+        }
+        g[s][r] = drive * w * (1.0f - w);
+        if (im.dldi != nullptr) im.dldi[gi] = g[s][r];
+      }
+    }
+  }
+  {
+    for (int off = 32; off > 0; off >>= 1) {
+      pe += __shfl_down(pe, off, 64);
+      bg += __shfl_down(bg, off, 64);
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+      red[2 * wave] = pe;
+      red[2 * wave + 1] = bg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && im.partials != nullptr) {
+      double sp = 0.0, sb = 0.0;
+      for (int w = 0; w < C::NT / 64; ++w) {
+        sp += red[2 * w];
+        sb += red[2 * w + 1];
+      }
+      double* out = im.partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
+      out[0] = sp;
+      out[1] = sb;
+    }
+  }
+  if (im.target == nullptr || a.v == nullptr) return;
+  // ---- pass 2: rebuild E~_k, multiply by dLdI, forward transform to the window rows
+  float2* vv = a.v + (size_t)b * rows * N;
+  for (int k = 0; k < a.k; ++k) {
+    CF::inverse_to_regs(buf, t + (size_t)k * a.wr * N, a.row_of_u, x0, a.tw, v);
+#pragma unroll
+    for (int s = 0; s < CF::PERM; ++s)
+#pragma unroll
+      for (int r = 0; r < CF::RM; ++r) v[s][r] = make_float2(v[s][r].x * g[s][r], v[s][r].y * g[s][r]);
+    __syncthreads();
+    CF::forward_from_regs(buf, v, a.tw, vv + (size_t)k * a.wr * N, a.row_of_u, x0);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- K8 pruned
+template <int N>
+__global__ void __launch_bounds__(ColCfg<N>::NT)
+k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
+  using C = ColCfg<N>;
+  using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ double red[2 * C::NT / 64];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int x0 = blockIdx.x * C::LINES;
+  const size_t tile = (size_t)b * N * N;
+  float2 v[CF::PERM][CF::RM];
+  CF::inverse_to_regs(buf, a.p + (size_t)b * a.wr * N, a.row_of_u, x0, a.tw, v);
+  double nrm = 0.0, dot = 0.0;
+  const int* rect = ga.rect != nullptr ? ga.rect + 4 * b : nullptr;
+#pragma unroll
+  for (int s = 0; s < CF::PERM; ++s) {
+    const int q = threadIdx.x + s * C::NT;
+    const int line = q % C::LINES, i = q / C::LINES;
+    const int x = x0 + line;
+    if (!((CF::BM % C::NT == 0 || q < CF::BM) && x < N)) continue;
+#pragma unroll
+    for (int r = 0; r < CF::RM; ++r) {
+      const int y = i + r * CF::TM;
+      const size_t gi = tile + (size_t)y * N + x;
+      const float gm = ga.scale * v[s][r].x;
+      if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gm;
+      if (ga.theta != nullptr) {
+        float gt = -0.5f * gm * sinf(ga.theta[gi]);
+        if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
+        if (ga.grad_theta != nullptr) ga.grad_theta[gi] = gt;
+        nrm += (double)gt * (double)gt;
+        if (ga.dir != nullptr) dot += (double)gt * (double)ga.dir[gi];
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nrm += __shfl_down(nrm, off, 64);
+    dot += __shfl_down(dot, off, 64);
+  }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) {
+    red[2 * wave] = nrm;
+    red[2 * wave + 1] = dot;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && ga.partials != nullptr) {
+    double sn = 0.0, sd = 0.0;
+    for (int w = 0; w < C::NT / 64; ++w) {
+      sn += red[2 * w];
+      sd += red[2 * w + 1];
+    }
+    double* out = ga.partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
+    out[0] = sn;
+    out[1] = sd;
+  }
+}
+
 // ---------------------------------------------------------------- reduce
 // out[b*stride + j] = sum_blk partials[(b*nblk + blk)*2 + j], fixed order.
 __global__ void k_reduce_pairs(const double* __restrict__ partials, int nblk, double* __restrict__ out,
@@ -470,6 +636,12 @@ struct SocsImpl {
   static void image_adjoint(const SocsArgs& a, const ImageArgs& im, int batch, hipStream_t s) {
     using C = ColCfg<N>;
     dim3 g(col_blocks(), batch);
+    if constexpr (fft::factorize(N).count >= 2) {
+      if (a.pruned) {
+        hipLaunchKernelGGL(k_col_image_adj_pruned<N>, g, dim3(C::NT), 0, s, a, im);
+        return;
+      }
+    }
     hipLaunchKernelGGL(k_col_image_adj<N>, g, dim3(C::NT), 0, s, a, im);
   }
 
@@ -481,6 +653,12 @@ struct SocsImpl {
     dim3 g7((a.wr + R::LINES - 1) / R::LINES, batch);
     hipLaunchKernelGGL(k_final_row_inv<N>, g7, dim3(R::NT), 0, s, a);
     dim3 g8(col_blocks(), batch);
+    if constexpr (fft::factorize(N).count >= 2) {
+      if (a.pruned) {
+        hipLaunchKernelGGL(k_final_col_inv_pruned<N>, g8, dim3(C::NT), 0, s, a, ga);
+        return;
+      }
+    }
     hipLaunchKernelGGL(k_final_col_inv<N>, g8, dim3(C::NT), 0, s, a, ga);
   }
 };
@@ -494,6 +672,11 @@ static const SocsOps* make_ops() {
   static const SocsOps ops = {N, &SocsImpl<N>::col_blocks, &SocsImpl<N>::forward_half,
                               &SocsImpl<N>::image_adjoint, &SocsImpl<N>::adjoint_tail};
   return &ops;
+}
+
+int first_radix(int n) {
+  const fft::Radices f = fft::factorize(n);
+  return f.count >= 2 ? f.r[0] : 0;
 }
 
 const SocsOps* socs_ops_for(int n) {

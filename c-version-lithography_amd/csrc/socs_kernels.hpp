@@ -26,6 +26,8 @@ struct SocsArgs {
   float2* y;                      // [b][k][wr][wc]
   float2* p;                      // [b][wr][N]
   const int* active;              // [b] nonzero = evaluate tile b; null = all tiles
+  const int* row_of_u;            // [N] window row of column frequency u, or -1
+  int pruned;                     // window fits the pruned column transforms
 };
 
 struct ImageArgs {
@@ -60,6 +62,8 @@ struct SocsOps {
 };
 
 const SocsOps* socs_ops_for(int n);
+// First radix of the column factorisation of n (0 if n has a single stage).
+int first_radix(int n);
 void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s);
 
 }  // namespace litho

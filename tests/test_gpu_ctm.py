@@ -107,3 +107,62 @@ def test_precompute_matches_oracle(gpu_ctx, n, k):
     a1, _, _ = im.forward(im.Cache.from_socs(mine), tgt, 0.225, 50.0)
     a2, _, _ = im.forward(im.Cache.from_socs(ref), tgt, 0.225, 50.0)
     assert np.max(np.abs(a1 - a2)) <= 1e-10 * np.max(np.abs(a2))
+
+
+@pytest.fixture(scope="module")
+def kernels2048(gpu_ctx):
+    dk = gpu_ctx.socs_precompute(2048, 4.0, 193.0, 1.35, 0.6, 0.9, 24)
+    return op.SocsKernels(n=2048, win_y0=dk["y0"], win_x0=dk["x0"], kernels=dk["kernels"], weights=dk["weights"],
+                          weight_sum=dk["weight_sum"])
+
+
+def test_eval_parity_2048(gpu_ctx, kernels2048):
+    # the headline configuration: 2048^2, K = 24, 219 x 219 window; FP32 device vs FP64 oracle with the
+    # same kernel file.  aerial rel L-inf <= 1e-5, gradient rel L2 <= 1e-4, PE rel <= 1e-5
+    print("case eval 2048 k=24", flush=True)
+    import os
+    la = litho_amd()
+    im.set_threads(len(os.sched_getaffinity(0)))
+    n = 2048
+    plan = la.Plan.from_socs(gpu_ctx, kernels2048)
+    tgt = targets().manhattan_target(n, 2048)
+    theta = oc.initialize_theta(tgt).astype(np.float32)
+    a, w, gm = empty(n), empty(n), empty(n)
+    mask = im.mask_from_theta(theta.astype(np.float64)).astype(np.float32)
+    pe, _ = plan.pe_gradient(dev(mask), dev(tgt), grad_mask=gm, aerial=a, wafer=w, threshold=0.225, alpha=50.0)
+    cache = im.Cache.from_socs(kernels2048)
+    ra, rw, fields = im.forward(cache, mask.astype(np.float64), 0.225, 50.0)
+    rg = im.pe_gradient(cache, rw, tgt.astype(np.float64), fields, 50.0)
+    del fields
+    ea = float(np.max(np.abs(host(a)[0] - ra)) / np.max(np.abs(ra)))
+    eg = float(np.linalg.norm(host(gm)[0] - rg) / np.linalg.norm(rg))
+    rpe = im.pe_loss(rw, tgt)
+    print(f"2048: aerial rel Linf {ea:.2e}, gradient rel L2 {eg:.2e}, PE {pe[0]:.4f} vs {rpe:.4f}")
+    assert ea <= 1e-5 and eg <= 1e-4 and abs(pe[0] - rpe) <= 1e-5 * rpe
+    plan.close()
+    im.set_threads(1)
+
+
+def test_lbfgs_2048_final_pe(gpu_ctx, kernels2048):
+    # the full headline solve: L-BFGS m = 10, 50 iterations, tol 1e-2.  Final PE within 0.5 % of the
+    # FP64 oracle's; thresholded-mask (M >= 0.5) pixel mismatches are counted and printed.
+    print("case lbfgs 2048 50 iterations", flush=True)
+    import os
+    la = litho_amd()
+    im.set_threads(len(os.sched_getaffinity(0)))
+    n = 2048
+    plan = la.Plan.from_socs(gpu_ctx, kernels2048)
+    tgt = targets().manhattan_target(n, 2048)
+    out = empty(n)
+    r = plan.ctm_optimize(dev(tgt), out[0], method="lbfgs", iterations=50, history=10, tol=1e-2,
+                          threshold=0.225, alpha=50.0)
+    cache = im.Cache.from_socs(kernels2048)
+    mask, hist, theta, info = oc.lbfgs(cache, tgt.astype(np.float64), 0.225, 50.0, 50, 10, 1e-2)
+    mism = int(np.sum((host(out)[0] >= 0.5) != (mask >= 0.5)))
+    near = int(np.sum(np.abs(mask - 0.5) < 1e-4))
+    print(f"2048 lbfgs: gpu PE {r['final_pe']:.4f} ({r['evaluations']} evals, status {r['status']}) "
+          f"oracle PE {info['final_pe']:.4f} ({len(hist)} evals, {info['status']}); "
+          f"binary mismatches {mism} of {n*n} (oracle pixels within 1e-4 of 0.5: {near})")
+    assert abs(r["final_pe"] - info["final_pe"]) <= 5e-3 * info["final_pe"]
+    plan.close()
+    im.set_threads(1)
