@@ -16,6 +16,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG.parent / "_build" / "litho_amd"
 LIB = PKG / "liblitho_amd.so"
+BENCH = PKG / "litho_bench"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -62,6 +63,14 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stdout}\n{p.stderr}")
     os.replace(tmp, LIB)
+    # C++ caller of the C ABI, used for profiling runs without Python
+    btmp = BENCH.with_suffix(".tmp")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-x", "hip", str(CSRC / "litho_bench_main.cpp"), "-o", str(btmp),
+           f"--offload-arch={ARCH}", "-L", str(PKG), "-llitho_amd", f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"litho_bench link failed:\n{p.stdout}\n{p.stderr}")
+    os.replace(btmp, BENCH)
     return LIB
 
 

@@ -50,17 +50,16 @@ struct ColumnFFT {
         const bool in = x < N;
         const float2 a = (in && ra >= 0) ? src[(size_t)ra * N + x] : make_float2(0.f, 0.f);
         const float2 c = (in && rb >= 0) ? src[(size_t)rb * N + x] : make_float2(0.f, 0.f);
-        float2* dst = buf + line * LS + i * R1;
-        store_two_tap<0>(dst, a, c);
+        store_two_tap<0>(buf + line * LS, i * R1, sw(i * R1), a, c);
       }
     }
   }
   template <int R>
-  __device__ __forceinline__ static void store_two_tap(float2* dst, float2 a, float2 c) {
+  __device__ __forceinline__ static void store_two_tap(float2* line, int o, int so, float2 a, float2 c) {
     if constexpr (R < R1) {
       // out[r] = a + c * w^{(R1-1) r}, w = e^{+2 pi i / R1}
-      dst[R] = cadd(a, twiddle_ct<(R1 - 1) * R, R1, +1>(c));
-      store_two_tap<R + 1>(dst, a, c);
+      line[sw_at<sw_uniform_write<1, R1>(), 1>(o, so, R)] = cadd(a, twiddle_ct<(R1 - 1) * R, R1, +1>(c));
+      store_two_tap<R + 1>(line, o, so, a, c);
     }
   }
 
@@ -95,9 +94,10 @@ struct ColumnFFT {
       const int q = threadIdx.x + s * NT;
       if (BM % NT == 0 || q < BM) {
         const int line = q % LINES, i = q / LINES;
-        const float2* p = buf + line * LS + i;
+        const float2* p = buf + line * LS;
+        const int si = sw(i);
 #pragma unroll
-        for (int r = 0; r < RM; ++r) v[s][r] = p[r * TM];
+        for (int r = 0; r < RM; ++r) v[s][r] = p[sw_at<sw_uniform_read<TM>(), TM>(i, si, r)];
 #pragma unroll
         for (int r = 1; r < RM; ++r) v[s][r] = cmulc(v[s][r], tw[r * i]);
         Dft<RM, +1>::run(v[s]);
@@ -116,9 +116,10 @@ struct ColumnFFT {
       if (BM % NT == 0 || q < BM) {
         const int line = q % LINES, i = q / LINES;
         Dft<RM, -1>::run(v[s]);
-        float2* d = buf + line * LS + i * RM;
+        float2* d = buf + line * LS;
+        const int so = sw(i * RM);
 #pragma unroll
-        for (int r = 0; r < RM; ++r) d[r] = v[s][r];
+        for (int r = 0; r < RM; ++r) d[sw_at<sw_uniform_write<1, RM>(), 1>(i * RM, so, r)] = v[s][r];
       }
     }
     __syncthreads();
@@ -130,9 +131,10 @@ struct ColumnFFT {
         const int line = q % LINES, i = q / LINES;
         const int x = x0 + line;
         float2 w[R1];
-        const float2* p = buf + line * LS + i;
+        const float2* p = buf + line * LS;
+        const int si = sw(i);
 #pragma unroll
-        for (int r = 0; r < R1; ++r) w[r] = p[r * T1];
+        for (int r = 0; r < R1; ++r) w[r] = p[sw_at<sw_uniform_read<T1>(), T1>(i, si, r)];
 #pragma unroll
         for (int r = 1; r < R1; ++r) w[r] = cmul(w[r], tw[r * i]);
         float2 o0 = w[0], o1 = w[0];

@@ -233,6 +233,44 @@ struct Dft {
   }
 };
 
+// ------------------------------------------------------------------ LDS layout
+// Element p of a line lives at sw(p) = p + p/16: one pad slot per 16 complex
+// values, so the strided Stockham writes (stride R = 2..16) and the contiguous
+// reads both hit 16 distinct 8-byte bank slots per 16 lanes.  Line strides are
+// padded_stride(N) = N + N/16 + 2, i.e. 2 (mod 16) slots, which also spreads
+// lanes that address the same element of neighbouring lines.
+#ifndef LITHO_LDS_SWIZZLE
+#define LITHO_LDS_SWIZZLE 1
+#endif
+#if LITHO_LDS_SWIZZLE
+__host__ __device__ constexpr int sw(int p) { return p + (p >> 4); }
+__host__ __device__ constexpr int padded_stride(int n) { return n + (n >> 4) + 2; }
+#else
+__host__ __device__ constexpr int sw(int p) { return p; }
+__host__ __device__ constexpr int padded_stride(int n) { return n + 1; }
+#endif
+
+// Swizzled address of element base + r*S (r = 0..R-1) given sw(base).  When the
+// shifted elements never straddle a 16-slot boundary differently for different
+// lanes, the offset is the compile-time constant r*S + (r*S)/16 and folds into
+// the LDS instruction's immediate; otherwise it is computed per element.
+//   reads  base + r*T           : uniform when T % 16 == 0
+//   writes (i-k)*R + k + r*P    : uniform when P % 16 == 0, or 16 % P == 0 and
+//                                 (R*P % 16 == 0 or 16 % (R*P) == 0)
+// ((i-k)*R is a multiple of R*P; with P | 16 the 16-slot blocks are unions of
+// P-blocks, so k never moves an element across a block boundary.)
+template <int S>
+__host__ __device__ constexpr bool sw_uniform_read() { return S % 16 == 0; }
+template <int P, int R>
+__host__ __device__ constexpr bool sw_uniform_write() {
+  return P % 16 == 0 || (16 % P == 0 && ((R * P) % 16 == 0 || 16 % (R * P) == 0));
+}
+template <bool UNIFORM, int S>
+__device__ __forceinline__ int sw_at(int base, int sw_base, int r) {
+  if constexpr (UNIFORM && LITHO_LDS_SWIZZLE) return sw_base + r * S + ((r * S) >> 4);
+  else return sw(base + r * S);
+}
+
 // ------------------------------------------------------------------ passes
 // One Stockham pass of radix R and span P over LINES lines of length N with
 // line stride LS (in float2) and NT threads.  Input position i + r*(N/R);
@@ -250,9 +288,10 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
     if (TOTAL % NT == 0 || q < TOTAL) {
       const int line = q / T;
       const int i = q - line * T;
-      const float2* src = buf + line * LS + i;
+      const float2* src = buf + line * LS;
+      const int si = sw(i);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[s][r] = src[r * T];
+      for (int r = 0; r < R; ++r) v[s][r] = src[sw_at<sw_uniform_read<T>(), T>(i, si, r)];
     }
   }
   __syncthreads();
@@ -272,9 +311,11 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
         }
       }
       Dft<R, DIR>::run(v[s]);
-      float2* dst = buf + line * LS + (i - k) * R + k;
+      float2* dst = buf + line * LS;
+      const int o = (i - k) * R + k;
+      const int so = sw(o);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * P] = v[s][r];
+      for (int r = 0; r < R; ++r) dst[sw_at<sw_uniform_write<P, R>(), P>(o, so, r)] = v[s][r];
     }
   }
   __syncthreads();

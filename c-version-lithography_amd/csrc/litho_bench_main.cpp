@@ -1,0 +1,131 @@
+// litho_bench: a C++ caller of the C ABI (include/litho_amd.h), used for
+// profiling runs without Python.  It mirrors the reference's timing harness
+// demo/demo_CTM_parallel_benchmark.cpp: set-up (optics, SOCS precompute, plan)
+// outside the timer, then timed work on one tile.
+//
+//   litho_bench eval  N K REPEATS [KFILE]   one CTM grad-eval per repeat (stage timings)
+//   litho_bench ctm   N K ITERS   [KFILE]   one L-BFGS CTM solve
+//   litho_bench save  N K KFILE             precompute the SOCS kernels and save them
+//   litho_bench copy  MIB                   device-to-device copy of MIB MiB (counter calibration)
+// With KFILE the kernels are read from a file written by `save` (no precompute).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/litho_amd.h"
+
+namespace {
+
+void check(int rc, const char* what) {
+  if (rc != LITHO_OK) {
+    std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, litho_last_error());
+    std::exit(1);
+  }
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "%s: %s\n", what, hipGetErrorString(e));
+    std::exit(1);
+  }
+}
+
+// deterministic Manhattan-like target: 20 x 20 px squares on a 30 px lattice
+std::vector<float> lattice_target(int n) {
+  std::vector<float> t((size_t)n * n, 0.f);
+  for (int y = 0; y < n; ++y)
+    for (int x = 0; x < n; ++x) t[(size_t)y * n + x] = ((y % 30) < 20 && (x % 30) < 20) ? 1.f : 0.f;
+  return t;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s eval N K REPEATS | ctm N K ITERS | copy MIB\n", argv[0]);
+    return 2;
+  }
+  const std::string mode = argv[1];
+  if (mode == "copy") {
+    const size_t bytes = (size_t)std::atol(argv[2]) << 20;
+    void *a = nullptr, *b = nullptr;
+    hip_check(hipMalloc(&a, bytes), "hipMalloc");
+    hip_check(hipMalloc(&b, bytes), "hipMalloc");
+    hip_check(hipMemset(a, 1, bytes), "hipMemset");
+    for (int i = 0; i < 3; ++i) hip_check(hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
+    hip_check(hipDeviceSynchronize(), "sync");
+    std::printf("copied %zu bytes x3\n", bytes);
+    return 0;
+  }
+  if (argc < 5) return 2;
+  const int n = std::atoi(argv[2]), k = std::atoi(argv[3]);
+  const int reps = mode == "save" ? 0 : std::atoi(argv[4]);
+  litho_ctx* ctx = nullptr;
+  check(litho_ctx_create(0, &ctx), "litho_ctx_create");
+  int wr = 0, wc = 0, y0 = 0, x0 = 0, ns = 0;
+  double wsum = 0.0;
+  check(litho_socs_window(n, 4.0, 193.0, 1.35, 0.6, 0.9, &wr, &wc, &y0, &x0, &ns, &wsum), "litho_socs_window");
+  std::vector<double> kern((size_t)k * wr * wc * 2), lam(k);
+  const char* kfile = argc > 5 ? argv[5] : (mode == "save" ? argv[4] : nullptr);
+  if (kfile != nullptr && mode != "save") {
+    FILE* f = std::fopen(kfile, "rb");
+    if (f == nullptr || std::fread(kern.data(), sizeof(double), kern.size(), f) != kern.size() ||
+        std::fread(lam.data(), sizeof(double), lam.size(), f) != lam.size()) {
+      std::fprintf(stderr, "cannot read kernels from %s\n", kfile);
+      return 1;
+    }
+    std::fclose(f);
+  } else {
+    check(litho_socs_precompute(ctx, n, 4.0, 193.0, 1.35, 0.6, 0.9, k, kern.data(), lam.data(), nullptr),
+          "litho_socs_precompute");
+  }
+  if (mode == "save") {
+    FILE* f = std::fopen(kfile, "wb");
+    if (f == nullptr) return 1;
+    std::fwrite(kern.data(), sizeof(double), kern.size(), f);
+    std::fwrite(lam.data(), sizeof(double), lam.size(), f);
+    std::fclose(f);
+    std::printf("saved %d kernels (%dx%d window) to %s\n", k, wr, wc, kfile);
+    check(litho_ctx_destroy(ctx), "litho_ctx_destroy");
+    return 0;
+  }
+  litho_plan* plan = nullptr;
+  check(litho_plan_create(ctx, n, k, wr, wc, y0, x0, kern.data(), lam.data(), wsum, 1, &plan), "litho_plan_create");
+  const size_t bytes = (size_t)n * n * sizeof(float);
+  float *target = nullptr, *theta = nullptr, *mask = nullptr;
+  hip_check(hipMalloc(&target, bytes), "hipMalloc");
+  hip_check(hipMalloc(&theta, bytes), "hipMalloc");
+  hip_check(hipMalloc(&mask, bytes), "hipMalloc");
+  const std::vector<float> t = lattice_target(n);
+  hip_check(hipMemcpy(target, t.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  check(litho_initialize_theta(ctx, target, theta, (size_t)n * n), "litho_initialize_theta");
+  if (mode == "eval") {
+    double st[4];
+    check(litho_profile_ctm_eval(plan, 1, theta, target, 0.225, 50.0, reps, st), "litho_profile_ctm_eval");
+    std::printf("n=%d k=%d window=%dx%d stages_ms forward=%.4f column=%.4f adjoint_tail=%.4f eval=%.4f\n", n, k, wr,
+                wc, st[0], st[1], st[2], st[3]);
+  } else if (mode == "ctm") {
+    litho_ctm_options o{1, reps, 0.9, 10, 1e-2, 0.225, 50.0};
+    litho_ctm_info info{};
+    const auto t0 = std::chrono::steady_clock::now();
+    check(litho_ctm_optimize(plan, target, nullptr, &o, mask, nullptr, nullptr, 0, &info), "litho_ctm_optimize");
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("n=%d k=%d lbfgs iterations=%d evaluations=%d final_pe=%.6f time=%.4f s (%.1f grad-evals/s)\n", n, k,
+                info.iterations, info.evaluations, info.final_pe, s, info.evaluations / s);
+  } else {
+    std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
+    return 2;
+  }
+  hip_check(hipFree(target), "hipFree");
+  hip_check(hipFree(theta), "hipFree");
+  hip_check(hipFree(mask), "hipFree");
+  check(litho_plan_destroy(plan), "litho_plan_destroy");
+  check(litho_ctx_destroy(ctx), "litho_ctx_destroy");
+  return 0;
+}

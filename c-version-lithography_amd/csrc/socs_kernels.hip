@@ -44,16 +44,44 @@ template <int N>
 struct RowCfg {
   static constexpr int LINES = N >= 1024 ? 4 : (N >= 256 ? 8 : 16);
   static constexpr int NT = N >= 512 ? 256 : (N >= 128 ? 128 : 64);
-  static constexpr int LS = N;  // line stride (float2)
+  static constexpr int LS = fft::padded_stride(N);  // line stride (float2), swizzled lines
 };
+
+#ifndef LITHO_COL_NT_LARGE
+#define LITHO_COL_NT_LARGE 1024
+#endif
+#ifndef LITHO_TW_LDS
+#define LITHO_TW_LDS 1
+#endif
 
 template <int N>
 struct ColCfg {
   static constexpr int LINES = N >= 256 ? 8 : 16;
-  static constexpr int NT = N >= 1024 ? 512 : (N >= 256 ? 256 : 64);
-  static constexpr int LS = N + 1;                      // pad against bank conflicts
+  static constexpr int NT = N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? 256 : 64);
+  static constexpr int LS = fft::padded_stride(N);    // line stride (float2), swizzled lines
   static constexpr int ELEMS = LINES * N;
   static constexpr int PER = (ELEMS + NT - 1) / NT;     // owned elements per thread
+  // one block per CU (its LDS): allow the registers of NT/256 waves per SIMD
+  static constexpr int MAX_WAVES_PER_EU = NT >= 512 ? NT / 256 : 1;
+  // twiddle table copied to LDS when it fits beside the lines (160 KB per CU)
+  static constexpr bool TW_LDS =
+      LITHO_TW_LDS && (size_t)LINES * LS * 8 + (size_t)N * 8 + (size_t)NT / 64 * 16 <= 160 * 1024;
+};
+
+// Twiddle source of the pruned column kernels: the LDS copy or the global
+// table.  The opaque zero offset taken per call stops the compiler from
+// hoisting the (loop-invariant) twiddle loads out of the K loops, which would
+// keep ~60 twiddle registers live across the loop and spill.
+template <int N>
+struct ColTwiddles {
+  float2* lds;
+  const float2* glob;
+  __device__ __forceinline__ const float2* get() const {
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    if constexpr (ColCfg<N>::TW_LDS) return lds + z;
+    return glob + z;
+  }
 };
 
 // ---------------------------------------------------------------- K1
@@ -93,7 +121,7 @@ k_row_fwd(SocsArgs a, int mode, const float* base, const float* __restrict__ dir
       }
       m2[h] = m;
     }
-    buf[l * C::LS + x] = make_float2(m2[0], m2[1]);
+    buf[l * C::LS + fft::sw(x)] = make_float2(m2[0], m2[1]);
   }
   __syncthreads();
   fft::fft_lines<N, PAIRS, C::LS, C::NT, -1>(buf, a.tw);
@@ -103,7 +131,7 @@ k_row_fwd(SocsArgs a, int mode, const float* base, const float* __restrict__ dir
     const int l = e / a.wc, j = e - l * a.wc;
     const int v = a.col_freq[j];
     const int vn = v == 0 ? 0 : N - v;
-    const float2 z = buf[l * C::LS + v], zc = cconj(buf[l * C::LS + vn]);
+    const float2 z = buf[l * C::LS + fft::sw(v)], zc = cconj(buf[l * C::LS + fft::sw(vn)]);
     const float2 xa = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
     const float2 xb = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
     const int y = row0 + 2 * l;
@@ -125,7 +153,7 @@ k_col_fwd(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, y = e - l * N;
     const int j = j0 + l;
-    buf[l * C::LS + y] = j < a.wc ? rt[(size_t)j * N + y] : make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::sw(y)] = j < a.wc ? rt[(size_t)j * N + y] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
@@ -133,7 +161,7 @@ k_col_fwd(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * a.wr; e += C::NT) {
     const int l = e / a.wr, i = e - l * a.wr;
     const int j = j0 + l;
-    if (j < a.wc) mh[(size_t)i * a.wc + j] = buf[l * C::LS + a.row_freq[i]];
+    if (j < a.wc) mh[(size_t)i * a.wc + j] = buf[l * C::LS + fft::sw(a.row_freq[i])];
   }
 }
 
@@ -149,7 +177,7 @@ k_row_inv(SocsArgs a) {
   const int q0 = blockIdx.x * C::LINES;
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
-    buf[l * C::LS + x] = make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::sw(x)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   const float2* mh = a.mhat + (size_t)b * a.wr * a.wc;
@@ -158,7 +186,7 @@ k_row_inv(SocsArgs a) {
     const int q = q0 + l;
     if (q < rows) {
       const int i = q % a.wr;
-      buf[l * C::LS + a.col_freq[j]] = cmul(a.h[(size_t)q * a.wc + j], mh[(size_t)i * a.wc + j]);
+      buf[l * C::LS + fft::sw(a.col_freq[j])] = cmul(a.h[(size_t)q * a.wc + j], mh[(size_t)i * a.wc + j]);
     }
   }
   __syncthreads();
@@ -167,7 +195,7 @@ k_row_inv(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
     const int q = q0 + l;
-    if (q < rows) t[(size_t)q * N + x] = buf[l * C::LS + x];
+    if (q < rows) t[(size_t)q * N + x] = buf[l * C::LS + fft::sw(x)];
   }
 }
 
@@ -181,12 +209,12 @@ __device__ __forceinline__ void load_window_columns(float2* buf, const float2* _
   for (int e = threadIdx.x; e < wr * C::LINES; e += C::NT) {
     const int i = e / C::LINES, c = e - i * C::LINES;
     const int x = x0 + c;
-    if (x < N) buf[c * C::LS + row_freq[i]] = src[(size_t)i * N + x];
+    if (x < N) buf[c * C::LS + fft::sw(row_freq[i])] = src[(size_t)i * N + x];
   }
 }
 
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT)
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_col_image_adj(SocsArgs a, ImageArgs im) {
   using C = ColCfg<N>;
   __shared__ float2 buf[C::LINES * C::LS];
@@ -203,7 +231,7 @@ k_col_image_adj(SocsArgs a, ImageArgs im) {
   for (int s = 0; s < C::PER; ++s) {
     acc[s] = 0.f;
     const int e = threadIdx.x + s * C::NT;
-    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + e / C::LINES] = make_float2(0.f, 0.f);
+    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::sw(e / C::LINES)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   // ---- pass 1: aerial image
@@ -215,7 +243,7 @@ k_col_image_adj(SocsArgs a, ImageArgs im) {
     for (int s = 0; s < C::PER; ++s) {
       const int e = threadIdx.x + s * C::NT;
       if (e < C::ELEMS) {
-        float2* p = buf + (e % C::LINES) * C::LS + e / C::LINES;
+        float2* p = buf + (e % C::LINES) * C::LS + fft::sw(e / C::LINES);
         const float2 z = *p;
         acc[s] = fmaf(z.x, z.x, fmaf(z.y, z.y, acc[s]));
         *p = make_float2(0.f, 0.f);
@@ -290,7 +318,7 @@ k_col_image_adj(SocsArgs a, ImageArgs im) {
     for (int s = 0; s < C::PER; ++s) {
       const int e = threadIdx.x + s * C::NT;
       if (e < C::ELEMS) {
-        float2* p = buf + (e % C::LINES) * C::LS + e / C::LINES;
+        float2* p = buf + (e % C::LINES) * C::LS + fft::sw(e / C::LINES);
         const float2 z = *p;
         *p = make_float2(z.x * g[s], z.y * g[s]);
       }
@@ -301,13 +329,13 @@ k_col_image_adj(SocsArgs a, ImageArgs im) {
     for (int e = threadIdx.x; e < a.wr * C::LINES; e += C::NT) {
       const int i = e / C::LINES, c = e - i * C::LINES;
       const int x = x0 + c;
-      if (x < N) vk[(size_t)i * N + x] = buf[c * C::LS + a.row_freq[i]];
+      if (x < N) vk[(size_t)i * N + x] = buf[c * C::LS + fft::sw(a.row_freq[i])];
     }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < C::PER; ++s) {
       const int e = threadIdx.x + s * C::NT;
-      if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + e / C::LINES] = make_float2(0.f, 0.f);
+      if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::sw(e / C::LINES)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
   }
@@ -327,7 +355,7 @@ k_row_adj(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
     const int q = q0 + l;
-    buf[l * C::LS + x] = q < rows ? v[(size_t)q * N + x] : make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::sw(x)] = q < rows ? v[(size_t)q * N + x] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
@@ -335,7 +363,7 @@ k_row_adj(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * a.wc; e += C::NT) {
     const int l = e / a.wc, j = e - l * a.wc;
     const int q = q0 + l;
-    if (q < rows) yk[(size_t)q * a.wc + j] = cmulc(buf[l * C::LS + a.col_freq[j]], a.h[(size_t)q * a.wc + j]);
+    if (q < rows) yk[(size_t)q * a.wc + j] = cmulc(buf[l * C::LS + fft::sw(a.col_freq[j])], a.h[(size_t)q * a.wc + j]);
   }
 }
 
@@ -351,7 +379,7 @@ k_final_row_inv(SocsArgs a) {
   const int i0 = blockIdx.x * C::LINES;
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
-    buf[l * C::LS + x] = make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::sw(x)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   const float2* yk = a.y + (size_t)b * rows * a.wc;
@@ -365,7 +393,7 @@ k_final_row_inv(SocsArgs a) {
         s.x += u.x;
         s.y += u.y;
       }
-      buf[l * C::LS + a.col_freq[j]] = s;
+      buf[l * C::LS + fft::sw(a.col_freq[j])] = s;
     }
   }
   __syncthreads();
@@ -374,13 +402,13 @@ k_final_row_inv(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
     const int i = i0 + l;
-    if (i < a.wr) p[(size_t)i * N + x] = buf[l * C::LS + x];
+    if (i < a.wr) p[(size_t)i * N + x] = buf[l * C::LS + fft::sw(x)];
   }
 }
 
 // ---------------------------------------------------------------- K8
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT)
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_final_col_inv(SocsArgs a, GradArgs ga) {
   using C = ColCfg<N>;
   __shared__ float2 buf[C::LINES * C::LS];
@@ -392,7 +420,7 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
 #pragma unroll
   for (int s = 0; s < C::PER; ++s) {
     const int e = threadIdx.x + s * C::NT;
-    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + e / C::LINES] = make_float2(0.f, 0.f);
+    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::sw(e / C::LINES)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   load_window_columns<N>(buf, a.p + (size_t)b * a.wr * N, a.wr, a.row_freq, x0);
@@ -407,7 +435,7 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
     const int x = x0 + c;
     if (e < C::ELEMS && x < N) {
       const size_t gi = tile + (size_t)y * N + x;
-      const float gm = ga.scale * buf[c * C::LS + y].x;
+      const float gm = ga.scale * buf[c * C::LS + fft::sw(y)].x;
       if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gm;
       if (ga.theta != nullptr) {
         float gt = -0.5f * gm * sinf(ga.theta[gi]);
@@ -445,18 +473,24 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
 // fft_columns.hpp.  Element ownership follows the last inverse stage:
 // column q % LINES, row i + r*TM with q = tid + s*NT, i = q / LINES.
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT)
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
   using C = ColCfg<N>;
   using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;
   __shared__ float2 buf[C::LINES * C::LS];
   __shared__ double red[2 * C::NT / 64];
+  __shared__ float2 tws[C::TW_LDS ? N : 1];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
   const int x0 = blockIdx.x * C::LINES;
   const size_t tile = (size_t)b * N * N;
   const int rows = a.k * a.wr;
   const float2* t = a.t + (size_t)b * rows * N;
+  if constexpr (C::TW_LDS) {
+    for (int e = threadIdx.x; e < N; e += C::NT) tws[e] = a.tw[e];
+    __syncthreads();
+  }
+  const ColTwiddles<N> twd{tws, a.tw};
   float acc[CF::PERM][CF::RM];
   float2 v[CF::PERM][CF::RM];
 #pragma unroll
@@ -464,8 +498,9 @@ k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
 #pragma unroll
     for (int r = 0; r < CF::RM; ++r) acc[s][r] = 0.f;
   // ---- pass 1: aerial image
+#pragma unroll 1
   for (int k = 0; k < a.k; ++k) {
-    CF::inverse_to_regs(buf, t + (size_t)k * a.wr * N, a.row_of_u, x0, a.tw, v);
+    CF::inverse_to_regs(buf, t + (size_t)k * a.wr * N, a.row_of_u, x0, twd.get(), v);
 #pragma unroll
     for (int s = 0; s < CF::PERM; ++s)
 #pragma unroll
@@ -534,21 +569,22 @@ k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
   if (im.target == nullptr || a.v == nullptr) return;
   // ---- pass 2: rebuild E~_k, multiply by dLdI, forward transform to the window rows
   float2* vv = a.v + (size_t)b * rows * N;
+#pragma unroll 1
   for (int k = 0; k < a.k; ++k) {
-    CF::inverse_to_regs(buf, t + (size_t)k * a.wr * N, a.row_of_u, x0, a.tw, v);
+    CF::inverse_to_regs(buf, t + (size_t)k * a.wr * N, a.row_of_u, x0, twd.get(), v);
 #pragma unroll
     for (int s = 0; s < CF::PERM; ++s)
 #pragma unroll
       for (int r = 0; r < CF::RM; ++r) v[s][r] = make_float2(v[s][r].x * g[s][r], v[s][r].y * g[s][r]);
     __syncthreads();
-    CF::forward_from_regs(buf, v, a.tw, vv + (size_t)k * a.wr * N, a.row_of_u, x0);
+    CF::forward_from_regs(buf, v, twd.get(), vv + (size_t)k * a.wr * N, a.row_of_u, x0);
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------- K8 pruned
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT)
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
   using C = ColCfg<N>;
   using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;

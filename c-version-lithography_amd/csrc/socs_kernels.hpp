@@ -7,9 +7,11 @@ namespace litho {
 
 // Tile sizes with compiled FFT instantiations: the CPU-test sizes, the Micro
 // tile sizes (core + 2*halo) and the single-tile sizes 512 / 2048.
+#ifndef LITHO_SUPPORTED_SIZES
 #define LITHO_SUPPORTED_SIZES(X)                                                            \
   X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
   X(320) X(384) X(512) X(640) X(768) X(896) X(1024) X(1152) X(1280) X(1536) X(2048) X(2176)
+#endif
 
 // Plan-resident data shared by all kernels of one tile size (per-tile buffers
 // are indexed by blockIdx.y with the strides implied by the shapes).
