@@ -36,7 +36,39 @@ double zoom_trial(double a_lo, double f_lo, double g_lo, double a_hi, double f_h
 namespace {
 
 struct Buffers {
-  DevMem x, xp, g, gp, d, mask, best, hs, hy, meta, coefs, pair_part, dir_part, vec_red, active, step;
+  DevMem x, xp, g, gp, d, mask, hs, hy, pair_part, dir_part, vec_red;
+};
+
+// Host-written parameters of a batch (active flags, step coefficients, L-BFGS
+// slot lists and direction coefficients), staged in one pinned block and
+// uploaded region by region with one transfer each.  The host rewrites a
+// region only after a stream synchronisation that follows its last upload.
+struct Params {
+  size_t off_step, off_meta, off_coef, bytes;
+  HostPinned host;
+  DevMem dev;
+  static size_t layout(int cap, size_t* step, size_t* meta, size_t* coef) {
+    *step = (size_t)cap * sizeof(int);
+    *meta = *step + (size_t)cap * sizeof(float);
+    *coef = (*meta + (size_t)cap * kMetaStride * sizeof(int) + 7) / 8 * 8;
+    return *coef + (size_t)cap * kCoefStride * sizeof(double);
+  }
+  explicit Params(int cap)
+      : bytes(layout(cap, &off_step, &off_meta, &off_coef)), host(bytes), dev(bytes) {}
+  int* active() const { return host.as<int>(); }
+  float* step() const { return reinterpret_cast<float*>(host.as<char>() + off_step); }
+  int* meta() const { return reinterpret_cast<int*>(host.as<char>() + off_meta); }
+  double* coef() const { return reinterpret_cast<double*>(host.as<char>() + off_coef); }
+  const int* d_active() const { return dev.as<int>(); }
+  const float* d_step() const { return reinterpret_cast<const float*>(dev.as<char>() + off_step); }
+  const int* d_meta() const { return reinterpret_cast<const int*>(dev.as<char>() + off_meta); }
+  const double* d_coef() const { return reinterpret_cast<const double*>(dev.as<char>() + off_coef); }
+  void upload(size_t off, size_t end, hipStream_t s) const {
+    LITHO_HIP(hipMemcpyAsync(dev.as<char>() + off, host.as<char>() + off, end - off, hipMemcpyHostToDevice, s));
+  }
+  void upload_active_step(hipStream_t s) const { upload(0, off_meta, s); }
+  void upload_meta(hipStream_t s) const { upload(off_meta, off_coef, s); }
+  void upload_meta_coef(hipStream_t s) const { upload(off_meta, bytes, s); }
 };
 
 void copy_tile(float* dst, const float* src, size_t n, hipStream_t s) {
@@ -52,8 +84,7 @@ std::vector<CtmTileResult> run_gd(Plan& plan, int batch, const float* targets, c
   bf.x.alloc(batch * n2 * sizeof(float));
   bf.g.alloc(batch * n2 * sizeof(float));
   bf.mask.alloc(batch * n2 * sizeof(float));
-  bf.active.alloc(batch * sizeof(int));
-  bf.step.alloc(batch * sizeof(float));
+  Params pm(batch);
   float* x = bf.x.as<float>();
   if (theta0 != nullptr)
     LITHO_HIP(hipMemcpyAsync(x, theta0, batch * n2 * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -64,21 +95,27 @@ std::vector<CtmTileResult> run_gd(Plan& plan, int batch, const float* targets, c
   std::vector<CtmTileResult> res(batch);
   std::vector<int> running(batch, 1);
   std::vector<double> min_pe(batch, std::numeric_limits<double>::max());
-  std::vector<float> coef(batch, 0.f);
-  std::vector<int> flags(batch, 1);
-  LITHO_HIP(hipMemcpyAsync(bf.active.ptr, flags.data(), batch * sizeof(int), hipMemcpyHostToDevice, s));
+  int* flags = pm.active();
+  float* coef = pm.step();
+  for (int b = 0; b < batch; ++b) {
+    flags[b] = 1;
+    coef[b] = 0.f;
+  }
 
   for (int it = 0; it < opt.iterations; ++it) {
     EvalRequest rq;
     rq.mode = 1;
     rq.base = x;
     if (it > 0) {
-      for (int b = 0; b < batch; ++b) coef[b] = running[b] ? (float)(-opt.learning_rate) : 0.f;
-      LITHO_HIP(hipMemcpyAsync(bf.step.ptr, coef.data(), batch * sizeof(float), hipMemcpyHostToDevice, s));
+      for (int b = 0; b < batch; ++b) {
+        coef[b] = running[b] ? (float)(-opt.learning_rate) : 0.f;
+        flags[b] = running[b];
+      }
       rq.dir = bf.g.as<float>();
-      rq.coef = bf.step.as<float>();
+      rq.coef = pm.d_step();
       rq.theta_out = x;
     }
+    pm.upload_active_step(s);
     rq.mask_out = bf.mask.as<float>();
     rq.im.thr = (float)opt.threshold;
     rq.im.alpha = (float)opt.alpha;
@@ -87,7 +124,7 @@ std::vector<CtmTileResult> run_gd(Plan& plan, int batch, const float* targets, c
     rq.ga.scale = (float)(4.0 * opt.alpha);
     rq.ga.theta = x;
     rq.ga.grad_theta = bf.g.as<float>();
-    rq.active = bf.active.as<int>();
+    rq.active = pm.d_active();
     run_eval(plan, batch, rq);
     const double* red = fetch_reductions(plan, batch);
     bool any = false;
@@ -111,16 +148,11 @@ std::vector<CtmTileResult> run_gd(Plan& plan, int batch, const float* targets, c
       }
     }
     if (!any) break;
-    bool changed = false;
-    for (int b = 0; b < batch; ++b)
-      if (flags[b] != running[b]) { flags[b] = running[b]; changed = true; }
-    if (changed)
-      LITHO_HIP(hipMemcpyAsync(bf.active.ptr, flags.data(), batch * sizeof(int), hipMemcpyHostToDevice, s));
   }
   // the reference applies the update after the last evaluation too (CTM_Optimizer.cpp:164-165)
   for (int b = 0; b < batch; ++b) coef[b] = running[b] ? (float)(-opt.learning_rate) : 0.f;
-  LITHO_HIP(hipMemcpyAsync(bf.step.ptr, coef.data(), batch * sizeof(float), hipMemcpyHostToDevice, s));
-  launch_axpy(bf.g.as<float>(), x, bf.step.as<float>(), nullptr, n2, batch, s);
+  pm.upload_active_step(s);
+  launch_axpy(bf.g.as<float>(), x, pm.d_step(), nullptr, n2, batch, s);
   for (int b = 0; b < batch; ++b) res[b].final_pe = min_pe[b];
   if (out_theta != nullptr)
     LITHO_HIP(hipMemcpyAsync(out_theta, x, batch * n2 * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -192,13 +224,10 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
   bf.mask.alloc(batch * n2 * sizeof(float));
   bf.hs.alloc(batch * (size_t)(kMaxHist + 1) * n2 * sizeof(float));
   bf.hy.alloc(batch * (size_t)(kMaxHist + 1) * n2 * sizeof(float));
-  bf.meta.alloc(batch * kMetaStride * sizeof(int));
-  bf.coefs.alloc(batch * kCoefStride * sizeof(double));
   bf.pair_part.alloc(batch * (size_t)kOptBlocks * kPairDots * sizeof(double));
   bf.dir_part.alloc(batch * (size_t)kOptBlocks * sizeof(double));
   bf.vec_red.alloc(batch * kPairDots * sizeof(double));
-  bf.active.alloc(batch * sizeof(int));
-  bf.step.alloc(batch * sizeof(float));
+  Params pm(batch);
   HostPinned hvec(batch * kPairDots * sizeof(double));
   float* x = bf.x.as<float>();
   float* xp = bf.xp.as<float>();
@@ -214,15 +243,22 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
   std::vector<LbfgsTile> st(batch);
   std::vector<CtmTileResult> res(batch);
   for (int b = 0; b < batch; ++b) st[b].spare = 0;
-  std::vector<float> stepv(batch, 0.f);
-  std::vector<int> flags(batch, 1), meta(batch * kMetaStride, -1);
-  std::vector<double> coefs(batch * kCoefStride, 0.0);
+  float* stepv = pm.step();
+  int* flags = pm.active();
+  int* meta = pm.meta();
+  double* coefs = pm.coef();
+  for (int b = 0; b < batch; ++b) {
+    stepv[b] = 0.f;
+    flags[b] = 1;
+  }
+  for (int i = 0; i < batch * kMetaStride; ++i) meta[i] = -1;
+  for (int i = 0; i < batch * kCoefStride; ++i) coefs[i] = 0.0;
   const double eps = std::numeric_limits<double>::epsilon();
 
+  // xp / gp must already hold the accepted point and its gradient (the pair
+  // kernel shifts them; the first line search copies them)
   auto begin_line_search = [&](int b, double dg0) {
     LbfgsTile& t = st[b];
-    copy_tile(xp + b * n2, x + b * n2, n2, s);
-    copy_tile(gp + b * n2, g + b * n2, n2, s);
     t.f0 = t.f;
     t.dg0 = dg0;
     t.trials = 0;
@@ -266,8 +302,9 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
   for (;;) {
     // ---- (A) new directions for tiles that accepted a step
     bool need_pair = false;
+    int max_m = 0;
     for (int b = 0; b < batch; ++b) {
-      int* mb = meta.data() + b * kMetaStride;
+      int* mb = meta + b * kMetaStride;
       for (int i = 0; i < kMetaStride; ++i) mb[i] = 0;
       mb[0] = -1;
       if (st[b].phase == LbfgsTile::NEED_PAIR) {
@@ -276,13 +313,16 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
         mb[0] = (int)t.slots.size();
         mb[1] = t.spare;
         for (size_t j = 0; j < t.slots.size(); ++j) mb[2 + j] = t.slots[j];
+        max_m = std::max(max_m, mb[0]);
       }
     }
     if (need_pair) {
-      LITHO_HIP(hipMemcpyAsync(bf.meta.ptr, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, s));
-      launch_lbfgs_pair(x, xp, g, gp, bf.hs.as<float>(), bf.hy.as<float>(), bf.meta.as<int>(), n2, batch,
+      const int pair_nv = 4 * (max_m + 1);   // dots per tile: 4 per old slot, then the new pair
+      pm.upload_meta(s);
+      launch_lbfgs_pair(x, xp, g, gp, bf.hs.as<float>(), bf.hy.as<float>(), pm.d_meta(), n2, batch, max_m, true,
                         bf.pair_part.as<double>(), s);
-      launch_reduce_vec(bf.pair_part.as<double>(), kOptBlocks, kPairDots, bf.vec_red.as<double>(), batch, s);
+      launch_reduce_fixed(bf.pair_part.as<double>(), kOptBlocks, pair_nv, bf.vec_red.as<double>(), kPairDots, batch,
+                          s);
       LITHO_HIP(hipMemcpyAsync(hvec.ptr, bf.vec_red.ptr, batch * kPairDots * sizeof(double), hipMemcpyDeviceToHost, s));
       LITHO_HIP(hipStreamSynchronize(s));
       const double* dots = hvec.as<double>();
@@ -291,7 +331,7 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
         LbfgsTile& t = st[b];
         const double* v = dots + (size_t)b * kPairDots;
         const int m = (int)t.slots.size();
-        const double ys = v[4 * kMaxHist + 0], yy = v[4 * kMaxHist + 1];
+        const double ys = v[4 * max_m + 0], yy = v[4 * max_m + 1];
         std::vector<double> p(m), q(m);
         for (int j = 0; j < m; ++j) {
           p[j] = v[4 * j + 2];
@@ -326,8 +366,8 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
             p2[i] = p[i + drop];
             q2[i] = q[i + drop];
           }
-          p2[m2 - 1] = v[4 * kMaxHist + 2];
-          q2[m2 - 1] = v[4 * kMaxHist + 3];
+          p2[m2 - 1] = v[4 * max_m + 2];
+          q2[m2 - 1] = v[4 * max_m + 3];
           // spare slot for the next pair: the freed slot or an unused index
           if (freed >= 0) {
             t.spare = freed;
@@ -342,21 +382,22 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
           p = p2;
           q = q2;
         }
-        solve_compact(t, p, q, coefs.data() + (size_t)b * kCoefStride);
+        solve_compact(t, p, q, coefs + (size_t)b * kCoefStride);
       }
       // rebuild meta with the committed slot lists for the direction kernel
+      int dir_m = 0;
       for (int b = 0; b < batch; ++b) {
-        int* mb = meta.data() + b * kMetaStride;
+        int* mb = meta + b * kMetaStride;
         if (st[b].phase != LbfgsTile::NEED_PAIR) continue;
         mb[0] = (int)st[b].slots.size();
         mb[1] = st[b].spare;
         for (size_t j = 0; j < st[b].slots.size(); ++j) mb[2 + j] = st[b].slots[j];
+        dir_m = std::max(dir_m, mb[0]);
       }
-      LITHO_HIP(hipMemcpyAsync(bf.meta.ptr, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, s));
-      LITHO_HIP(hipMemcpyAsync(bf.coefs.ptr, coefs.data(), coefs.size() * sizeof(double), hipMemcpyHostToDevice, s));
-      launch_lbfgs_direction(g, bf.hs.as<float>(), bf.hy.as<float>(), bf.meta.as<int>(), bf.coefs.as<double>(), n2, d,
-                             batch, bf.dir_part.as<double>(), s);
-      launch_reduce_vec(bf.dir_part.as<double>(), kOptBlocks, 1, bf.vec_red.as<double>(), batch, s);
+      pm.upload_meta_coef(s);
+      launch_lbfgs_direction(g, bf.hs.as<float>(), bf.hy.as<float>(), pm.d_meta(), pm.d_coef(), n2, d, batch, dir_m,
+                             bf.dir_part.as<double>(), s);
+      launch_reduce_fixed(bf.dir_part.as<double>(), kOptBlocks, 1, bf.vec_red.as<double>(), 1, batch, s);
       LITHO_HIP(hipMemcpyAsync(hvec.ptr, bf.vec_red.ptr, batch * sizeof(double), hipMemcpyDeviceToHost, s));
       LITHO_HIP(hipStreamSynchronize(s));
       for (int b = 0; b < batch; ++b) {
@@ -376,13 +417,12 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
       any = any || ev;
     }
     if (!any) break;
-    LITHO_HIP(hipMemcpyAsync(bf.active.ptr, flags.data(), batch * sizeof(int), hipMemcpyHostToDevice, s));
-    LITHO_HIP(hipMemcpyAsync(bf.step.ptr, stepv.data(), batch * sizeof(float), hipMemcpyHostToDevice, s));
+    pm.upload_active_step(s);
     EvalRequest rq;
     rq.mode = 1;
     rq.base = xp;
     rq.dir = d;
-    rq.coef = bf.step.as<float>();
+    rq.coef = pm.d_step();
     rq.theta_out = x;
     rq.mask_out = bf.mask.as<float>();
     rq.im.thr = (float)opt.threshold;
@@ -393,7 +433,7 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
     rq.ga.theta = x;
     rq.ga.grad_theta = g;
     rq.ga.dir = d;
-    rq.active = bf.active.as<int>();
+    rq.active = pm.d_active();
     run_eval(plan, batch, rq);
     const double* red = fetch_reductions(plan, batch);
     for (int b = 0; b < batch; ++b) {
@@ -470,11 +510,16 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
       init_dir = init_dir || flags[b];
     }
     if (init_dir) {
-      LITHO_HIP(hipMemcpyAsync(bf.active.ptr, flags.data(), batch * sizeof(int), hipMemcpyHostToDevice, s));
-      LITHO_HIP(hipMemcpyAsync(bf.step.ptr, stepv.data(), batch * sizeof(float), hipMemcpyHostToDevice, s));
-      launch_scale(g, d, bf.step.as<float>(), bf.active.as<int>(), n2, batch, s);
-      for (int b = 0; b < batch; ++b)
-        if (flags[b]) begin_line_search(b, -st[b].gnorm * st[b].gnorm);
+      pm.upload_active_step(s);
+      launch_scale(g, d, pm.d_step(), pm.d_active(), n2, batch, s);
+      for (int b = 0; b < batch; ++b) {
+        if (!flags[b]) continue;
+        copy_tile(xp + b * n2, x + b * n2, n2, s);
+        copy_tile(gp + b * n2, g + b * n2, n2, s);
+        begin_line_search(b, -st[b].gnorm * st[b].gnorm);
+      }
+      // the staged flags / steps are rewritten next iteration: wait for the upload
+      LITHO_HIP(hipStreamSynchronize(s));
     }
   }
   // final theta: x, or xp after a failed line search

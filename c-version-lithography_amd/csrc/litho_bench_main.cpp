@@ -4,7 +4,7 @@
 // outside the timer, then timed work on one tile.
 //
 //   litho_bench eval  N K REPEATS [KFILE]   one CTM grad-eval per repeat (stage timings)
-//   litho_bench ctm   N K ITERS   [KFILE]   one L-BFGS CTM solve
+//   litho_bench ctm   N K ITERS   KFILE [S] S L-BFGS CTM solves (default 3), each timed
 //   litho_bench save  N K KFILE             precompute the SOCS kernels and save them
 //   litho_bench copy  MIB                   device-to-device copy of MIB MiB (counter calibration)
 // With KFILE the kernels are read from a file written by `save` (no precompute).
@@ -111,13 +111,16 @@ int main(int argc, char** argv) {
     std::printf("n=%d k=%d window=%dx%d stages_ms forward=%.4f column=%.4f adjoint_tail=%.4f eval=%.4f\n", n, k, wr,
                 wc, st[0], st[1], st[2], st[3]);
   } else if (mode == "ctm") {
+    const int solves = argc > 6 ? std::atoi(argv[6]) : 3;
     litho_ctm_options o{1, reps, 0.9, 10, 1e-2, 0.225, 50.0};
-    litho_ctm_info info{};
-    const auto t0 = std::chrono::steady_clock::now();
-    check(litho_ctm_optimize(plan, target, nullptr, &o, mask, nullptr, nullptr, 0, &info), "litho_ctm_optimize");
-    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    std::printf("n=%d k=%d lbfgs iterations=%d evaluations=%d final_pe=%.6f time=%.4f s (%.1f grad-evals/s)\n", n, k,
-                info.iterations, info.evaluations, info.final_pe, s, info.evaluations / s);
+    for (int i = 0; i < solves; ++i) {
+      litho_ctm_info info{};
+      const auto t0 = std::chrono::steady_clock::now();
+      check(litho_ctm_optimize(plan, target, nullptr, &o, mask, nullptr, nullptr, 0, &info), "litho_ctm_optimize");
+      const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      std::printf("solve %d: n=%d k=%d lbfgs iterations=%d evaluations=%d final_pe=%.6f time=%.4f s (%.1f grad-evals/s)\n",
+                  i, n, k, info.iterations, info.evaluations, info.final_pe, s, info.evaluations / s);
+    }
   } else {
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;

@@ -5,6 +5,8 @@
 
 #include "opt_kernels.hpp"
 
+#include <utility>
+
 namespace litho {
 
 // CTM_Optimizer::initialize_theta (CTM_Optimizer.cpp:66-73)
@@ -68,7 +70,7 @@ void launch_mask_from_theta(const float* theta, float* mask, size_t count, hipSt
 
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double* __restrict__ out) {
-  __shared__ double red[kOptThreads / 64][NV];
+  __shared__ double red[kOptThreads / 64][NV > 0 ? NV : 1];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double x = acc[v];
@@ -83,15 +85,23 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double* __
   }
 }
 
+__device__ __forceinline__ double dot2(float2 a, float2 b) {
+  return fma((double)a.x, (double)b.x, (double)a.y * (double)b.y);
+}
+
 // L-BFGS pair update.  For tile b: s_new = x - xp, y_new = g - gp written to
-// slot new_slot of the history; dot products (FP64) per block:
+// slot new_slot of the history; FP64 dot products per block, NV = 4 (MB + 1):
 //   [4j+0] s_j . y_new   [4j+1] y_j . y_new   [4j+2] s_j . g   [4j+3] y_j . g
-// for the active slots j < m (slot index slots[b*kMaxHist + j]) and the new pair
-// at position j = m (slot new_slot).
+// for the active slots j < m <= MB (slot index slots[j]), then the new pair at
+// position 4 MB.  With shift set, xp <- x and gp <- g afterwards (the accepted
+// point becomes the base of the next line search).  Vectors are read as float2:
+// n is even for every supported tile (N^2 with N even).
+template <int MB>
 __global__ void __launch_bounds__(kOptThreads)
-k_lbfgs_pair(const float* __restrict__ x, const float* __restrict__ xp, const float* __restrict__ g,
-             const float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy,
-             const int* __restrict__ meta, size_t n, double* __restrict__ partials) {
+k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* __restrict__ g,
+             float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy, const int* __restrict__ meta,
+             size_t n, int shift, double* __restrict__ partials) {
+  constexpr int NV = 4 * (MB + 1);
   const int b = blockIdx.y;
   const int m = meta[b * kMetaStride + 0];
   if (m < 0) return;
@@ -101,51 +111,56 @@ k_lbfgs_pair(const float* __restrict__ x, const float* __restrict__ xp, const fl
   const size_t hist = (size_t)(kMaxHist + 1) * n;  // per-tile history stride
   float* hs_b = hs + (size_t)b * hist;
   float* hy_b = hy + (size_t)b * hist;
-  double acc[kPairDots];
+  double acc[NV];
 #pragma unroll
-  for (int v = 0; v < kPairDots; ++v) acc[v] = 0.0;
-  const float* sp[kMaxHist];
-  const float* yp[kMaxHist];
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  const float2* sp[MB > 0 ? MB : 1];
+  const float2* yp[MB > 0 ? MB : 1];
 #pragma unroll
-  for (int j = 0; j < kMaxHist; ++j) {
+  for (int j = 0; j < MB; ++j) {
     const int sl = j < m ? slots[j] : 0;
-    sp[j] = hs_b + (size_t)sl * n;
-    yp[j] = hy_b + (size_t)sl * n;
+    sp[j] = reinterpret_cast<const float2*>(hs_b + (size_t)sl * n);
+    yp[j] = reinterpret_cast<const float2*>(hy_b + (size_t)sl * n);
   }
-  float* sn = hs_b + (size_t)new_slot * n;
-  float* yn = hy_b + (size_t)new_slot * n;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const float s = x[tb + i] - xp[tb + i];
-    const float y = g[tb + i] - gp[tb + i];
-    const float gv = g[tb + i];
+  float2* sn = reinterpret_cast<float2*>(hs_b + (size_t)new_slot * n);
+  float2* yn = reinterpret_cast<float2*>(hy_b + (size_t)new_slot * n);
+  const float2* x2 = reinterpret_cast<const float2*>(x + tb);
+  float2* xp2 = reinterpret_cast<float2*>(xp + tb);
+  const float2* g2 = reinterpret_cast<const float2*>(g + tb);
+  float2* gp2 = reinterpret_cast<float2*>(gp + tb);
+  const size_t n2 = n / 2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+    const float2 xv = x2[i], xpv = xp2[i], gv = g2[i], gpv = gp2[i];
+    const float2 s = make_float2(xv.x - xpv.x, xv.y - xpv.y);
+    const float2 y = make_float2(gv.x - gpv.x, gv.y - gpv.y);
     sn[i] = s;
     yn[i] = y;
+    if (shift) {
+      xp2[i] = xv;
+      gp2[i] = gv;
+    }
 #pragma unroll
-    for (int j = 0; j < kMaxHist; ++j) {
+    for (int j = 0; j < MB; ++j) {
       if (j < m) {
-        const double sj = sp[j][i], yj = yp[j][i];
-        acc[4 * j + 0] += sj * (double)y;
-        acc[4 * j + 1] += yj * (double)y;
-        acc[4 * j + 2] += sj * (double)gv;
-        acc[4 * j + 3] += yj * (double)gv;
+        const float2 sj = sp[j][i], yj = yp[j][i];
+        acc[4 * j + 0] += dot2(sj, y);
+        acc[4 * j + 1] += dot2(yj, y);
+        acc[4 * j + 2] += dot2(sj, gv);
+        acc[4 * j + 3] += dot2(yj, gv);
       }
     }
-    acc[4 * kMaxHist + 0] += (double)s * (double)y;
-    acc[4 * kMaxHist + 1] += (double)y * (double)y;
-    acc[4 * kMaxHist + 2] += (double)s * (double)gv;
-    acc[4 * kMaxHist + 3] += (double)y * (double)gv;
+    acc[4 * MB + 0] += dot2(s, y);
+    acc[4 * MB + 1] += dot2(y, y);
+    acc[4 * MB + 2] += dot2(s, gv);
+    acc[4 * MB + 3] += dot2(y, gv);
   }
-  block_reduce_store<kPairDots>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x) * kPairDots);
+  block_reduce_store<NV>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x) * NV);
 }
 
-void launch_lbfgs_pair(const float* x, const float* xp, const float* g, const float* gp, float* hs, float* hy,
-                       const int* meta, size_t n, int batch, double* partials, hipStream_t s) {
-  hipLaunchKernelGGL(k_lbfgs_pair, dim3(kOptBlocks, batch), dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, meta, n,
-                     partials);
-}
-
-// d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m active slots, plus the
-// FP64 partial of g . d.  coef layout per tile: [0] cg, [1+j] cs_j, [1+kMaxHist+j] cy_j.
+// d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB active slots,
+// plus the FP64 partial of g . d.  coef layout per tile: [0] cg, [1+j] cs_j,
+// [1+kMaxHist+j] cy_j.
+template <int MB>
 __global__ void __launch_bounds__(kOptThreads)
 k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, const float* __restrict__ hy,
                   const int* __restrict__ meta, const double* __restrict__ coef, size_t n, float* __restrict__ d,
@@ -160,48 +175,97 @@ k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, con
   const float* hs_b = hs + (size_t)b * hist;
   const float* hy_b = hy + (size_t)b * hist;
   const float cg = (float)cb[0];
-  float cs[kMaxHist], cy[kMaxHist];
-  const float* sp[kMaxHist];
-  const float* yp[kMaxHist];
+  float cs[MB > 0 ? MB : 1], cy[MB > 0 ? MB : 1];
+  const float2* sp[MB > 0 ? MB : 1];
+  const float2* yp[MB > 0 ? MB : 1];
 #pragma unroll
-  for (int j = 0; j < kMaxHist; ++j) {
+  for (int j = 0; j < MB; ++j) {
     const int sl = j < m ? slots[j] : 0;
     cs[j] = j < m ? (float)cb[1 + j] : 0.f;
     cy[j] = j < m ? (float)cb[1 + kMaxHist + j] : 0.f;
-    sp[j] = hs_b + (size_t)sl * n;
-    yp[j] = hy_b + (size_t)sl * n;
+    sp[j] = reinterpret_cast<const float2*>(hs_b + (size_t)sl * n);
+    yp[j] = reinterpret_cast<const float2*>(hy_b + (size_t)sl * n);
   }
+  const float2* g2 = reinterpret_cast<const float2*>(g + tb);
+  float2* d2 = reinterpret_cast<float2*>(d + tb);
   double acc[1] = {0.0};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const float gv = g[tb + i];
-    float v = cg * gv;
+  const size_t n2 = n / 2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+    const float2 gv = g2[i];
+    float2 v = make_float2(cg * gv.x, cg * gv.y);
 #pragma unroll
-    for (int j = 0; j < kMaxHist; ++j)
-      if (j < m) v = fmaf(cs[j], sp[j][i], fmaf(cy[j], yp[j][i], v));
-    d[tb + i] = v;
-    acc[0] += (double)gv * (double)v;
+    for (int j = 0; j < MB; ++j) {
+      if (j < m) {
+        const float2 sj = sp[j][i], yj = yp[j][i];
+        v.x = fmaf(cs[j], sj.x, fmaf(cy[j], yj.x, v.x));
+        v.y = fmaf(cs[j], sj.y, fmaf(cy[j], yj.y, v.y));
+      }
+    }
+    d2[i] = v;
+    acc[0] += dot2(gv, v);
   }
   block_reduce_store<1>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x));
 }
 
+namespace {
+
+using PairLaunch = void (*)(dim3, hipStream_t, const float*, float*, const float*, float*, float*, float*, const int*,
+                            size_t, int, double*);
+using DirLaunch = void (*)(dim3, hipStream_t, const float*, const float*, const float*, const int*, const double*,
+                           size_t, float*, double*);
+
+template <int MB>
+void pair_launch(dim3 grid, hipStream_t s, const float* x, float* xp, const float* g, float* gp, float* hs, float* hy,
+                 const int* meta, size_t n, int shift, double* partials) {
+  hipLaunchKernelGGL(k_lbfgs_pair<MB>, grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, meta, n, shift, partials);
+}
+template <int MB>
+void dir_launch(dim3 grid, hipStream_t s, const float* g, const float* hs, const float* hy, const int* meta,
+                const double* coef, size_t n, float* d, double* partials) {
+  hipLaunchKernelGGL(k_lbfgs_direction<MB>, grid, dim3(kOptThreads), 0, s, g, hs, hy, meta, coef, n, d, partials);
+}
+template <int... I>
+constexpr PairLaunch pair_table_entry(int m, std::integer_sequence<int, I...>) {
+  constexpr PairLaunch t[] = {&pair_launch<I>...};
+  return t[m];
+}
+template <int... I>
+constexpr DirLaunch dir_table_entry(int m, std::integer_sequence<int, I...>) {
+  constexpr DirLaunch t[] = {&dir_launch<I>...};
+  return t[m];
+}
+
+}  // namespace
+
+void launch_lbfgs_pair(const float* x, float* xp, const float* g, float* gp, float* hs, float* hy, const int* meta,
+                       size_t n, int batch, int max_m, bool shift, double* partials, hipStream_t s) {
+  pair_table_entry(max_m, std::make_integer_sequence<int, kMaxHist + 1>{})(
+      dim3(kOptBlocks, batch), s, x, xp, g, gp, hs, hy, meta, n, shift ? 1 : 0, partials);
+}
+
 void launch_lbfgs_direction(const float* g, const float* hs, const float* hy, const int* meta, const double* coef,
-                            size_t n, float* d, int batch, double* partials, hipStream_t s) {
-  hipLaunchKernelGGL(k_lbfgs_direction, dim3(kOptBlocks, batch), dim3(kOptThreads), 0, s, g, hs, hy, meta, coef, n,
-                     d, partials);
+                            size_t n, float* d, int batch, int max_m, double* partials, hipStream_t s) {
+  dir_table_entry(max_m, std::make_integer_sequence<int, kMaxHist + 1>{})(dim3(kOptBlocks, batch), s, g, hs, hy,
+                                                                          meta, coef, n, d, partials);
 }
 
-// out[b*nv + v] = sum_blk partials[(b*nblk + blk)*nv + v], fixed order
-__global__ void k_reduce_vec(const double* __restrict__ partials, int nblk, int nv, double* __restrict__ out) {
-  const int b = blockIdx.x;
-  for (int v = threadIdx.x; v < nv; v += blockDim.x) {
-    double s = 0.0;
-    for (int i = 0; i < nblk; ++i) s += partials[((size_t)b * nblk + i) * nv + v];
-    out[(size_t)b * nv + v] = s;
-  }
+// One block per (v, tile): thread t sums blocks t, t + 256, ... in order, then a
+// fixed shuffle tree and a fixed sum over the four waves.
+__global__ void __launch_bounds__(256)
+k_reduce_fixed(const double* __restrict__ partials, int nblk, int nv, double* __restrict__ out, int out_stride) {
+  const int v = blockIdx.x, b = blockIdx.y;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) s += partials[((size_t)b * nblk + i) * nv + v];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[(size_t)b * out_stride + v] = ((w[0] + w[1]) + w[2]) + w[3];
 }
 
-void launch_reduce_vec(const double* partials, int nblk, int nv, double* out, int batch, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_vec, dim3(batch), dim3(64), 0, s, partials, nblk, nv, out);
+void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_fixed, dim3(nv, batch), dim3(256), 0, s, partials, nblk, nv, out, out_stride);
 }
 
 }  // namespace litho

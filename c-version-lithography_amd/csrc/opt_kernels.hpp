@@ -8,7 +8,7 @@
 namespace litho {
 
 constexpr int kOptThreads = 256;
-constexpr int kOptBlocks = 512;          // grid-stride blocks per tile
+constexpr int kOptBlocks = 1024;         // grid-stride blocks per tile of the L-BFGS vector kernels
 constexpr int kMaxHist = 12;             // largest supported L-BFGS history m
 constexpr int kPairDots = 4 * (kMaxHist + 1);
 constexpr int kMetaStride = 2 + kMaxHist;   // [m, new_slot, slots[kMaxHist]]
@@ -20,10 +20,16 @@ void launch_scale(const float* x, float* y, const float* coef, const int* active
 void launch_axpy(const float* x, float* y, const float* coef, const int* active, size_t n, int batch,
                  hipStream_t s);
 void launch_mask_from_theta(const float* theta, float* mask, size_t count, hipStream_t s);
-void launch_lbfgs_pair(const float* x, const float* xp, const float* g, const float* gp, float* hs, float* hy,
-                       const int* meta, size_t n, int batch, double* partials, hipStream_t s);
+// max_m: the largest active history length m over the batch (selects the
+// instantiation; tiles with a smaller m skip the unused slots).  When shift is
+// set the kernel also stores xp = x and gp = g after forming the new pair.
+void launch_lbfgs_pair(const float* x, float* xp, const float* g, float* gp, float* hs, float* hy, const int* meta,
+                       size_t n, int batch, int max_m, bool shift, double* partials, hipStream_t s);
 void launch_lbfgs_direction(const float* g, const float* hs, const float* hy, const int* meta, const double* coef,
-                            size_t n, float* d, int batch, double* partials, hipStream_t s);
-void launch_reduce_vec(const double* partials, int nblk, int nv, double* out, int batch, hipStream_t s);
+                            size_t n, float* d, int batch, int max_m, double* partials, hipStream_t s);
+// out[b * out_stride + v] = sum over blk of partials[(b * nblk + blk) * nv + v] for
+// v < nv, summed in a fixed tree order (deterministic, independent of scheduling).
+void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,
+                         hipStream_t s);
 
 }  // namespace litho

@@ -26,11 +26,12 @@
 //   k_row_adj        (k, window row): forward along x -> Y_k = conj(H~_k) V^
 //   k_final_row_inv  window rows: sum_k Y_k, inverse along x -> P
 //   k_final_col_inv  columns: inverse along y -> G_M, theta chain, norms
-//   k_reduce         fixed-order FP64 sums of the per-block partials
+//   (per-block FP64 partials are summed by launch_reduce_fixed, opt_kernels.hip)
 #include <hip/hip_runtime.h>
 
 #include "fft_columns.hpp"
 #include "fft_engine.hpp"
+#include "opt_kernels.hpp"
 #include "socs_kernels.hpp"
 
 namespace litho {
@@ -641,18 +642,6 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
   }
 }
 
-// ---------------------------------------------------------------- reduce
-// out[b*stride + j] = sum_blk partials[(b*nblk + blk)*2 + j], fixed order.
-__global__ void k_reduce_pairs(const double* __restrict__ partials, int nblk, double* __restrict__ out,
-                               int out_stride) {
-  const int b = blockIdx.x;
-  const int j = threadIdx.x;  // 0 or 1
-  if (j >= 2) return;
-  double s = 0.0;
-  for (int i = 0; i < nblk; ++i) s += partials[((size_t)b * nblk + i) * 2 + j];
-  out[(size_t)b * out_stride + j] = s;
-}
-
 // ---------------------------------------------------------------- launchers
 template <int N>
 struct SocsImpl {
@@ -700,7 +689,7 @@ struct SocsImpl {
 };
 
 void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_pairs, dim3(batch), dim3(64), 0, s, partials, nblk, out, out_stride);
+  launch_reduce_fixed(partials, nblk, 2, out, out_stride, batch, s);
 }
 
 template <int N>
