@@ -56,6 +56,22 @@ def column_kernel_bytes(n, k):
     return 16 * k * n * n + 4 * n * n
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "k_col_image_adj_pruned_2048_hbm_traffic.md"
+TRAFFIC_KEY = "traffic_bytes_per_launch k_col_image_adj_pruned<2048>:"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the column kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith(TRAFFIC_KEY):
+                return int(line[len(TRAFFIC_KEY):].strip())
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(kernels, target, seconds=20.0):
     from oracle import imaging as im
     from oracle import ctm as oc
@@ -148,9 +164,10 @@ def main():
         "config": {"workload": "single-tile CTM L-BFGS, 2048x2048 @4nm, K=24 (219x219 window), m=10, "
                                "50 iterations, tol 1e-2, seeded Manhattan target",
                    "n": N, "k": K, "window": W, "evals_per_step": round(evals / args.steps, 2)},
-        "roofline": {"bound": "hbm", "kernel": "k_col_image_adj<2048>", "achieved": round(col_bytes / col_s / 1e9, 1),
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(col_bytes / col_s / HBM_PEAK, 4),
-                     "traffic": None, "launch_ms": round(st[1], 4)},
+        "roofline": {"bound": "hbm", "kernel": "k_col_image_adj_pruned<2048>",
+                     "achieved": round(col_bytes / col_s / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(col_bytes / col_s / HBM_PEAK, 4), "traffic": measured_traffic(),
+                     "algorithmic_bytes": col_bytes, "launch_ms": round(st[1], 4)},
         "eval_roofline": {"algorithmic_bytes": eval_bytes, "eval_ms": round(st[3], 4),
                           "achieved_GBps": round(eval_bytes / (st[3] * 1e-3) / 1e9, 1),
                           "frac": round(eval_bytes / (st[3] * 1e-3) / HBM_PEAK, 4)},
