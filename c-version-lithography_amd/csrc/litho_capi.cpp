@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "ctm_driver.hpp"
+#include "lsm.hpp"
 #include "optics.hpp"
 #include "plan.hpp"
 
@@ -51,6 +52,12 @@ litho::Plan& plan_of(litho_plan* p) {
 }
 
 void set_device(const litho::Plan& p) { LITHO_HIP(hipSetDevice(p.ctx->device)); }
+
+litho::Context& ctx_of(litho_ctx* c) {
+  require(c != nullptr, "null context");
+  LITHO_HIP(hipSetDevice(c->c.device));
+  return c->c;
+}
 
 }  // namespace
 
@@ -284,6 +291,44 @@ int litho_initialize_theta(litho_ctx* ctx, const float* pattern, float* theta, s
     LITHO_HIP(hipSetDevice(ctx->c.device));
     litho::launch_init_theta(pattern, theta, count, ctx->c.stream);
     LITHO_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int litho_ls_terms(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, const float* vn, double dx,
+                   double dy, double curvature_weight, float* delta, float* kappa) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    litho::ls_terms(c, batch, rows, cols, phi, vn, dx, dy, curvature_weight, delta, kappa);
+    LITHO_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+double litho_ls_dt(double cfl, double dx, double dy, double max_h1, double max_h2, double curvature_weight) {
+  return litho::ls_dt(cfl, dx, dy, max_h1, max_h2, curvature_weight);
+}
+
+int litho_ls_reinit_sdf(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, double dx, double dy,
+                        float* sdf) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    litho::ls_reinit_sdf(c, batch, rows, cols, phi, dx, dy, sdf);
+    LITHO_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int litho_lsm_step(litho_ctx* ctx, int batch, int n, const float* phi, const float* mask_gradient,
+                   const int* update_rects, const litho_lsm_options* options, int reinit, float* phi_out,
+                   float* mask_out, double* dt_out) {
+  return guarded([&] {
+    require(options != nullptr, "null options");
+    litho::Context& c = ctx_of(ctx);
+    litho::LsmOptions o;
+    o.dx = options->dx;
+    o.dy = options->dy;
+    o.cfl = options->cfl;
+    o.curvature_weight = options->curvature_weight;
+    litho::lsm_step(c, batch, n, phi, mask_gradient, update_rects, o, reinit != 0, phi_out, mask_out, dt_out);
+    LITHO_HIP(hipStreamSynchronize(c.stream));
   });
 }
 

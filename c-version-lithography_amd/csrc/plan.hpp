@@ -75,6 +75,16 @@ struct HostPinned {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  DevMem scratch;        // grow-only workspace of the level-set and tiling calls
+  // At least `bytes` of workspace.  Growing it waits for the stream first, so
+  // work still queued on the old buffer finishes before it is freed.
+  void* workspace(size_t bytes) {
+    if (scratch.bytes < bytes) {
+      if (stream != nullptr) LITHO_HIP(hipStreamSynchronize(stream));
+      scratch.alloc(bytes);
+    }
+    return scratch.ptr;
+  }
 };
 
 // Device state of one tile size: scaled kernels, tables and the workspace of

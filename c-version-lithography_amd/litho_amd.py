@@ -29,6 +29,10 @@ class CtmOptions(C.Structure):
                 ("threshold", C.c_double), ("alpha", C.c_double)]
 
 
+class LsmOptions(C.Structure):
+    _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("cfl", C.c_double), ("curvature_weight", C.c_double)]
+
+
 class CtmInfo(C.Structure):
     _fields_ = [("evaluations", C.c_int), ("iterations", C.c_int), ("final_pe", C.c_double),
                 ("final_grad_norm", C.c_double), ("status", C.c_int)]
@@ -59,6 +63,10 @@ SIGNATURES = {
     "litho_ctm_optimize": (_I, [_P, _P, _P, C.POINTER(CtmOptions), _P, _P, _DP, _I, C.POINTER(CtmInfo)]),
     "litho_initialize_theta": (_I, [_P, _P, _P, C.c_size_t]),
     "litho_profile_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _I, _DP]),
+    "litho_ls_terms": (_I, [_P, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P]),
+    "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
+    "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
+    "litho_lsm_step": (_I, [_P, _I, _I, _P, _P, _IP, C.POINTER(LsmOptions), _I, _P, _P, _DP]),
 }
 
 _lib = None
@@ -134,6 +142,43 @@ class Context:
         out = dict(win)
         out.update(kernels=kern[..., 0] + 1j * kern[..., 1], weights=w, singular_values=sv)
         return out
+
+
+    # ------------------------------------------------------------ level set
+    def ls_terms(self, phi, vn=None, delta=None, kappa=None, dx=1.0, dy=1.0, curvature_weight=0.01):
+        """evolve_normal_WENO_godunov delta and evolve_kappa on (batch, rows, cols) float32 tensors."""
+        b, r, c = _shape3(phi)
+        check(self.lib.litho_ls_terms(self.h, b, r, c, ptr(phi), ptr(vn), dx, dy, curvature_weight, ptr(delta),
+                                      ptr(kappa)))
+
+    def reinit_sdf(self, phi, sdf, dx=1.0, dy=1.0):
+        b, r, c = _shape3(phi)
+        check(self.lib.litho_ls_reinit_sdf(self.h, b, r, c, ptr(phi), dx, dy, ptr(sdf)))
+
+    def lsm_step(self, phi, grad, rects, phi_out, mask_out=None, dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01,
+                 reinit=False):
+        """One RK2 update of (batch, n, n) tiles; rects: (batch, 4) y0, y1, x0, x1.  Returns dt per tile."""
+        b, r, c = _shape3(phi)
+        if r != c:
+            raise ValueError("lsm_step tiles are square")
+        rc = np.ascontiguousarray(rects, dtype=np.int32).reshape(b, 4)
+        dt = np.zeros(b)
+        o = LsmOptions(dx, dy, cfl, curvature_weight)
+        check(self.lib.litho_lsm_step(self.h, b, r, ptr(phi), ptr(grad), rc.ctypes.data_as(_IP), C.byref(o),
+                                      1 if reinit else 0, ptr(phi_out), ptr(mask_out), _dp(dt)))
+        return dt
+
+
+def _shape3(t):
+    if t.dim() == 2:
+        return 1, t.shape[0], t.shape[1]
+    if t.dim() == 3:
+        return t.shape[0], t.shape[1], t.shape[2]
+    raise ValueError("expected (rows, cols) or (batch, rows, cols)")
+
+
+def ls_dt(cfl, dx, dy, max_h1, max_h2, curvature_weight) -> float:
+    return float(load().litho_ls_dt(cfl, dx, dy, max_h1, max_h2, curvature_weight))
 
 
 def socs_window(n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out) -> dict:

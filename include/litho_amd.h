@@ -149,6 +149,49 @@ int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, cons
  * pattern > 0, else 0.7 pi.  Device arrays of n*n floats. */
 int litho_initialize_theta(litho_ctx* ctx, const float* pattern, float* theta, size_t count);
 
+/* ---------------------------------------------------------------- level set
+ * Fields are DEVICE float32 arrays of `batch` row-major rows x cols tiles
+ * (rows, cols >= 3; cols <= 4096); stencil arithmetic is FP64. */
+
+/* LevelSetUtils::evolve_normal_WENO_godunov (level_set_utils.cpp:150-225) and
+ * evolve_kappa (level_set_utils.cpp:257-278): delta = Vn |grad phi| with WENO5-JS
+ * one-sided derivatives and Godunov upwinding (edge padding 3), kappa =
+ * curvature_weight (phi_xx phi_y^2 - 2 phi_x phi_y phi_xy + phi_yy phi_x^2) /
+ * |grad phi|^2 (0 where |grad phi|^2 <= DBL_EPSILON).  H1 = H2 = |Vn| are not
+ * returned.  delta or kappa may be null (vn may then be null too). */
+int litho_ls_terms(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, const float* vn, double dx,
+                   double dy, double curvature_weight, float* delta, float* kappa);
+
+/* LevelSetUtils::get_dt_normal_kappa (level_set_utils.cpp:280-288), host:
+ * cfl / (max_h1/dx + max_h2/dy + 2 b (1/dx^2 + 1/dy^2)), or 1e-3 when that
+ * denominator is below 1e-9. */
+double litho_ls_dt(double cfl, double dx, double dy, double max_h1, double max_h2, double curvature_weight);
+
+/* LevelSetUtils::reinit_sdf (level_set_utils.cpp:290-319): inside = phi > 0,
+ * sdf = (dist_in - dist_out) dx with OpenCV distanceTransform(DIST_L2, 5)
+ * semantics (5x5 chamfer, 16-bit fixed point).  sdf may alias phi. */
+int litho_ls_reinit_sdf(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, double dx, double dy,
+                        float* sdf);
+
+typedef struct {
+  double dx, dy;             /* FullChipLSMOptions dx / dy (full_chip_lsm.h:8-66) */
+  double cfl;                /* CFL number */
+  double curvature_weight;   /* b */
+} litho_lsm_options;
+
+/* One RK2 level-set update of `batch` n x n Micros, the loop body of
+ * FullChipLSMOptimizer::optimize (full_chip_lsm.cpp:208-242):
+ *   Vn = mask_gradient inside the tile's update window (Core + Guard), 0 outside;
+ *   N(phi) = kappa(phi) - Vn |grad phi|;  dt from stage 1;
+ *   phi1 = phi + dt N(phi) (window);  phi' = phi/2 + (phi1 + dt N(phi1))/2 (window);
+ *   with reinit != 0 the window is re-initialised to reinit_sdf((phi' >= 0) - 0.5).
+ * update_rects: HOST array of batch x {y0, y1, x0, x1} (half-open, tile-local).
+ * phi_out may alias phi.  mask_out (nullable): Binary render, phi_out >= 0.
+ * dt_out (HOST, nullable): the step of each tile; passing it waits for the step. */
+int litho_lsm_step(litho_ctx* ctx, int batch, int n, const float* phi, const float* mask_gradient,
+                   const int* update_rects, const litho_lsm_options* options, int reinit, float* phi_out,
+                   float* mask_out, double* dt_out);
+
 #ifdef __cplusplus
 }
 #endif

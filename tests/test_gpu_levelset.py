@@ -1,0 +1,176 @@
+"""GPU parity of the level-set kernels against the FP64 oracle, through the C ABI.
+
+Inputs are float32 on the device; the oracle runs in FP64 on the same float32
+values.  Tolerances: delta / kappa / RK2 update absolute error <= 1e-5 x the
+field's max magnitude (float32 storage, FP64 stencils on both sides); the
+chamfer re-initialisation is integer arithmetic and must match bit for bit
+(after float32 rounding of the FP64 combination); Binary masks must match
+exactly except where the oracle's phi lies within 1e-5 of the threshold.
+"""
+import numpy as np
+import pytest
+
+from conftest import litho_amd
+from oracle import fullchip as fc
+from oracle import levelset as ls
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def host(t):
+    return t.detach().cpu().double().numpy()
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def smooth_phi(n, seed, m=None):
+    """Signed distance of a few discs (positive inside) plus a smooth ripple."""
+    rng = np.random.default_rng(seed)
+    m = m or n
+    y, x = np.mgrid[0:n, 0:m].astype(np.float64)
+    phi = np.full((n, m), -1e9)
+    for _ in range(4):
+        cy, cx = rng.uniform(0.2, 0.8) * n, rng.uniform(0.2, 0.8) * m
+        r = rng.uniform(0.08, 0.25) * min(n, m)
+        phi = np.maximum(phi, r - np.hypot(y - cy, x - cx))
+    return phi + 0.3 * np.sin(x / 7.0) * np.cos(y / 5.0)
+
+
+def manhattan(n, seed):
+    rng = np.random.default_rng(seed)
+    m = np.zeros((n, n))
+    for _ in range(max(3, n // 16)):
+        h, w = rng.integers(4, max(5, n // 4), size=2)
+        y0, x0 = rng.integers(0, n - h), rng.integers(0, n - w)
+        m[y0:y0 + h, x0:x0 + w] = 1.0
+    return m
+
+
+@pytest.mark.parametrize("n,m,seed", [(32, 32, 1), (64, 48, 2), (640, 640, 3)])
+def test_ls_terms_match_oracle(gpu_ctx, n, m, seed):
+    import torch
+    phi = f32(smooth_phi(n, seed, m))
+    rng = np.random.default_rng(seed + 100)
+    vn = f32(rng.standard_normal((n, m)))
+    vn[rng.random((n, m)) < 0.2] = 0.0
+    dx, dy, b = 1.0, 1.3, 0.01
+    d_delta = torch.zeros((1, n, m), dtype=torch.float32, device="cuda")
+    d_kappa = torch.zeros_like(d_delta)
+    print(f"case ls_terms {n}x{m} seed {seed}", flush=True)
+    gpu_ctx.ls_terms(dev(phi)[None], dev(vn)[None], d_delta, d_kappa, dx=dx, dy=dy, curvature_weight=b)
+    delta_ref, h1, h2 = ls.evolve_normal_weno_godunov(phi, vn, dy, dx)
+    kappa_ref = ls.evolve_kappa(phi, dx, dy, b)
+    ed = np.max(np.abs(host(d_delta)[0] - delta_ref)) / np.max(np.abs(delta_ref))
+    ek = np.max(np.abs(host(d_kappa)[0] - kappa_ref)) / np.max(np.abs(kappa_ref))
+    print(f"ls_terms {n}x{m}: delta rel Linf {ed:.2e}, kappa rel Linf {ek:.2e}")
+    assert ed <= 1e-5 and ek <= 1e-5
+
+
+def test_ls_terms_plane_closed_form(gpu_ctx):
+    """A plane phi = 2x (reference test_level_set_godunov.cpp: |grad phi| = 2 for
+    either sign of Vn, kappa = 0)."""
+    import torch
+    n = 24
+    x = np.tile(np.arange(n, dtype=np.float64), (n, 1))
+    phi = 2.0 * x
+    vn = np.where(np.arange(n)[:, None] % 2 == 0, 1.0, -1.0) * np.ones((n, n))
+    d_delta = torch.zeros((1, n, n), dtype=torch.float32, device="cuda")
+    d_kappa = torch.zeros_like(d_delta)
+    gpu_ctx.ls_terms(dev(phi)[None], dev(vn)[None], d_delta, d_kappa, curvature_weight=0.01)
+    # interior: the 7-point WENO stencil sees no clamped edge pixel
+    np.testing.assert_allclose(host(d_delta)[0][:, 3:-3], 2.0 * vn[:, 3:-3], rtol=0, atol=1e-6)
+    assert np.max(np.abs(host(d_kappa)[0])) == 0.0
+
+
+def test_ls_dt_matches_oracle():
+    la = litho_amd()
+    for h1, h2 in [(0.0, 0.0), (0.5, 0.5), (3.0, 3.0)]:
+        for b in (0.0, 0.01):
+            assert la.ls_dt(0.7, 1.0, 1.0, h1, h2, b) == ls.get_dt_normal_kappa(0.7, 1.0, 1.0, [h1], [h2], b)
+
+
+@pytest.mark.parametrize("n,m,seed", [(40, 40, 4), (96, 72, 5), (640, 640, 6)])
+def test_reinit_sdf_bit_exact(gpu_ctx, n, m, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    phi = f32(smooth_phi(n, seed, m))
+    phi[rng.random((n, m)) < 0.01] *= -1.0          # isolated sign flips: both transforms busy
+    out = torch.zeros((1, n, m), dtype=torch.float32, device="cuda")
+    print(f"case reinit_sdf {n}x{m} seed {seed}", flush=True)
+    gpu_ctx.reinit_sdf(dev(phi)[None], out, dx=0.5, dy=0.5)
+    ref = np.float32(ls.reinit_sdf(phi, 0.5, 0.5))
+    got = out.cpu().numpy()[0]
+    bad = int(np.count_nonzero(got != ref))
+    print(f"reinit_sdf {n}x{m}: {bad} mismatches")
+    assert bad == 0
+
+
+def test_reinit_sdf_all_inside_and_all_outside(gpu_ctx):
+    """Edge cases: no zero pixel for one transform (distances saturate at DIST_MAX)."""
+    import torch
+    n = 16
+    for sign in (1.0, -1.0):
+        phi = np.full((n, n), sign)
+        out = torch.zeros((1, n, n), dtype=torch.float32, device="cuda")
+        gpu_ctx.reinit_sdf(dev(phi)[None], out)
+        ref = np.float32(ls.reinit_sdf(phi, 1.0, 1.0))
+        assert np.array_equal(out.cpu().numpy()[0], ref)
+
+
+def _tile(n, rect):
+    r = fc.Rect(*rect)
+    return fc.Tile(0, 0, 0, 0, 0, r, fc.Rect(0, n, 0, n), r, r)
+
+
+@pytest.mark.parametrize("n,reinit", [(64, False), (64, True), (640, False), (640, True)])
+def test_lsm_step_matches_oracle(gpu_ctx, n, reinit):
+    import torch
+    rects = [(6, n - 6, 4, n - 8), (0, n, 0, n)]
+    phis, grads = [], []
+    for b in range(2):
+        m = manhattan(n, 10 + b)
+        phis.append(f32(ls.reinit_sdf(m - 0.5, 1.0, 1.0)))
+        rng = np.random.default_rng(20 + b)
+        grads.append(f32(0.05 * rng.standard_normal((n, n))))
+    d_phi = dev(np.stack(phis))
+    d_grad = dev(np.stack(grads))
+    out = torch.zeros_like(d_phi)
+    mask = torch.zeros_like(d_phi)
+    lo = fc.LSMOptions(iterations=1, dx=1.0, dy=1.0, cfl=0.5, curvature_weight=0.01,
+                       reinit_interval=1 if reinit else 0)
+    print(f"case lsm_step n={n} reinit={reinit}", flush=True)
+    dt = gpu_ctx.lsm_step(d_phi, d_grad, rects, out, mask, dx=1.0, dy=1.0, cfl=0.5, curvature_weight=0.01,
+                          reinit=reinit)
+    got_phi, got_mask = host(out), host(mask)
+    for b in range(2):
+        ref_phi, ref_mask, ref_dt = fc.lsm_update_tile(phis[b], grads[b], _tile(n, rects[b]), n, lo, 1)
+        assert abs(dt[b] - ref_dt) <= 1e-12 * ref_dt
+        err = np.max(np.abs(got_phi[b] - ref_phi)) / np.max(np.abs(ref_phi))
+        # pixels whose oracle value sits at the threshold may round either way
+        if reinit:
+            pre, _, _ = fc.lsm_update_tile(phis[b], grads[b], _tile(n, rects[b]), n,
+                                          fc.LSMOptions(**{**lo.__dict__, "reinit_interval": 0}), 1)
+            near = np.abs(pre) < 1e-5
+        else:
+            near = np.abs(ref_phi) < 1e-5
+        mism = int(np.count_nonzero((got_mask[b] != ref_mask) & ~near))
+        print(f"lsm_step n={n} reinit={reinit} tile {b}: phi rel Linf {err:.2e}, dt {dt[b]:.6f}, "
+              f"mask mismatches {mism} (near-threshold pixels {int(near.sum())})")
+        assert err <= 1e-5
+        assert mism == 0
+
+
+def test_lsm_step_rejects_bad_window(gpu_ctx):
+    import torch
+    la = litho_amd()
+    n = 16
+    phi = torch.zeros((1, n, n), dtype=torch.float32, device="cuda")
+    with pytest.raises(la.LithoError):
+        gpu_ctx.lsm_step(phi, phi, [(0, n + 1, 0, n)], phi)
