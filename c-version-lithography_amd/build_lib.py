@@ -24,8 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
-SOURCES = ["socs_kernels.hip", "opt_kernels.hip", "precompute.hip", "levelset.hip", "plan.cpp", "optics.cpp",
-           "ctm_driver.cpp", "lsm.cpp", "litho_capi.cpp"]
+SOURCES = ["socs_kernels.hip", "opt_kernels.hip", "precompute.hip", "levelset.hip", "tiling.hip", "plan.cpp",
+           "optics.cpp", "ctm_driver.cpp", "lsm.cpp", "fullchip.cpp", "litho_capi.cpp"]
 
 
 def _compile(src: Path, obj: Path) -> tuple[str, int, str]:

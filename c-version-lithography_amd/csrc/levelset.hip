@@ -264,8 +264,8 @@ __device__ long long block_min_scan(long long v, long long* sh, bool reverse) {
 // to the inside, 1: of inside pixels to the outside.  Thread t owns columns
 // [t * per, (t + 1) * per), per <= kChamferPer.
 __global__ void __launch_bounds__(kChamferThreads)
-k_chamfer(const float* __restrict__ phi, int rows, int cols, int inclusive, uint32_t* __restrict__ scratch,
-          float* __restrict__ dist) {
+k_chamfer(const float* __restrict__ phi, int rows, int cols, float level, int inclusive,
+          uint32_t* __restrict__ scratch, float* __restrict__ dist) {
   __shared__ uint32_t ring[3][kChamferMaxCols];
   __shared__ long long sh[kChamferThreads];
   const int t = blockIdx.x, which = blockIdx.y;
@@ -290,7 +290,7 @@ k_chamfer(const float* __restrict__ phi, int rows, int cols, int inclusive, uint
       w[e] = kInit * 4;
       if (j < j1) {
         const float v = ph[(size_t)i * cols + j];
-        const bool inside = inclusive ? v >= 0.f : v > 0.f;
+        const bool inside = inclusive ? v >= level : v > level;
         const bool src = which == 0 ? !inside : inside;
         long long t0 = 0;
         if (src) {
@@ -381,11 +381,11 @@ size_t chamfer_scratch_bytes(int batch, int rows, int cols) {
   return (size_t)batch * 2 * rows * cols * (sizeof(uint32_t) + sizeof(float));
 }
 
-void launch_reinit_sdf(const float* phi, int batch, int rows, int cols, double dx, double dy, bool inclusive,
+void launch_reinit_sdf(const float* phi, int batch, int rows, int cols, double dx, double dy, float level, bool inclusive,
                        uint32_t* scratch, float* dist, const int* rects, float* sdf, float* mask_out, hipStream_t s) {
   (void)dy;   // the reference scales both distances by dx (level_set_utils.cpp:306-316)
-  hipLaunchKernelGGL(k_chamfer, dim3(batch, 2), dim3(kChamferThreads), 0, s, phi, rows, cols, inclusive ? 1 : 0,
-                     scratch, dist);
+  hipLaunchKernelGGL(k_chamfer, dim3(batch, 2), dim3(kChamferThreads), 0, s, phi, rows, cols, level,
+                     inclusive ? 1 : 0, scratch, dist);
   const size_t plane = (size_t)rows * cols;
   const int blocks = (int)((plane + 255) / 256 < 1024 ? (plane + 255) / 256 : 1024);
   hipLaunchKernelGGL(k_sdf_combine, dim3(blocks, batch), dim3(256), 0, s, phi, dist, rows, cols, dx, rects, sdf,

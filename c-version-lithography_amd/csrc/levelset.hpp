@@ -34,13 +34,13 @@ void launch_lsm_stage(int stage, const float* phi_in, const float* base, const f
                       hipStream_t s);
 
 // Chamfer signed distance (level_set_utils.cpp:290-319 over OpenCV's
-// distanceTransform(DIST_L2, 5)): inside = phi > 0 (or phi >= 0 when
+// distanceTransform(DIST_L2, 5)): inside = phi > level (or phi >= level when
 // inclusive), sdf = dist_in dx - dist_out dx with the 5x5 chamfer distances in
 // 16-bit fixed point.  With rects (nullable) only the rectangle of each tile is
 // written to sdf (the LSM re-initialisation of the update window); mask_out
 // (nullable) then receives sdf-updated tile >= 0 over the whole tile.
 size_t chamfer_scratch_bytes(int batch, int rows, int cols);
-void launch_reinit_sdf(const float* phi, int batch, int rows, int cols, double dx, double dy, bool inclusive,
+void launch_reinit_sdf(const float* phi, int batch, int rows, int cols, double dx, double dy, float level, bool inclusive,
                        uint32_t* scratch, float* dist /* [batch][2][rows][cols] */, const int* rects, float* sdf,
                        float* mask_out, hipStream_t s);
 

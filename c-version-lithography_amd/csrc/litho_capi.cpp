@@ -8,9 +8,11 @@
 #include <vector>
 
 #include "ctm_driver.hpp"
+#include "fullchip.hpp"
 #include "lsm.hpp"
 #include "optics.hpp"
 #include "plan.hpp"
+#include "tiling.hpp"
 
 struct litho_ctx {
   litho::Context c;
@@ -329,6 +331,101 @@ int litho_lsm_step(litho_ctx* ctx, int batch, int n, const float* phi, const flo
     o.curvature_weight = options->curvature_weight;
     litho::lsm_step(c, batch, n, phi, mask_gradient, update_rects, o, reinit != 0, phi_out, mask_out, dt_out);
     LITHO_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int litho_partition_build(int height, int width, int grid_rows, int grid_cols, int halo, int guard,
+                          int* tile_size, litho_tile* tiles) {
+  return guarded([&] {
+    require(tile_size != nullptr && tiles != nullptr, "null output");
+    std::vector<litho_tile> t;
+    litho::partition_build(height, width, grid_rows, grid_cols, halo, guard, tile_size, t);
+    std::memcpy(tiles, t.data(), t.size() * sizeof(litho_tile));
+  });
+}
+
+int litho_partition_extract(litho_ctx* ctx, const float* global, int height, int width, const litho_tile* tiles,
+                            int num_tiles, int tile_size, float* local_out) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    require(global != nullptr && local_out != nullptr && height > 0 && width > 0 && tile_size > 0, "invalid argument");
+    litho::validate_tiles(tiles, num_tiles, tile_size);
+    const std::vector<int> tab = litho::tile_table(tiles, num_tiles);
+    litho::DevMem d(tab.size() * sizeof(int));
+    LITHO_HIP(hipMemcpyAsync(d.ptr, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    litho::launch_extract(global, height, width, d.as<int>(), num_tiles, tile_size, local_out, c.stream);
+    LITHO_HIP(hipGetLastError());
+    LITHO_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int litho_partition_assemble(litho_ctx* ctx, const float* local, const litho_tile* tiles, int num_tiles,
+                             int tile_size, int height, int width, float* global_out) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    require(local != nullptr && global_out != nullptr && height > 0 && width > 0, "invalid argument");
+    litho::validate_tiles(tiles, num_tiles, tile_size);
+    for (int i = 0; i < num_tiles; ++i) {
+      const litho_rect& r = tiles[i].core_global;
+      require(r.y0 >= 0 && r.x0 >= 0 && r.y1 <= height && r.x1 <= width, "Micro Core outside the chip");
+    }
+    const std::vector<int> tab = litho::tile_table(tiles, num_tiles);
+    litho::DevMem d(tab.size() * sizeof(int));
+    LITHO_HIP(hipMemcpyAsync(d.ptr, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    litho::launch_assemble(local, d.as<int>(), num_tiles, tile_size, width, global_out, tile_size * tile_size,
+                           c.stream);
+    LITHO_HIP(hipGetLastError());
+    LITHO_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int litho_tile_exchange(litho_plan* plan, const litho_tile* tiles, int num_tiles, const float* local_masks,
+                        const float* local_targets, const litho_exchange_options* options, float* mask_gradient,
+                        float* raw_gradient, float* wafer, litho_tile_loss* per_tile, double* totals) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(options != nullptr, "null options");
+    set_device(p);
+    litho::Exchanger ex(p, tiles, num_tiles);
+    ex.prepare(local_targets, *options);
+    ex.run(local_masks, *options, mask_gradient, raw_gradient, wafer, per_tile, totals);
+  });
+}
+
+int litho_fullchip_ctm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                       const float* initial_pattern, const float* global_target, const litho_exchange_options* options,
+                       int iterations, double learning_rate, float* global_mask_out, float* local_masks_out,
+                       double* history, int* best_iteration, double* best_loss) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(options != nullptr, "null options");
+    set_device(p);
+    const litho::FullChipResult r =
+        litho::fullchip_ctm(p, tiles, num_tiles, height, width, initial_pattern, global_target, *options, iterations,
+                            learning_rate, global_mask_out, local_masks_out, history);
+    if (best_iteration != nullptr) *best_iteration = r.best_iteration;
+    if (best_loss != nullptr) *best_loss = r.best_loss;
+  });
+}
+
+int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                       const float* initial_binary, const float* global_target, const litho_exchange_options* options,
+                       const litho_lsm_options* lsm, int iterations, int reinit_interval, float* global_mask_out,
+                       float* global_phi_out, double* history, int* best_iteration, double* best_loss) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(options != nullptr && lsm != nullptr, "null options");
+    set_device(p);
+    litho::LsmOptions lo;
+    lo.dx = lsm->dx;
+    lo.dy = lsm->dy;
+    lo.cfl = lsm->cfl;
+    lo.curvature_weight = lsm->curvature_weight;
+    const litho::FullChipResult r =
+        litho::fullchip_lsm(p, tiles, num_tiles, height, width, initial_binary, global_target, *options, lo,
+                            iterations, reinit_interval, global_mask_out, global_phi_out, history);
+    if (best_iteration != nullptr) *best_iteration = r.best_iteration;
+    if (best_loss != nullptr) *best_loss = r.best_loss;
   });
 }
 

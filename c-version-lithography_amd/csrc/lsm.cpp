@@ -43,7 +43,7 @@ void ls_reinit_sdf(Context& ctx, int batch, int rows, int cols, const float* phi
   char* ws = static_cast<char*>(ctx.workspace(chamfer_scratch_bytes(batch, rows, cols)));
   uint32_t* scratch = reinterpret_cast<uint32_t*>(ws);
   float* dist = reinterpret_cast<float*>(ws + (size_t)batch * 2 * plane * sizeof(uint32_t));
-  launch_reinit_sdf(phi, batch, rows, cols, dx, dy, false, scratch, dist, nullptr, sdf, nullptr, ctx.stream);
+  launch_reinit_sdf(phi, batch, rows, cols, dx, dy, 0.f, false, scratch, dist, nullptr, sdf, nullptr, ctx.stream);
   LITHO_HIP(hipGetLastError());
 }
 
@@ -80,7 +80,7 @@ void lsm_step(Context& ctx, int batch, int n, const float* phi, const float* gra
     uint32_t* scratch = reinterpret_cast<uint32_t*>(ws + off_ch);
     float* dist = reinterpret_cast<float*>(ws + off_ch + (size_t)batch * 2 * plane * sizeof(uint32_t));
     // signs = (phi >= 0) - 0.5: inside means phi_out >= 0 (full_chip_lsm.cpp:236-239)
-    launch_reinit_sdf(phi_out, batch, n, n, o.dx, o.dy, true, scratch, dist, d_rects, phi_out, mask_out, s);
+    launch_reinit_sdf(phi_out, batch, n, n, o.dx, o.dy, 0.f, true, scratch, dist, d_rects, phi_out, mask_out, s);
   }
   LITHO_HIP(hipGetLastError());
   if (dt_out != nullptr) {

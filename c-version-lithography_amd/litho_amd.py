@@ -33,6 +33,51 @@ class LsmOptions(C.Structure):
     _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("cfl", C.c_double), ("curvature_weight", C.c_double)]
 
 
+class Rect(C.Structure):
+    _fields_ = [("y0", C.c_int), ("y1", C.c_int), ("x0", C.c_int), ("x1", C.c_int)]
+
+    def tuple(self):
+        return (self.y0, self.y1, self.x0, self.x1)
+
+
+class Neighbor(C.Structure):
+    _fields_ = [("id", C.c_int), ("direction", C.c_int), ("guard_global", Rect), ("my_guard_local", Rect),
+                ("neighbor_overlap_local", Rect), ("overlap_global", Rect), ("my_overlap_local", Rect),
+                ("neighbor_guard_local", Rect)]
+
+
+class Tile(C.Structure):
+    _fields_ = [("id", C.c_int), ("row", C.c_int), ("col", C.c_int), ("halo", C.c_int), ("guard", C.c_int),
+                ("core_global", Rect), ("tile_global", Rect), ("core_local", Rect), ("update_local", Rect),
+                ("num_neighbors", C.c_int), ("neighbors", Neighbor * 4)]
+
+
+class ExchangeOptions(C.Structure):
+    """litho_exchange_options; defaults follow MicroGradientOptions."""
+    _fields_ = [("loss_type", C.c_int), ("loss_region", C.c_int), ("pe_weight", C.c_double),
+                ("penalty_weight", C.c_double), ("penalty_threshold", C.c_double), ("collect_core_metrics", C.c_int),
+                ("enable_cept", C.c_int), ("cept_guard_only", C.c_int), ("cept_weight", C.c_double),
+                ("enable_grad_fusion", C.c_int), ("fusion_mode", C.c_int), ("fusion_weight_owner", C.c_double),
+                ("fusion_weight_neighbor", C.c_double), ("threshold", C.c_double), ("alpha", C.c_double)]
+
+
+def exchange_options(loss_type="PE", loss_region="Core", pe_weight=1.0, penalty_weight=2.0, penalty_threshold=0.01,
+                     collect_core_metrics=False, enable_cept=False, cept_guard_only=False, cept_weight=1.1,
+                     enable_grad_fusion=False, fusion_mode="Average", fusion_weight_owner=0.5,
+                     fusion_weight_neighbor=0.5, threshold=0.25, alpha=50.0) -> ExchangeOptions:
+    return ExchangeOptions({"PE": 0, "PEPenalty": 1}[loss_type], {"Core": 0, "CoreGuard": 1}[loss_region],
+                           pe_weight, penalty_weight, penalty_threshold, int(collect_core_metrics), int(enable_cept),
+                           int(cept_guard_only), cept_weight, int(enable_grad_fusion),
+                           {"Average": 0, "Linear": 1}[fusion_mode], fusion_weight_owner, fusion_weight_neighbor,
+                           threshold, alpha)
+
+
+class TileLoss(C.Structure):
+    _fields_ = [("loss", C.c_double), ("data_loss", C.c_double), ("pe_loss", C.c_double),
+                ("penalty_loss", C.c_double), ("bg_penalty_raw", C.c_double), ("cept_loss", C.c_double),
+                ("core_pe_loss", C.c_double), ("core_bg_penalty_raw", C.c_double)]
+
+
 class CtmInfo(C.Structure):
     _fields_ = [("evaluations", C.c_int), ("iterations", C.c_int), ("final_pe", C.c_double),
                 ("final_grad_norm", C.c_double), ("status", C.c_int)]
@@ -67,6 +112,15 @@ SIGNATURES = {
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
     "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
     "litho_lsm_step": (_I, [_P, _I, _I, _P, _P, _IP, C.POINTER(LsmOptions), _I, _P, _P, _DP]),
+    "litho_partition_build": (_I, [_I, _I, _I, _I, _I, _I, _IP, C.POINTER(Tile)]),
+    "litho_partition_extract": (_I, [_P, _P, _I, _I, C.POINTER(Tile), _I, _I, _P]),
+    "litho_partition_assemble": (_I, [_P, _P, C.POINTER(Tile), _I, _I, _I, _I, _P]),
+    "litho_tile_exchange": (_I, [_P, C.POINTER(Tile), _I, _P, _P, C.POINTER(ExchangeOptions), _P, _P, _P,
+                                 C.POINTER(TileLoss), _DP]),
+    "litho_fullchip_ctm": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _P, C.POINTER(ExchangeOptions), _I, _D, _P, _P,
+                                _DP, _IP, _DP]),
+    "litho_fullchip_lsm": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _P, C.POINTER(ExchangeOptions),
+                                C.POINTER(LsmOptions), _I, _I, _P, _P, _DP, _IP, _DP]),
 }
 
 _lib = None
@@ -169,6 +223,25 @@ class Context:
         return dt
 
 
+    # ------------------------------------------------------------ partition
+    def partition_extract(self, global_, tiles, tile_size, out):
+        h, w = global_.shape
+        check(self.lib.litho_partition_extract(self.h, ptr(global_), h, w, tiles, len(tiles), tile_size, ptr(out)))
+
+    def partition_assemble(self, local, tiles, tile_size, out):
+        h, w = out.shape
+        check(self.lib.litho_partition_assemble(self.h, ptr(local), tiles, len(tiles), tile_size, h, w, ptr(out)))
+
+
+def partition_build(height, width, grid_rows, grid_cols, halo, guard):
+    """Partition::build through the library: (tile_size, ctypes array of Tile)."""
+    lib = load()
+    tiles = (Tile * (grid_rows * grid_cols))()
+    ts = C.c_int()
+    check(lib.litho_partition_build(height, width, grid_rows, grid_cols, halo, guard, C.byref(ts), tiles))
+    return ts.value, tiles
+
+
 def _shape3(t):
     if t.dim() == 2:
         return 1, t.shape[0], t.shape[1]
@@ -231,6 +304,37 @@ class Plan:
 
     def forward(self, mask, aerial=None, wafer=None, threshold=0.25, alpha=50.0, batch=1):
         check(self.lib.litho_socs_forward(self.h, batch, ptr(mask), ptr(aerial), ptr(wafer), threshold, alpha))
+
+    # ------------------------------------------------------------ full chip
+    def tile_exchange(self, tiles, masks, targets, options, mask_gradient, raw_gradient=None, wafer=None):
+        """MicroGradientExchange::compute on device Micros; returns (per-tile TileLoss list, totals)."""
+        per = (TileLoss * len(tiles))()
+        tot = np.zeros(3)
+        check(self.lib.litho_tile_exchange(self.h, tiles, len(tiles), ptr(masks), ptr(targets), C.byref(options),
+                                           ptr(mask_gradient), ptr(raw_gradient), ptr(wafer), per, _dp(tot)))
+        return list(per), tot
+
+    def fullchip_ctm(self, tiles, initial_pattern, target, options, iterations, learning_rate, global_mask_out,
+                     local_masks_out=None):
+        h, w = target.shape
+        hist = np.zeros((iterations + 1, 3))
+        bi, bl = C.c_int(), C.c_double()
+        check(self.lib.litho_fullchip_ctm(self.h, tiles, len(tiles), h, w, ptr(initial_pattern), ptr(target),
+                                          C.byref(options), iterations, learning_rate, ptr(global_mask_out),
+                                          ptr(local_masks_out), _dp(hist), C.byref(bi), C.byref(bl)))
+        return dict(best_iteration=bi.value, best_loss=bl.value, history=hist)
+
+    def fullchip_lsm(self, tiles, initial_binary, target, options, iterations, global_mask_out, global_phi_out=None,
+                     dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01, reinit_interval=20):
+        h, w = target.shape
+        hist = np.zeros((iterations + 1, 5))
+        bi, bl = C.c_int(), C.c_double()
+        lo = LsmOptions(dx, dy, cfl, curvature_weight)
+        check(self.lib.litho_fullchip_lsm(self.h, tiles, len(tiles), h, w, ptr(initial_binary), ptr(target),
+                                          C.byref(options), C.byref(lo), iterations, reinit_interval,
+                                          ptr(global_mask_out), ptr(global_phi_out), _dp(hist), C.byref(bi),
+                                          C.byref(bl)))
+        return dict(best_iteration=bi.value, best_loss=bl.value, history=hist)
 
     def pe_gradient(self, mask, target, grad_mask=None, aerial=None, wafer=None, pe_window=None,
                     penalty_window=None, pe_weight=1.0, penalty=0.0, penalty_threshold=0.01,

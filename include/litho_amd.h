@@ -192,6 +192,110 @@ int litho_lsm_step(litho_ctx* ctx, int batch, int n, const float* phi, const flo
                    const int* update_rects, const litho_lsm_options* options, int reinit, float* phi_out,
                    float* mask_out, double* dt_out);
 
+/* ---------------------------------------------------------------- full chip
+ * Micro tiling of a height x width chip into grid_rows x grid_cols Micros of
+ * tile_size x tile_size pixels (Core + Halo, periodic halo), Partition::build
+ * (partition.cpp:21-80).  Rectangles are half-open [y0, y1) x [x0, x1). */
+typedef struct {
+  int y0, y1, x0, x1;
+} litho_rect;
+
+typedef struct {
+  int id;                               /* neighbour Micro id */
+  int direction;                        /* 0 left, 1 right, 2 top, 3 bottom */
+  litho_rect guard_global;              /* my Guard strip inside the neighbour's Core, global */
+  litho_rect my_guard_local;            /* the same strip in my local coordinates */
+  litho_rect neighbor_overlap_local;    /* the same strip in the neighbour's local coordinates */
+  litho_rect overlap_global;            /* my Core strip inside the neighbour's Guard, global */
+  litho_rect my_overlap_local;          /* that strip in my local coordinates */
+  litho_rect neighbor_guard_local;      /* that strip in the neighbour's local coordinates */
+} litho_neighbor;                       /* MicroNeighbor (partition.h:26-45) */
+
+typedef struct {
+  int id, row, col, halo, guard;
+  litho_rect core_global, tile_global, core_local, update_local;   /* update = Core + Guard */
+  int num_neighbors;
+  litho_neighbor neighbors[4];          /* in left, right, top, bottom order, existing ones only */
+} litho_tile;                           /* MicroTile (partition.h:47-65) */
+
+/* Partition::build.  tiles: caller array of grid_rows * grid_cols entries, id =
+ * row * grid_cols + col.  tile_size = max core side + 2 halo, rounded up to even. */
+int litho_partition_build(int height, int width, int grid_rows, int grid_cols, int halo, int guard,
+                          int* tile_size, litho_tile* tiles);
+
+/* Partition::extract_local_tile for every Micro (partition.cpp:83-95): periodic
+ * gather of the device height x width array into num_tiles device tiles. */
+int litho_partition_extract(litho_ctx* ctx, const float* global, int height, int width, const litho_tile* tiles,
+                            int num_tiles, int tile_size, float* local_out);
+
+/* Partition::assemble_global (partition.cpp:105-113): Core scatter of device
+ * tiles into the device height x width array (pixels outside every Core untouched). */
+int litho_partition_assemble(litho_ctx* ctx, const float* local, const litho_tile* tiles, int num_tiles,
+                             int tile_size, int height, int width, float* global_out);
+
+/* MicroGradientOptions (micro_gradient_exchange.h:19-40), PE and PEPenalty data
+ * losses (the EPE modes are not part of this library). */
+typedef struct {
+  int loss_type;                 /* 0 PE (window = loss region), 1 PEPenalty (PE on target > .5, penalty on < .5) */
+  int loss_region;               /* 0 Core, 1 CoreGuard */
+  double pe_weight;
+  double penalty_weight;
+  double penalty_threshold;
+  int collect_core_metrics;
+  int enable_cept;
+  int cept_guard_only;
+  double cept_weight;
+  int enable_grad_fusion;
+  int fusion_mode;               /* 0 Average, 1 Linear */
+  double fusion_weight_owner;
+  double fusion_weight_neighbor;
+  double threshold;              /* resist threshold */
+  double alpha;                  /* resist steepness */
+} litho_exchange_options;
+
+typedef struct {                 /* MicroGradientData scalars (micro_gradient_exchange.h:98-112) */
+  double loss, data_loss, pe_loss, penalty_loss, bg_penalty_raw, cept_loss;
+  double core_pe_loss, core_bg_penalty_raw;   /* collect_core_metrics only */
+} litho_tile_loss;
+
+/* MicroGradientExchange::compute(local_masks, ...) (micro_gradient_exchange.cpp:134-334)
+ * on device-resident Micros: batched imaging + windowed PE / penalty adjoint of
+ * every Micro through the plan (plan tile size = tile_size; Micros are processed
+ * in chunks of the plan's batch capacity), then the four-neighbour exchange:
+ * Guard fusion (Average / Linear) of the raw gradients and CEPT, with
+ * mask_gradient = fused gradient + cept_weight * CEPT gradient.
+ * local_masks, local_targets: device [num_tiles][tile_size^2].  mask_gradient:
+ * device output; raw_gradient, wafer (device) and per_tile (host, num_tiles) are
+ * nullable.  totals (host, nullable): {loss, data_loss, cept_loss}. */
+int litho_tile_exchange(litho_plan* plan, const litho_tile* tiles, int num_tiles, const float* local_masks,
+                        const float* local_targets, const litho_exchange_options* options, float* mask_gradient,
+                        float* raw_gradient, float* wafer, litho_tile_loss* per_tile, double* totals);
+
+/* FullChipCTMOptimizer::optimize(initial_pattern, global_target, options)
+ * (full_chip_ctm.cpp:25-152): theta per Micro from initialize_theta of the
+ * pattern, fixed-step gradient descent on theta inside Core + Guard (Halo frozen),
+ * CEPT forced guard-only; the best total loss over iterations 0..n wins.
+ * initial_pattern, global_target: device height x width.  global_mask_out:
+ * device height x width (best Micros' Cores).  local_masks_out (device,
+ * nullable): best Micros, [num_tiles][tile_size^2].  history (host, nullable):
+ * (iterations + 1) x {loss, data_loss, cept_loss}. */
+int litho_fullchip_ctm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                       const float* initial_pattern, const float* global_target, const litho_exchange_options* options,
+                       int iterations, double learning_rate, float* global_mask_out, float* local_masks_out,
+                       double* history, int* best_iteration, double* best_loss);
+
+/* FullChipLSMOptimizer::optimize(initial_binary, global_target, options) with the
+ * Binary renderer (full_chip_lsm.cpp:39-255): phi per Micro = reinit_sdf(mask - 0.5),
+ * loss region forced to Core + Guard, CEPT guard-only, Core metrics collected;
+ * each iteration one RK2 update (litho_lsm_step) of every Micro with re-init
+ * every reinit_interval updates (0: never), then the exchange.
+ * global_mask_out / global_phi_out: device height x width (nullable).
+ * history (host, nullable): (iterations + 1) x {loss, data_loss, cept_loss, min_dt, max_dt}. */
+int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                       const float* initial_binary, const float* global_target, const litho_exchange_options* options,
+                       const litho_lsm_options* lsm, int iterations, int reinit_interval, float* global_mask_out,
+                       float* global_phi_out, double* history, int* best_iteration, double* best_loss);
+
 #ifdef __cplusplus
 }
 #endif

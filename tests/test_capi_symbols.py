@@ -58,3 +58,38 @@ def test_error_reporting_without_gpu():
         assert e.code == la.EINVAL and "Grid" in str(e)
     else:
         raise AssertionError("size 1 must be rejected")
+
+
+def test_partition_build_matches_oracle():
+    """Partition::build (host-only entry point) equals the oracle's restatement."""
+    la = litho_amd()
+    from oracle import fullchip as fc
+    for args in [(96, 96, 2, 2, 16, 8), (100, 90, 3, 2, 12, 12), (8192, 8192, 16, 16, 64, 40), (64, 64, 1, 1, 8, 4)]:
+        ts, tiles = la.partition_build(*args)
+        meta = fc.build(*args)
+        assert ts == meta.tile_size
+        assert len(tiles) == len(meta.tiles)
+        for t, m in zip(tiles, meta.tiles):
+            assert (t.id, t.row, t.col, t.halo, t.guard) == (m.id, m.row, m.col, m.halo, m.guard)
+            for name in ("core_global", "tile_global", "core_local", "update_local"):
+                r = getattr(m, name)
+                assert getattr(t, name).tuple() == (r.y0, r.y1, r.x0, r.x1), name
+            assert t.num_neighbors == len(m.neighbors)
+            for j, nb in enumerate(m.neighbors):
+                e = t.neighbors[j]
+                assert (e.id, e.direction) == (nb.id, nb.direction)
+                for name in ("guard_global", "my_guard_local", "neighbor_overlap_local", "overlap_global",
+                             "my_overlap_local", "neighbor_guard_local"):
+                    r = getattr(nb, name)
+                    assert getattr(e, name).tuple() == (r.y0, r.y1, r.x0, r.x1), name
+
+
+def test_partition_build_rejects_bad_grid():
+    la = litho_amd()
+    for args in [(0, 96, 2, 2, 16, 8), (96, 96, 0, 2, 16, 8), (96, 96, 2, 2, 8, 16)]:
+        try:
+            la.partition_build(*args)
+        except la.LithoError as e:
+            assert e.code == la.EINVAL
+        else:
+            raise AssertionError(f"{args} must be rejected")
