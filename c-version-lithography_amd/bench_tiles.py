@@ -1,0 +1,218 @@
+"""Tile-throughput benchmarks of the batched and served paths (one GPU).
+
+  python c-version-lithography_amd/bench_tiles.py batched [--tiles 96] [--iterations 30]
+      96 independent tiles, Cores 512 / 768 / 1024 (1:1:1) with a 64-pixel halo
+      per side -> N = 640 / 896 / 1152 (SOCS windows 67 / 97 / 123), K = 24,
+      30 L-BFGS iterations (m = 10), seeded Manhattan targets.  Plain: every
+      tile solved alone (litho_ctm_optimize, one tile per call).  Batched:
+      litho_batch_ctm with autotuned chunks.  Prints one JSON line with
+      grad-evals/s of both and their ratio.
+
+  python c-version-lithography_amd/bench_tiles.py served [--requests 160] [--iterations 20] [--load 0.7]
+      Requests of sides 256 / 512 / 1024 / 2048 plus a 64-pixel halo per side
+      (N = 384 / 640 / 1152 / 2176, uniformly drawn, seeded), K = 24, 20 CTM
+      gradient-descent iterations each (one evaluation per iteration: a
+      deterministic service time).  Capacity = requests/s of the server with
+      every request submitted at once; then a seeded Poisson arrival trace at
+      `load` x capacity is replayed against (a) the continuous-batching server
+      (litho_server_*) and (b) the plain path, one request at a time in arrival
+      order (litho_ctm_optimize).  Prints one JSON line with tiles/s, p50 and
+      p99 latency (arrival -> result) of both, and whether every request's mask
+      is the same in both runs.
+
+Inputs are resident in HBM before the timed regions; the SOCS precompute and
+plan creation are set-up, outside the timers.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+OPTICS = dict(pixel_nm=4.0, wavelength_nm=193.0, na=1.35, sigma_in=0.6, sigma_out=0.9)
+THR, ALPHA, K = 0.225, 50.0, 24
+
+
+def _load(name, path):
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+la = _load("litho_amd", HERE / "litho_amd.py")
+tg = _load("litho_targets", HERE / "targets.py")
+
+
+def make_plan(ctx, n, capacity):
+    d = ctx.socs_precompute(n, OPTICS["pixel_nm"], OPTICS["wavelength_nm"], OPTICS["na"], OPTICS["sigma_in"],
+                            OPTICS["sigma_out"], K)
+    return la.Plan(ctx, n, d["kernels"], d["weights"], d["weight_sum"], d["y0"], d["x0"], capacity=capacity), d
+
+
+def batched(args):
+    import torch
+    ctx = la.Context(0)
+    sizes = [512 + 2 * 64, 768 + 2 * 64, 1024 + 2 * 64]
+    per = args.tiles // 3
+    plans, wins = [], []
+    t0 = time.perf_counter()
+    for n in sizes:
+        p, d = make_plan(ctx, n, per)
+        plans.append(p)
+        wins.append(d["rows"])
+    print(f"set-up: {time.perf_counter() - t0:.1f} s, windows {wins}", file=sys.stderr, flush=True)
+    tiles = []
+    for pi, n in enumerate(sizes):
+        for j in range(per):
+            t = torch.from_numpy(tg.manhattan_target(n, 1000 * pi + j)).cuda()
+            tiles.append((pi, t, torch.zeros_like(t)))
+    torch.cuda.synchronize()
+    kw = dict(method="lbfgs", iterations=args.iterations, history=10, tol=0.0, threshold=THR, alpha=ALPHA)
+    # warm-up: first launches, autotune (cached in the plans)
+    la.batch_ctm(plans, [tiles[0], tiles[per], tiles[2 * per]], **kw)
+    _, used = la.batch_ctm(plans, tiles[:1] + tiles[per:per + 1] + tiles[2 * per:2 * per + 1], **kw)
+    # plain: one tile per call
+    t0 = time.perf_counter()
+    ev_plain = 0
+    for pi, t, out in tiles:
+        r = plans[pi].ctm_optimize(t, out, **kw)
+        ev_plain += r["evaluations"]
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain_masks = [out.clone() for _, _, out in tiles]
+    print(f"plain: {len(tiles)} tiles, {ev_plain} grad-evals in {t_plain:.3f} s", file=sys.stderr, flush=True)
+    # batched
+    for _, _, out in tiles:
+        out.zero_()
+    t0 = time.perf_counter()
+    info, used = la.batch_ctm(plans, tiles, **kw)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    ev_batch = sum(i["evaluations"] for i in info)
+    same = all(torch.equal(a, out) for a, (_, _, out) in zip(plain_masks, tiles))
+    print(f"batched: chunks {used}, {ev_batch} grad-evals in {t_batch:.3f} s; masks equal to plain: {same}",
+          file=sys.stderr, flush=True)
+    line = {"metric": "batched uneven-tile CTM grad-evals/s vs plain (one tile per call)",
+            "value": round(ev_batch / t_batch, 2), "plain": round(ev_plain / t_plain, 2),
+            "speedup": round((ev_batch / t_batch) / (ev_plain / t_plain), 3), "unit": "grad-evals/s",
+            "tiles": len(tiles), "sizes": sizes, "windows": wins, "k": K, "iterations": args.iterations,
+            "chunks": used, "evaluations": ev_batch, "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4),
+            "identical_masks": same}
+    for p in plans:
+        p.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def served(args):
+    import torch
+    ctx = la.Context(0)
+    sizes = [256 + 128, 512 + 128, 1024 + 128, 2048 + 128]
+    caps = [16, 16, 8, 4]
+    plans, wins = [], []
+    t0 = time.perf_counter()
+    for n, c in zip(sizes, caps):
+        p, d = make_plan(ctx, n, c)
+        plans.append(p)
+        wins.append(d["rows"])
+    print(f"set-up: {time.perf_counter() - t0:.1f} s, windows {wins}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(2024)
+    kinds = rng.integers(0, len(sizes), size=args.requests)
+    tgts = [torch.from_numpy(tg.manhattan_target(sizes[k], 5000 + i)).cuda() for i, k in enumerate(kinds)]
+    kw = dict(method="gradient_descent", iterations=args.iterations, learning_rate=0.9, threshold=THR, alpha=ALPHA)
+    torch.cuda.synchronize()
+
+    def run_server(arrivals):
+        outs = [torch.zeros_like(t) for t in tgts]
+        srv = la.Server(plans, **kw)
+        # warm: first launches and graph capture of every pool
+        warm = [srv.submit(k, tgts[i], torch.zeros_like(tgts[i])) for i, k in
+                [(int(np.argmax(kinds == j)), j) for j in range(len(sizes)) if (kinds == j).any()]]
+        for w in warm:
+            srv.wait(w)
+        start = time.perf_counter()
+        tickets, sub = [], []
+        for i, k in enumerate(kinds):
+            if arrivals is not None:
+                delay = start + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            sub.append(time.perf_counter() - start)
+            tickets.append(srv.submit(int(k), tgts[i], outs[i]))
+        res = [srv.wait(tk) for tk in tickets]
+        end = time.perf_counter() - start
+        srv.close()
+        arr = np.array(sub) if arrivals is None else np.asarray(arrivals)
+        # arrival -> result: submission lag behind the schedule + queue + service
+        lat = np.array([(sub[i] - arr[i]) * 1e3 + r["queued_ms"] + r["service_ms"] for i, r in enumerate(res)])
+        return outs, lat, end, sum(r["evaluations"] for r in res), arr
+
+    outs_cap, lat_cap, span_cap, ev_cap, _ = run_server(None)
+    capacity = args.requests / span_cap
+    print(f"server capacity: {args.requests} requests in {span_cap:.3f} s = {capacity:.1f} req/s "
+          f"({ev_cap / span_cap:.0f} grad-evals/s)", file=sys.stderr, flush=True)
+    rate = args.load * capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=args.requests))
+    outs_srv, lat_srv, span_srv, ev_srv, _ = run_server(arrivals)
+    # plain: one request at a time, in arrival order, same trace
+    outs_plain = [torch.zeros_like(t) for t in tgts]
+    plans[0].ctm_optimize(tgts[0], torch.zeros_like(tgts[0]), **kw)
+    lat_plain = np.zeros(args.requests)
+    start = time.perf_counter()
+    for i, k in enumerate(kinds):
+        delay = start + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        plans[int(k)].ctm_optimize(tgts[i], outs_plain[i], **kw)
+        torch.cuda.synchronize()
+        lat_plain[i] = (time.perf_counter() - start - arrivals[i]) * 1e3
+    span_plain = time.perf_counter() - start
+    same = all(torch.equal(a, b) for a, b in zip(outs_srv, outs_plain)) and \
+        all(torch.equal(a, b) for a, b in zip(outs_cap, outs_plain))
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)
+    line = {"metric": "served CTM tiles/s and latency, continuous batching vs plain",
+            "value": round(args.requests / span_srv, 2), "unit": "tiles/s",
+            "capacity_tiles_per_s": round(capacity, 2), "offered_load": args.load,
+            "offered_tiles_per_s": round(rate, 2),
+            "served": {"tiles_per_s": round(args.requests / span_srv, 2), "p50_ms": pct(lat_srv, 50),
+                       "p99_ms": pct(lat_srv, 99), "grad_evals_per_s": round(ev_srv / span_srv, 1)},
+            "plain": {"tiles_per_s": round(args.requests / span_plain, 2), "p50_ms": pct(lat_plain, 50),
+                      "p99_ms": pct(lat_plain, 99)},
+            "requests": args.requests, "sizes": sizes, "windows": wins, "slots": caps, "k": K,
+            "iterations": args.iterations, "optimizer": "gradient_descent",
+            "identical_masks_alone_vs_mixed": same}
+    for p in plans:
+        p.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    b = sub.add_parser("batched")
+    b.add_argument("--tiles", type=int, default=96)
+    b.add_argument("--iterations", type=int, default=30)
+    v = sub.add_parser("served")
+    v.add_argument("--requests", type=int, default=160)
+    v.add_argument("--iterations", type=int, default=20)
+    v.add_argument("--load", type=float, default=0.7)
+    args = ap.parse_args()
+    if args.cmd == "batched":
+        batched(args)
+    else:
+        served(args)
+
+
+if __name__ == "__main__":
+    main()

@@ -25,7 +25,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
 SOURCES = ["socs_kernels.hip", "opt_kernels.hip", "precompute.hip", "levelset.hip", "tiling.hip", "plan.cpp",
-           "optics.cpp", "ctm_driver.cpp", "lsm.cpp", "fullchip.cpp", "litho_capi.cpp"]
+           "optics.cpp", "ctm_driver.cpp", "batch.cpp", "server.cpp", "lsm.cpp", "fullchip.cpp",
+           "litho_capi.cpp"]
 
 
 def _compile(src: Path, obj: Path) -> tuple[str, int, str]:

@@ -1,4 +1,5 @@
-// CTM gradient descent and L-BFGS with device-resident state.
+// CTM gradient descent and L-BFGS with device-resident state, as a pool of
+// optimizer slots (ctm_driver.hpp).
 //
 // Gradient descent restates CTM_Optimizer::_optimize_gradient_descent
 // (CTM_Optimizer.cpp:109-173).  L-BFGS restates the published algorithm of
@@ -15,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <memory>
+#include <string>
 
 namespace litho {
 
@@ -33,13 +36,21 @@ double zoom_trial(double a_lo, double f_lo, double g_lo, double a_hi, double f_h
   return a;
 }
 
+void validate_ctm_options(const CtmOptions& opt) {
+  if (opt.method != 0 && opt.method != 1) throw Error(1, "CTM optimizer must be gradient_descent or lbfgs");
+  if (opt.iterations <= 0) throw Error(1, "CTM max_iteration must be positive");
+  if (!std::isfinite(opt.learning_rate) || opt.learning_rate <= 0.0)
+    throw Error(1, "CTM learning_rate must be finite and positive");
+  if (opt.history <= 0) throw Error(1, "CTM lbfgs_history_size must be positive");
+  if (opt.method == 1 && opt.history > kMaxHist)
+    throw Error(1, "CTM lbfgs_history_size must be in [1, " + std::to_string(kMaxHist) + "]");
+  if (!std::isfinite(opt.tolerance) || opt.tolerance < 0.0)
+    throw Error(1, "CTM gradient_tolerance must be finite and non-negative");
+}
+
 namespace {
 
-struct Buffers {
-  DevMem x, xp, g, gp, d, mask, hs, hy, pair_part, dir_part, vec_red;
-};
-
-// Host-written parameters of a batch (active flags, step coefficients, L-BFGS
+// Host-written parameters of the slots (active flags, step coefficients, L-BFGS
 // slot lists and direction coefficients), staged in one pinned block and
 // uploaded region by region with one transfer each.  The host rewrites a
 // region only after a stream synchronisation that follows its last upload.
@@ -75,92 +86,8 @@ void copy_tile(float* dst, const float* src, size_t n, hipStream_t s) {
   LITHO_HIP(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
 
-// ------------------------------------------------------------ gradient descent
-std::vector<CtmTileResult> run_gd(Plan& plan, int batch, const float* targets, const float* theta0,
-                                  const CtmOptions& opt, float* out_masks, float* out_theta) {
-  hipStream_t s = plan.ctx->stream;
-  const size_t n2 = plan.tile_elems();
-  Buffers bf;
-  bf.x.alloc(batch * n2 * sizeof(float));
-  bf.g.alloc(batch * n2 * sizeof(float));
-  bf.mask.alloc(batch * n2 * sizeof(float));
-  Params pm(batch);
-  float* x = bf.x.as<float>();
-  if (theta0 != nullptr)
-    LITHO_HIP(hipMemcpyAsync(x, theta0, batch * n2 * sizeof(float), hipMemcpyDeviceToDevice, s));
-  else
-    launch_init_theta(targets, x, batch * n2, s);
-  float* best = out_masks;
+void alloc_slots(DevMem& m, int slots, size_t n2, size_t per = 1) { m.alloc((size_t)slots * per * n2 * sizeof(float)); }
 
-  std::vector<CtmTileResult> res(batch);
-  std::vector<int> running(batch, 1);
-  std::vector<double> min_pe(batch, std::numeric_limits<double>::max());
-  int* flags = pm.active();
-  float* coef = pm.step();
-  for (int b = 0; b < batch; ++b) {
-    flags[b] = 1;
-    coef[b] = 0.f;
-  }
-
-  for (int it = 0; it < opt.iterations; ++it) {
-    EvalRequest rq;
-    rq.mode = 1;
-    rq.base = x;
-    if (it > 0) {
-      for (int b = 0; b < batch; ++b) {
-        coef[b] = running[b] ? (float)(-opt.learning_rate) : 0.f;
-        flags[b] = running[b];
-      }
-      rq.dir = bf.g.as<float>();
-      rq.coef = pm.d_step();
-      rq.theta_out = x;
-    }
-    pm.upload_active_step(s);
-    rq.mask_out = bf.mask.as<float>();
-    rq.im.thr = (float)opt.threshold;
-    rq.im.alpha = (float)opt.alpha;
-    rq.im.target = targets;
-    rq.im.pe_weight = 1.f;
-    rq.ga.scale = (float)(4.0 * opt.alpha);
-    rq.ga.theta = x;
-    rq.ga.grad_theta = bf.g.as<float>();
-    rq.active = pm.d_active();
-    run_eval(plan, batch, rq);
-    const double* red = fetch_reductions(plan, batch);
-    bool any = false;
-    for (int b = 0; b < batch; ++b) {
-      if (!running[b]) continue;
-      const double pe = red[4 * b + 0];
-      const double gn = std::sqrt(red[4 * b + 2]);
-      res[b].history.push_back(pe);
-      res[b].evaluations++;
-      res[b].iterations = it + 1;
-      res[b].final_grad_norm = gn;
-      if (pe < min_pe[b]) {
-        min_pe[b] = pe;
-        copy_tile(best + b * n2, bf.mask.as<float>() + b * n2, n2, s);
-      }
-      if (gn <= opt.tolerance) {       // CTM_Optimizer.cpp:156-159
-        running[b] = 0;
-        res[b].status = 0;
-      } else {
-        any = true;
-      }
-    }
-    if (!any) break;
-  }
-  // the reference applies the update after the last evaluation too (CTM_Optimizer.cpp:164-165)
-  for (int b = 0; b < batch; ++b) coef[b] = running[b] ? (float)(-opt.learning_rate) : 0.f;
-  pm.upload_active_step(s);
-  launch_axpy(bf.g.as<float>(), x, pm.d_step(), nullptr, n2, batch, s);
-  for (int b = 0; b < batch; ++b) res[b].final_pe = min_pe[b];
-  if (out_theta != nullptr)
-    LITHO_HIP(hipMemcpyAsync(out_theta, x, batch * n2 * sizeof(float), hipMemcpyDeviceToDevice, s));
-  LITHO_HIP(hipStreamSynchronize(s));
-  return res;
-}
-
-// ------------------------------------------------------------------ L-BFGS
 struct LbfgsTile {
   enum Phase { INIT, BRACKET, ZOOM, NEED_PAIR, DONE };
   Phase phase = INIT;
@@ -207,57 +134,221 @@ void solve_compact(const LbfgsTile& t, const std::vector<double>& p, const std::
   }
 }
 
-std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets, const float* theta0,
-                                     const CtmOptions& opt, float* out_masks, float* out_theta) {
-  if (opt.history <= 0 || opt.history > kMaxHist)
-    throw Error(1, "CTM lbfgs_history_size must be in [1, " + std::to_string(kMaxHist) + "]");
-  hipStream_t s = plan.ctx->stream;
-  const size_t n2 = plan.tile_elems();
-  const LineSearchParams lp;
-  const int m_max = opt.history;
-  Buffers bf;
-  bf.x.alloc(batch * n2 * sizeof(float));
-  bf.xp.alloc(batch * n2 * sizeof(float));
-  bf.g.alloc(batch * n2 * sizeof(float));
-  bf.gp.alloc(batch * n2 * sizeof(float));
-  bf.d.alloc(batch * n2 * sizeof(float));
-  bf.mask.alloc(batch * n2 * sizeof(float));
-  bf.hs.alloc(batch * (size_t)(kMaxHist + 1) * n2 * sizeof(float));
-  bf.hy.alloc(batch * (size_t)(kMaxHist + 1) * n2 * sizeof(float));
-  bf.pair_part.alloc(batch * (size_t)kOptBlocks * kPairDots * sizeof(double));
-  bf.dir_part.alloc(batch * (size_t)kOptBlocks * sizeof(double));
-  bf.vec_red.alloc(batch * kPairDots * sizeof(double));
-  Params pm(batch);
-  HostPinned hvec(batch * kPairDots * sizeof(double));
-  float* x = bf.x.as<float>();
-  float* xp = bf.xp.as<float>();
-  float* g = bf.g.as<float>();
-  float* gp = bf.gp.as<float>();
-  float* d = bf.d.as<float>();
-  LITHO_HIP(hipMemsetAsync(d, 0, batch * n2 * sizeof(float), s));
-  if (theta0 != nullptr)
-    LITHO_HIP(hipMemcpyAsync(xp, theta0, batch * n2 * sizeof(float), hipMemcpyDeviceToDevice, s));
-  else
-    launch_init_theta(targets, xp, batch * n2, s);
+}  // namespace
 
-  std::vector<LbfgsTile> st(batch);
-  std::vector<CtmTileResult> res(batch);
-  for (int b = 0; b < batch; ++b) st[b].spare = 0;
-  float* stepv = pm.step();
-  int* flags = pm.active();
-  int* meta = pm.meta();
-  double* coefs = pm.coef();
-  for (int b = 0; b < batch; ++b) {
-    stepv[b] = 0.f;
-    flags[b] = 1;
+// ------------------------------------------------------------------ pool base
+struct CtmPool::Impl {
+  Plan& plan;
+  const int S;
+  const CtmOptions opt;
+  hipStream_t s;
+  const size_t n2;
+  std::vector<CtmJob> job;
+  std::vector<char> running;
+  std::vector<CtmTileResult> res;
+  std::vector<std::pair<long long, CtmTileResult>> finished;
+  DevMem tgt;        // [S][N^2] targets of the slots
+  Params pm;
+
+  Impl(Plan& p, int slots, const CtmOptions& o)
+      : plan(p), S(slots), opt(o), s(p.ctx->stream), n2(p.tile_elems()), job(slots), running(slots, 0),
+        res(slots), pm(slots) {
+    alloc_slots(tgt, S, n2);
+    for (int b = 0; b < S; ++b) {
+      pm.active()[b] = 0;
+      pm.step()[b] = 0.f;
+    }
   }
-  for (int i = 0; i < batch * kMetaStride; ++i) meta[i] = -1;
-  for (int i = 0; i < batch * kCoefStride; ++i) coefs[i] = 0.0;
-  const double eps = std::numeric_limits<double>::epsilon();
+  virtual ~Impl() = default;
+  int busy() const { return (int)std::count(running.begin(), running.end(), 1); }
+  int find_free() const {
+    for (int b = 0; b < S; ++b)
+      if (!running[b]) return b;
+    return -1;
+  }
+  // Copies the job's target into slot b and its initial theta into theta_dst[b].
+  void load_slot(int b, const CtmJob& j, float* theta_dst) {
+    if (j.target == nullptr || j.out_mask == nullptr) throw Error(1, "CTM: target and output mask are required");
+    copy_tile(tgt.as<float>() + b * n2, j.target, n2, s);
+    if (j.theta0 != nullptr)
+      copy_tile(theta_dst + b * n2, j.theta0, n2, s);
+    else
+      launch_init_theta(j.target, theta_dst + b * n2, n2, s);
+    job[b] = j;
+    res[b] = CtmTileResult{};
+    running[b] = 1;
+  }
+  void finish(int b) {
+    finished.emplace_back(job[b].id, std::move(res[b]));
+    res[b] = CtmTileResult{};
+    running[b] = 0;
+  }
+  EvalRequest base_request(float* mask_out, float* grad_theta) const {
+    EvalRequest rq;
+    rq.mode = 1;
+    rq.mask_out = mask_out;
+    rq.im.thr = (float)opt.threshold;
+    rq.im.alpha = (float)opt.alpha;
+    rq.im.target = tgt.as<float>();
+    rq.im.pe_weight = 1.f;
+    rq.ga.scale = (float)(4.0 * opt.alpha);
+    rq.ga.grad_theta = grad_theta;
+    rq.active = pm.d_active();
+    return rq;
+  }
+  virtual bool admit(const CtmJob& j) = 0;
+  virtual void step() = 0;
+  virtual size_t slot_state_bytes() const = 0;
+};
+
+// ------------------------------------------------------------ gradient descent
+namespace {
+
+struct GdPool final : CtmPool::Impl {
+  DevMem x, g, mask, best;
+  std::vector<double> min_pe;
+  std::vector<char> first;
+  std::unique_ptr<EvalGraph> graph;
+
+  GdPool(Plan& p, int slots, const CtmOptions& o) : Impl(p, slots, o), min_pe(slots), first(slots, 1) {
+    alloc_slots(x, S, n2);
+    alloc_slots(g, S, n2);
+    alloc_slots(mask, S, n2);
+    alloc_slots(best, S, n2);
+    // every evaluation: theta = x + coef * g (coef 0 for a slot's first
+    // evaluation, -lr after it), gradient to g
+    EvalRequest rq = base_request(mask.as<float>(), g.as<float>());
+    rq.base = x.as<float>();
+    rq.dir = g.as<float>();
+    rq.coef = pm.d_step();
+    rq.theta_out = x.as<float>();
+    rq.ga.theta = x.as<float>();
+    graph = std::make_unique<EvalGraph>(plan, S, rq);
+  }
+  size_t slot_state_bytes() const override { return 5 * n2 * sizeof(float); }
+
+  bool admit(const CtmJob& j) override {
+    const int b = find_free();
+    if (b < 0) return false;
+    load_slot(b, j, x.as<float>());
+    LITHO_HIP(hipMemsetAsync(g.as<float>() + b * n2, 0, n2 * sizeof(float), s));   // x + 0 * g must be x
+    min_pe[b] = std::numeric_limits<double>::max();
+    first[b] = 1;
+    return true;
+  }
+
+  void step() override {
+    if (busy() == 0) return;
+    int* flags = pm.active();
+    float* coef = pm.step();
+    for (int b = 0; b < S; ++b) {
+      flags[b] = running[b];
+      coef[b] = (running[b] && !first[b]) ? (float)(-opt.learning_rate) : 0.f;
+    }
+    pm.upload_active_step(s);
+    graph->launch();
+    const double* red = fetch_reductions(plan, S);
+    std::vector<int> done;
+    bool update = false;
+    for (int b = 0; b < S; ++b) {
+      if (!running[b]) continue;
+      first[b] = 0;
+      CtmTileResult& r = res[b];
+      const double pe = red[4 * b + 0];
+      const double gn = std::sqrt(red[4 * b + 2]);
+      r.history.push_back(pe);
+      r.evaluations++;
+      r.iterations = r.evaluations;
+      r.final_grad_norm = gn;
+      if (pe < min_pe[b]) {
+        min_pe[b] = pe;
+        copy_tile(best.as<float>() + b * n2, mask.as<float>() + b * n2, n2, s);
+      }
+      if (gn <= opt.tolerance) {             // CTM_Optimizer.cpp:156-159
+        r.status = 0;
+        done.push_back(b);
+      } else if (r.evaluations == opt.iterations) {
+        r.status = 1;
+        done.push_back(b);
+        update = true;
+      }
+    }
+    if (done.empty()) return;
+    // the reference applies the update after the last evaluation too (CTM_Optimizer.cpp:164-165)
+    if (update) {
+      for (int b = 0; b < S; ++b) {
+        flags[b] = 0;
+        coef[b] = 0.f;
+      }
+      for (int b : done)
+        if (res[b].status == 1) {
+          flags[b] = 1;
+          coef[b] = (float)(-opt.learning_rate);
+        }
+      pm.upload_active_step(s);
+      launch_axpy(g.as<float>(), x.as<float>(), pm.d_step(), pm.d_active(), n2, S, s);
+    }
+    for (int b : done) {
+      res[b].final_pe = min_pe[b];
+      copy_tile(job[b].out_mask, best.as<float>() + b * n2, n2, s);
+      if (job[b].out_theta != nullptr) copy_tile(job[b].out_theta, x.as<float>() + b * n2, n2, s);
+      finish(b);
+    }
+    LITHO_HIP(hipStreamSynchronize(s));   // the staged flags are rewritten next step
+  }
+};
+
+// ------------------------------------------------------------------ L-BFGS
+struct LbfgsPool final : CtmPool::Impl {
+  DevMem x, xp, g, gp, d, mask, hs, hy, pair_part, dir_part, vec_red;
+  HostPinned hvec;
+  std::vector<LbfgsTile> st;
+  std::unique_ptr<EvalGraph> graph;
+  const LineSearchParams lp;
+  const int m_max;
+
+  LbfgsPool(Plan& p, int slots, const CtmOptions& o)
+      : Impl(p, slots, o), hvec((size_t)slots * kPairDots * sizeof(double)), st(slots), m_max(o.history) {
+    alloc_slots(x, S, n2);
+    alloc_slots(xp, S, n2);
+    alloc_slots(g, S, n2);
+    alloc_slots(gp, S, n2);
+    alloc_slots(d, S, n2);
+    alloc_slots(mask, S, n2);
+    alloc_slots(hs, S, n2, kMaxHist + 1);
+    alloc_slots(hy, S, n2, kMaxHist + 1);
+    pair_part.alloc((size_t)S * kOptBlocks * kPairDots * sizeof(double));
+    dir_part.alloc((size_t)S * kOptBlocks * sizeof(double));
+    vec_red.alloc((size_t)S * kPairDots * sizeof(double));
+    int* meta = pm.meta();
+    double* coefs = pm.coef();
+    for (int i = 0; i < S * kMetaStride; ++i) meta[i] = -1;
+    for (int i = 0; i < S * kCoefStride; ++i) coefs[i] = 0.0;
+    // every evaluation reads the same buffers: theta = xp + step * d for the
+    // active slots, gradient to g, g . d for the line search
+    EvalRequest rq = base_request(mask.as<float>(), g.as<float>());
+    rq.base = xp.as<float>();
+    rq.dir = d.as<float>();
+    rq.coef = pm.d_step();
+    rq.theta_out = x.as<float>();
+    rq.ga.theta = x.as<float>();
+    rq.ga.dir = d.as<float>();
+    graph = std::make_unique<EvalGraph>(plan, S, rq);
+  }
+  size_t slot_state_bytes() const override { return (size_t)(6 + 2 * (kMaxHist + 1)) * n2 * sizeof(float); }
+
+  bool admit(const CtmJob& j) override {
+    const int b = find_free();
+    if (b < 0) return false;
+    load_slot(b, j, xp.as<float>());
+    LITHO_HIP(hipMemsetAsync(d.as<float>() + b * n2, 0, n2 * sizeof(float), s));   // first trial: x = xp + 0 * d
+    st[b] = LbfgsTile{};
+    return true;
+  }
 
   // xp / gp must already hold the accepted point and its gradient (the pair
   // kernel shifts them; the first line search copies them)
-  auto begin_line_search = [&](int b, double dg0) {
+  void begin_line_search(int b, double dg0) {
     LbfgsTile& t = st[b];
     t.f0 = t.f;
     t.dg0 = dg0;
@@ -279,14 +370,14 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
       return;
     }
     t.phase = LbfgsTile::BRACKET;
-  };
-  auto fail = [&](int b) {
+  }
+  void fail(int b) {
     st[b].phase = LbfgsTile::DONE;
     st[b].final_is_xp = true;
     st[b].f = st[b].f0;
     res[b].status = 2;
-  };
-  auto accept = [&](int b) {
+  }
+  void accept(int b) {
     LbfgsTile& t = st[b];
     if (t.gnorm <= opt.tolerance) {
       t.phase = LbfgsTile::DONE;
@@ -297,17 +388,20 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
     } else {
       t.phase = LbfgsTile::NEED_PAIR;
     }
-  };
+  }
 
-  for (;;) {
-    // ---- (A) new directions for tiles that accepted a step
+  // (A) new directions for the slots that accepted a step
+  void update_directions() {
+    int* meta = pm.meta();
+    double* coefs = pm.coef();
+    const double eps = std::numeric_limits<double>::epsilon();
     bool need_pair = false;
     int max_m = 0;
-    for (int b = 0; b < batch; ++b) {
+    for (int b = 0; b < S; ++b) {
       int* mb = meta + b * kMetaStride;
       for (int i = 0; i < kMetaStride; ++i) mb[i] = 0;
       mb[0] = -1;
-      if (st[b].phase == LbfgsTile::NEED_PAIR) {
+      if (running[b] && st[b].phase == LbfgsTile::NEED_PAIR) {
         need_pair = true;
         const LbfgsTile& t = st[b];
         mb[0] = (int)t.slots.size();
@@ -316,127 +410,116 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
         max_m = std::max(max_m, mb[0]);
       }
     }
-    if (need_pair) {
-      const int pair_nv = 4 * (max_m + 1);   // dots per tile: 4 per old slot, then the new pair
-      pm.upload_meta(s);
-      launch_lbfgs_pair(x, xp, g, gp, bf.hs.as<float>(), bf.hy.as<float>(), pm.d_meta(), n2, batch, max_m, true,
-                        bf.pair_part.as<double>(), s);
-      launch_reduce_fixed(bf.pair_part.as<double>(), kOptBlocks, pair_nv, bf.vec_red.as<double>(), kPairDots, batch,
-                          s);
-      LITHO_HIP(hipMemcpyAsync(hvec.ptr, bf.vec_red.ptr, batch * kPairDots * sizeof(double), hipMemcpyDeviceToHost, s));
-      LITHO_HIP(hipStreamSynchronize(s));
-      const double* dots = hvec.as<double>();
-      for (int b = 0; b < batch; ++b) {
-        if (st[b].phase != LbfgsTile::NEED_PAIR) continue;
-        LbfgsTile& t = st[b];
-        const double* v = dots + (size_t)b * kPairDots;
-        const int m = (int)t.slots.size();
-        const double ys = v[4 * max_m + 0], yy = v[4 * max_m + 1];
-        std::vector<double> p(m), q(m);
+    if (!need_pair) return;
+    const int pair_nv = 4 * (max_m + 1);   // dots per slot: 4 per old pair, then the new pair
+    pm.upload_meta(s);
+    launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(), hy.as<float>(),
+                      pm.d_meta(), n2, S, max_m, true, pair_part.as<double>(), s);
+    launch_reduce_fixed(pair_part.as<double>(), kOptBlocks, pair_nv, vec_red.as<double>(), kPairDots, S, s);
+    LITHO_HIP(hipMemcpyAsync(hvec.ptr, vec_red.ptr, (size_t)S * kPairDots * sizeof(double), hipMemcpyDeviceToHost, s));
+    LITHO_HIP(hipStreamSynchronize(s));
+    const double* dots = hvec.as<double>();
+    for (int b = 0; b < S; ++b) {
+      if (!running[b] || st[b].phase != LbfgsTile::NEED_PAIR) continue;
+      LbfgsTile& t = st[b];
+      const double* v = dots + (size_t)b * kPairDots;
+      const int m = (int)t.slots.size();
+      const double ys = v[4 * max_m + 0], yy = v[4 * max_m + 1];
+      std::vector<double> p(m), q(m);
+      for (int j = 0; j < m; ++j) {
+        p[j] = v[4 * j + 2];
+        q[j] = v[4 * j + 3];
+      }
+      if (ys > eps * yy) {          // BFGSMat::add_correction condition
+        std::vector<double> col_sy(m), col_yy(m);
         for (int j = 0; j < m; ++j) {
-          p[j] = v[4 * j + 2];
-          q[j] = v[4 * j + 3];
+          col_sy[j] = v[4 * j + 0];
+          col_yy[j] = v[4 * j + 1];
         }
-        if (ys > eps * yy) {          // BFGSMat::add_correction condition
-          std::vector<double> col_sy(m), col_yy(m);
-          for (int j = 0; j < m; ++j) {
-            col_sy[j] = v[4 * j + 0];
-            col_yy[j] = v[4 * j + 1];
+        const int drop = m == m_max ? 1 : 0;
+        const int freed = drop ? t.slots[0] : -1;
+        std::vector<int> ns(t.slots.begin() + drop, t.slots.end());
+        ns.push_back(t.spare);
+        const int m2 = (int)ns.size();
+        std::vector<std::vector<double>> sy2(m2, std::vector<double>(m2, 0.0)), yy2 = sy2;
+        for (int i = 0; i < m2 - 1; ++i)
+          for (int j = 0; j < m2 - 1; ++j) {
+            sy2[i][j] = t.sy[i + drop][j + drop];
+            yy2[i][j] = t.yy[i + drop][j + drop];
           }
-          int drop = m == m_max ? 1 : 0;
-          int freed = drop ? t.slots[0] : -1;
-          std::vector<int> ns(t.slots.begin() + drop, t.slots.end());
-          ns.push_back(t.spare);
-          const int m2 = (int)ns.size();
-          std::vector<std::vector<double>> sy2(m2, std::vector<double>(m2, 0.0)), yy2 = sy2;
-          for (int i = 0; i < m2 - 1; ++i)
-            for (int j = 0; j < m2 - 1; ++j) {
-              sy2[i][j] = t.sy[i + drop][j + drop];
-              yy2[i][j] = t.yy[i + drop][j + drop];
-            }
-          for (int i = 0; i < m2 - 1; ++i) {
-            sy2[i][m2 - 1] = col_sy[i + drop];
-            yy2[i][m2 - 1] = col_yy[i + drop];
-            yy2[m2 - 1][i] = col_yy[i + drop];
-          }
-          sy2[m2 - 1][m2 - 1] = ys;
-          yy2[m2 - 1][m2 - 1] = yy;
-          std::vector<double> p2(m2), q2(m2);
-          for (int i = 0; i < m2 - 1; ++i) {
-            p2[i] = p[i + drop];
-            q2[i] = q[i + drop];
-          }
-          p2[m2 - 1] = v[4 * max_m + 2];
-          q2[m2 - 1] = v[4 * max_m + 3];
-          // spare slot for the next pair: the freed slot or an unused index
-          if (freed >= 0) {
-            t.spare = freed;
-          } else {
-            int cand = 0;
-            while (std::find(ns.begin(), ns.end(), cand) != ns.end()) ++cand;
-            t.spare = cand;
-          }
-          t.slots = ns;
-          t.sy = sy2;
-          t.yy = yy2;
-          p = p2;
-          q = q2;
+        for (int i = 0; i < m2 - 1; ++i) {
+          sy2[i][m2 - 1] = col_sy[i + drop];
+          yy2[i][m2 - 1] = col_yy[i + drop];
+          yy2[m2 - 1][i] = col_yy[i + drop];
         }
-        solve_compact(t, p, q, coefs + (size_t)b * kCoefStride);
+        sy2[m2 - 1][m2 - 1] = ys;
+        yy2[m2 - 1][m2 - 1] = yy;
+        std::vector<double> p2(m2), q2(m2);
+        for (int i = 0; i < m2 - 1; ++i) {
+          p2[i] = p[i + drop];
+          q2[i] = q[i + drop];
+        }
+        p2[m2 - 1] = v[4 * max_m + 2];
+        q2[m2 - 1] = v[4 * max_m + 3];
+        // spare slot for the next pair: the freed slot or an unused index
+        if (freed >= 0) {
+          t.spare = freed;
+        } else {
+          int cand = 0;
+          while (std::find(ns.begin(), ns.end(), cand) != ns.end()) ++cand;
+          t.spare = cand;
+        }
+        t.slots = ns;
+        t.sy = sy2;
+        t.yy = yy2;
+        p = p2;
+        q = q2;
       }
-      // rebuild meta with the committed slot lists for the direction kernel
-      int dir_m = 0;
-      for (int b = 0; b < batch; ++b) {
-        int* mb = meta + b * kMetaStride;
-        if (st[b].phase != LbfgsTile::NEED_PAIR) continue;
-        mb[0] = (int)st[b].slots.size();
-        mb[1] = st[b].spare;
-        for (size_t j = 0; j < st[b].slots.size(); ++j) mb[2 + j] = st[b].slots[j];
-        dir_m = std::max(dir_m, mb[0]);
-      }
-      pm.upload_meta_coef(s);
-      launch_lbfgs_direction(g, bf.hs.as<float>(), bf.hy.as<float>(), pm.d_meta(), pm.d_coef(), n2, d, batch, dir_m,
-                             bf.dir_part.as<double>(), s);
-      launch_reduce_fixed(bf.dir_part.as<double>(), kOptBlocks, 1, bf.vec_red.as<double>(), 1, batch, s);
-      LITHO_HIP(hipMemcpyAsync(hvec.ptr, bf.vec_red.ptr, batch * sizeof(double), hipMemcpyDeviceToHost, s));
-      LITHO_HIP(hipStreamSynchronize(s));
-      for (int b = 0; b < batch; ++b) {
-        if (st[b].phase != LbfgsTile::NEED_PAIR) continue;
-        st[b].step = 1.0;
-        st[b].k += 1;
-        begin_line_search(b, hvec.as<double>()[b]);
-      }
+      solve_compact(t, p, q, coefs + (size_t)b * kCoefStride);
     }
-    // ---- (B) one batched evaluation for every tile with a pending trial
+    // rebuild meta with the committed slot lists for the direction kernel
+    int dir_m = 0;
+    for (int b = 0; b < S; ++b) {
+      int* mb = meta + b * kMetaStride;
+      if (!running[b] || st[b].phase != LbfgsTile::NEED_PAIR) continue;
+      mb[0] = (int)st[b].slots.size();
+      mb[1] = st[b].spare;
+      for (size_t j = 0; j < st[b].slots.size(); ++j) mb[2 + j] = st[b].slots[j];
+      dir_m = std::max(dir_m, mb[0]);
+    }
+    pm.upload_meta_coef(s);
+    launch_lbfgs_direction(g.as<float>(), hs.as<float>(), hy.as<float>(), pm.d_meta(), pm.d_coef(), n2, d.as<float>(),
+                           S, dir_m, dir_part.as<double>(), s);
+    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, vec_red.as<double>(), 1, S, s);
+    LITHO_HIP(hipMemcpyAsync(hvec.ptr, vec_red.ptr, (size_t)S * sizeof(double), hipMemcpyDeviceToHost, s));
+    LITHO_HIP(hipStreamSynchronize(s));
+    for (int b = 0; b < S; ++b) {
+      if (!running[b] || st[b].phase != LbfgsTile::NEED_PAIR) continue;
+      st[b].step = 1.0;
+      st[b].k += 1;
+      begin_line_search(b, hvec.as<double>()[b]);
+    }
+  }
+
+  // (B) one batched evaluation of every slot with a pending trial
+  void evaluate() {
+    int* flags = pm.active();
+    float* stepv = pm.step();
     bool any = false;
-    for (int b = 0; b < batch; ++b) {
+    for (int b = 0; b < S; ++b) {
       const LbfgsTile& t = st[b];
-      const bool ev = t.phase == LbfgsTile::INIT || t.phase == LbfgsTile::BRACKET || t.phase == LbfgsTile::ZOOM;
+      const bool ev = running[b] && (t.phase == LbfgsTile::INIT || t.phase == LbfgsTile::BRACKET ||
+                                     t.phase == LbfgsTile::ZOOM);
       flags[b] = ev ? 1 : 0;
-      stepv[b] = t.phase == LbfgsTile::INIT ? 0.f : (float)t.a;
+      stepv[b] = (ev && t.phase != LbfgsTile::INIT) ? (float)t.a : 0.f;
       any = any || ev;
     }
-    if (!any) break;
+    if (!any) return;
     pm.upload_active_step(s);
-    EvalRequest rq;
-    rq.mode = 1;
-    rq.base = xp;
-    rq.dir = d;
-    rq.coef = pm.d_step();
-    rq.theta_out = x;
-    rq.mask_out = bf.mask.as<float>();
-    rq.im.thr = (float)opt.threshold;
-    rq.im.alpha = (float)opt.alpha;
-    rq.im.target = targets;
-    rq.im.pe_weight = 1.f;
-    rq.ga.scale = (float)(4.0 * opt.alpha);
-    rq.ga.theta = x;
-    rq.ga.grad_theta = g;
-    rq.ga.dir = d;
-    rq.active = pm.d_active();
-    run_eval(plan, batch, rq);
-    const double* red = fetch_reductions(plan, batch);
-    for (int b = 0; b < batch; ++b) {
+    graph->launch();
+    const double* red = fetch_reductions(plan, S);
+    std::vector<int> started;
+    for (int b = 0; b < S; ++b) {
       if (!flags[b]) continue;
       LbfgsTile& t = st[b];
       const double f = red[4 * b + 0];
@@ -447,16 +530,14 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
       if (t.phase == LbfgsTile::INIT) {
         t.f = f;
         t.gnorm = std::sqrt(gn2);
+        t.k = 1;
         if (t.gnorm <= opt.tolerance) {   // LBFGS++ early exit, 1 evaluation
           t.phase = LbfgsTile::DONE;
-          t.k = 1;
           res[b].status = 0;
           continue;
         }
-        stepv[b] = -1.f;                  // d = -g
         t.step = 1.0 / t.gnorm;
-        t.k = 1;
-        t.phase = LbfgsTile::NEED_PAIR;   // marker: handled right below
+        started.push_back(b);             // d = -g, then the first line search
         continue;
       }
       t.trials += 1;
@@ -502,54 +583,93 @@ std::vector<CtmTileResult> run_lbfgs(Plan& plan, int batch, const float* targets
       }
       t.a = zoom_trial(t.lo_a, t.lo_f, t.lo_g, t.hi_a, t.hi_f, t.hi_g);
     }
-    // tiles that finished INIT: d = -g and start the first line search
-    bool init_dir = false;
-    for (int b = 0; b < batch; ++b) {
-      flags[b] = (st[b].phase == LbfgsTile::NEED_PAIR && st[b].k == 1 && st[b].slots.empty() &&
-                  res[b].evaluations == 1) ? 1 : 0;
-      init_dir = init_dir || flags[b];
+    if (started.empty()) return;
+    // slots whose first evaluation just ended: d = -g and the first line search
+    for (int b = 0; b < S; ++b) {
+      flags[b] = 0;
+      stepv[b] = 0.f;
     }
-    if (init_dir) {
-      pm.upload_active_step(s);
-      launch_scale(g, d, pm.d_step(), pm.d_active(), n2, batch, s);
-      for (int b = 0; b < batch; ++b) {
-        if (!flags[b]) continue;
-        copy_tile(xp + b * n2, x + b * n2, n2, s);
-        copy_tile(gp + b * n2, g + b * n2, n2, s);
-        begin_line_search(b, -st[b].gnorm * st[b].gnorm);
-      }
-      // the staged flags / steps are rewritten next iteration: wait for the upload
-      LITHO_HIP(hipStreamSynchronize(s));
+    for (int b : started) {
+      flags[b] = 1;
+      stepv[b] = -1.f;
+    }
+    pm.upload_active_step(s);
+    launch_scale(g.as<float>(), d.as<float>(), pm.d_step(), pm.d_active(), n2, S, s);
+    for (int b : started) {
+      copy_tile(xp.as<float>() + b * n2, x.as<float>() + b * n2, n2, s);
+      copy_tile(gp.as<float>() + b * n2, g.as<float>() + b * n2, n2, s);
+      begin_line_search(b, -st[b].gnorm * st[b].gnorm);
+    }
+    LITHO_HIP(hipStreamSynchronize(s));   // the staged flags / steps are rewritten next round
+  }
+
+  void step() override {
+    if (busy() == 0) return;
+    update_directions();
+    evaluate();
+    // write out the slots that finished: x, or xp after a failed line search
+    for (int b = 0; b < S; ++b) {
+      if (!running[b] || st[b].phase != LbfgsTile::DONE) continue;
+      const float* src = (st[b].final_is_xp ? xp.as<float>() : x.as<float>()) + b * n2;
+      if (job[b].out_theta != nullptr) copy_tile(job[b].out_theta, src, n2, s);
+      launch_mask_from_theta(src, job[b].out_mask, n2, s);
+      res[b].iterations = std::max(st[b].k, 1);
+      res[b].final_pe = st[b].f;
+      res[b].final_grad_norm = st[b].gnorm;
+      finish(b);
     }
   }
-  // final theta: x, or xp after a failed line search
-  for (int b = 0; b < batch; ++b) {
-    const float* src = st[b].final_is_xp ? xp + b * n2 : x + b * n2;
-    if (out_theta != nullptr) copy_tile(out_theta + b * n2, src, n2, s);
-    launch_mask_from_theta(src, out_masks + b * n2, n2, s);
-    res[b].iterations = std::max(st[b].k, 1);
-    res[b].final_pe = st[b].f;
-    res[b].final_grad_norm = st[b].gnorm;
-  }
-  LITHO_HIP(hipStreamSynchronize(s));
-  return res;
-}
+};
 
 }  // namespace
 
+// ------------------------------------------------------------------ CtmPool
+CtmPool::CtmPool(Plan& plan, int slots, const CtmOptions& opt) {
+  validate_ctm_options(opt);
+  if (slots <= 0 || slots > plan.capacity) throw Error(1, "CTM pool: slots must be in [1, plan capacity]");
+  if (opt.method == 0)
+    impl_ = std::make_unique<GdPool>(plan, slots, opt);
+  else
+    impl_ = std::make_unique<LbfgsPool>(plan, slots, opt);
+}
+CtmPool::~CtmPool() = default;
+int CtmPool::slots() const { return impl_->S; }
+int CtmPool::busy() const { return impl_->busy(); }
+bool CtmPool::admit(const CtmJob& job) { return impl_->admit(job); }
+void CtmPool::step() { impl_->step(); }
+std::vector<std::pair<long long, CtmTileResult>> CtmPool::take_finished() {
+  std::vector<std::pair<long long, CtmTileResult>> out;
+  out.swap(impl_->finished);
+  return out;
+}
+size_t CtmPool::slot_bytes(const Plan& plan, const CtmOptions& opt) {
+  const size_t n2 = plan.tile_elems() * sizeof(float);
+  return (opt.method == 0 ? 5 : 7 + 2 * (kMaxHist + 1)) * n2;
+}
+
 std::vector<CtmTileResult> run_ctm(Plan& plan, int batch, const float* targets, const float* theta0,
                                    const CtmOptions& opt, float* out_masks, float* out_theta) {
-  if (opt.method != 0 && opt.method != 1) throw Error(1, "CTM optimizer must be gradient_descent or lbfgs");
-  if (opt.iterations <= 0) throw Error(1, "CTM max_iteration must be positive");
-  if (!std::isfinite(opt.learning_rate) || opt.learning_rate <= 0.0)
-    throw Error(1, "CTM learning_rate must be finite and positive");
-  if (opt.history <= 0) throw Error(1, "CTM lbfgs_history_size must be positive");
-  if (!std::isfinite(opt.tolerance) || opt.tolerance < 0.0)
-    throw Error(1, "CTM gradient_tolerance must be finite and non-negative");
+  validate_ctm_options(opt);
   if (batch <= 0 || batch > plan.capacity) throw Error(1, "batch exceeds the plan capacity");
   if (targets == nullptr || out_masks == nullptr) throw Error(1, "CTM: target and output mask are required");
-  return opt.method == 0 ? run_gd(plan, batch, targets, theta0, opt, out_masks, out_theta)
-                         : run_lbfgs(plan, batch, targets, theta0, opt, out_masks, out_theta);
+  const size_t n2 = plan.tile_elems();
+  CtmPool pool(plan, batch, opt);
+  for (int b = 0; b < batch; ++b) {
+    CtmJob j;
+    j.id = b;
+    j.target = targets + b * n2;
+    j.theta0 = theta0 != nullptr ? theta0 + b * n2 : nullptr;
+    j.out_mask = out_masks + b * n2;
+    j.out_theta = out_theta != nullptr ? out_theta + b * n2 : nullptr;
+    pool.admit(j);
+  }
+  std::vector<CtmTileResult> res(batch);
+  while (pool.busy() > 0) {
+    pool.step();
+    for (auto& f : pool.take_finished()) res[(size_t)f.first] = std::move(f.second);
+  }
+  LITHO_HIP(hipStreamSynchronize(plan.ctx->stream));
+  return res;
 }
 
 }  // namespace litho

@@ -5,6 +5,7 @@
 //
 //   litho_bench eval  N K REPEATS [KFILE]   one CTM grad-eval per repeat (stage timings)
 //   litho_bench ctm   N K ITERS   KFILE [S] S L-BFGS CTM solves (default 3), each timed
+//   litho_bench evalb N K REPEATS BATCH     the same for BATCH equal tiles in one launch
 //   litho_bench save  N K KFILE             precompute the SOCS kernels and save them
 //   litho_bench copy  MIB                   device-to-device copy of MIB MiB (counter calibration)
 // With KFILE the kernels are read from a file written by `save` (no precompute).
@@ -66,13 +67,15 @@ int main(int argc, char** argv) {
   if (argc < 5) return 2;
   const int n = std::atoi(argv[2]), k = std::atoi(argv[3]);
   const int reps = mode == "save" ? 0 : std::atoi(argv[4]);
+  const int batch = mode == "evalb" && argc > 5 ? std::atoi(argv[5]) : 1;
+  if (batch < 1) return 2;
   litho_ctx* ctx = nullptr;
   check(litho_ctx_create(0, &ctx), "litho_ctx_create");
   int wr = 0, wc = 0, y0 = 0, x0 = 0, ns = 0;
   double wsum = 0.0;
   check(litho_socs_window(n, 4.0, 193.0, 1.35, 0.6, 0.9, &wr, &wc, &y0, &x0, &ns, &wsum), "litho_socs_window");
   std::vector<double> kern((size_t)k * wr * wc * 2), lam(k);
-  const char* kfile = argc > 5 ? argv[5] : (mode == "save" ? argv[4] : nullptr);
+  const char* kfile = mode == "evalb" ? nullptr : (argc > 5 ? argv[5] : (mode == "save" ? argv[4] : nullptr));
   if (kfile != nullptr && mode != "save") {
     FILE* f = std::fopen(kfile, "rb");
     if (f == nullptr || std::fread(kern.data(), sizeof(double), kern.size(), f) != kern.size() ||
@@ -96,20 +99,23 @@ int main(int argc, char** argv) {
     return 0;
   }
   litho_plan* plan = nullptr;
-  check(litho_plan_create(ctx, n, k, wr, wc, y0, x0, kern.data(), lam.data(), wsum, 1, &plan), "litho_plan_create");
-  const size_t bytes = (size_t)n * n * sizeof(float);
+  check(litho_plan_create(ctx, n, k, wr, wc, y0, x0, kern.data(), lam.data(), wsum, batch, &plan),
+        "litho_plan_create");
+  const size_t bytes = (size_t)batch * n * n * sizeof(float);
   float *target = nullptr, *theta = nullptr, *mask = nullptr;
   hip_check(hipMalloc(&target, bytes), "hipMalloc");
   hip_check(hipMalloc(&theta, bytes), "hipMalloc");
   hip_check(hipMalloc(&mask, bytes), "hipMalloc");
   const std::vector<float> t = lattice_target(n);
-  hip_check(hipMemcpy(target, t.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
-  check(litho_initialize_theta(ctx, target, theta, (size_t)n * n), "litho_initialize_theta");
-  if (mode == "eval") {
+  for (int b = 0; b < batch; ++b)
+    hip_check(hipMemcpy(target + (size_t)b * n * n, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice),
+              "hipMemcpy");
+  check(litho_initialize_theta(ctx, target, theta, (size_t)batch * n * n), "litho_initialize_theta");
+  if (mode == "eval" || mode == "evalb") {
     double st[4];
-    check(litho_profile_ctm_eval(plan, 1, theta, target, 0.225, 50.0, reps, st), "litho_profile_ctm_eval");
-    std::printf("n=%d k=%d window=%dx%d stages_ms forward=%.4f column=%.4f adjoint_tail=%.4f eval=%.4f\n", n, k, wr,
-                wc, st[0], st[1], st[2], st[3]);
+    check(litho_profile_ctm_eval(plan, batch, theta, target, 0.225, 50.0, reps, st), "litho_profile_ctm_eval");
+    std::printf("n=%d k=%d batch=%d window=%dx%d stages_ms forward=%.4f column=%.4f adjoint_tail=%.4f eval=%.4f\n", n,
+                k, batch, wr, wc, st[0], st[1], st[2], st[3]);
   } else if (mode == "ctm") {
     const int solves = argc > 6 ? std::atoi(argv[6]) : 3;
     litho_ctm_options o{1, reps, 0.9, 10, 1e-2, 0.225, 50.0};

@@ -7,11 +7,13 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "ctm_driver.hpp"
 #include "fullchip.hpp"
 #include "lsm.hpp"
 #include "optics.hpp"
 #include "plan.hpp"
+#include "server.hpp"
 #include "tiling.hpp"
 
 struct litho_ctx {
@@ -20,6 +22,10 @@ struct litho_ctx {
 
 struct litho_plan {
   litho::Plan* p;
+};
+
+struct litho_server {
+  litho::Server* s;
 };
 
 namespace {
@@ -63,6 +69,28 @@ litho::Context& ctx_of(litho_ctx* c) {
 
 }  // namespace
 
+namespace {
+
+litho::CtmOptions ctm_options(const litho_ctm_options& o) {
+  litho::CtmOptions c;
+  c.method = o.method;
+  c.iterations = o.iterations;
+  c.learning_rate = o.learning_rate;
+  c.history = o.lbfgs_history;
+  c.tolerance = o.gradient_tolerance;
+  c.threshold = o.threshold;
+  c.alpha = o.alpha;
+  return c;
+}
+void fill_info(const litho::CtmTileResult& r, litho_ctm_info* info) {
+  info->evaluations = r.evaluations;
+  info->iterations = r.iterations;
+  info->final_pe = r.final_pe;
+  info->final_grad_norm = r.final_grad_norm;
+  info->status = r.status;
+}
+}  // namespace
+
 extern "C" {
 
 const char* litho_last_error(void) { return g_last_error.c_str(); }
@@ -104,6 +132,13 @@ int litho_ctx_synchronize(litho_ctx* ctx) {
 }
 
 void* litho_ctx_stream(litho_ctx* ctx) { return ctx != nullptr ? (void*)ctx->c.stream : nullptr; }
+
+int litho_ctx_set_graphs(litho_ctx* ctx, int enable) {
+  return guarded([&] {
+    require(ctx != nullptr, "null context");
+    ctx->c.graphs = enable != 0;
+  });
+}
 
 int litho_socs_window(int n, double pixel_nm, double wavelength_nm, double na, double sigma_in, double sigma_out,
                       int* win_rows, int* win_cols, int* win_y0, int* win_x0, int* num_source_points,
@@ -237,6 +272,115 @@ int litho_ctm_eval(litho_plan* plan, int batch, const float* theta, const float*
       if (grad_norm2) grad_norm2[b] = red[4 * b + 2];
     }
   });
+}
+
+int litho_batch_ctm(litho_plan* const* plans, int num_plans, const litho_batch_tile* tiles, int num_tiles,
+                    const litho_ctm_options* options, int chunk, litho_ctm_info* info, int* chunk_used) {
+  return guarded([&] {
+    require(plans != nullptr && num_plans > 0 && options != nullptr, "null argument");
+    std::vector<litho::Plan*> ps;
+    for (int i = 0; i < num_plans; ++i) ps.push_back(&plan_of(plans[i]));
+    set_device(*ps[0]);
+    require(tiles != nullptr && num_tiles > 0, "no tiles");
+    std::vector<litho::BatchTile> bt(num_tiles);
+    for (int i = 0; i < num_tiles; ++i) {
+      bt[i].plan = tiles[i].plan_index;
+      bt[i].target = tiles[i].target;
+      bt[i].theta0 = tiles[i].theta0;
+      bt[i].out_mask = tiles[i].out_mask;
+      bt[i].out_theta = tiles[i].out_theta;
+    }
+    litho::CtmOptions o;
+    o.method = options->method;
+    o.iterations = options->iterations;
+    o.learning_rate = options->learning_rate;
+    o.history = options->lbfgs_history;
+    o.tolerance = options->gradient_tolerance;
+    o.threshold = options->threshold;
+    o.alpha = options->alpha;
+    require(o.method == 0 || o.method == 1, "CTM optimizer must be gradient_descent or lbfgs");
+    std::vector<int> used;
+    auto res = litho::run_batch(ps, bt.data(), num_tiles, o, chunk, &used);
+    for (int i = 0; i < num_tiles; ++i)
+      if (info != nullptr) {
+        info[i].evaluations = res[i].evaluations;
+        info[i].iterations = res[i].iterations;
+        info[i].final_pe = res[i].final_pe;
+        info[i].final_grad_norm = res[i].final_grad_norm;
+        info[i].status = res[i].status;
+      }
+    if (chunk_used != nullptr)
+      for (int i = 0; i < num_plans; ++i) chunk_used[i] = used[i];
+  });
+}
+
+int litho_server_create(litho_plan* const* plans, int num_plans, const litho_server_options* options,
+                        litho_server** out) {
+  return guarded([&] {
+    require(plans != nullptr && num_plans > 0 && options != nullptr && out != nullptr, "null argument");
+    std::vector<litho::Plan*> ps;
+    for (int i = 0; i < num_plans; ++i) ps.push_back(&plan_of(plans[i]));
+    set_device(*ps[0]);
+    litho::ServerOptions so;
+    so.ctm = ctm_options(options->ctm);
+    so.memory_budget = options->memory_budget_bytes;
+    auto* h = new litho_server();
+    try {
+      h->s = new litho::Server(ps, so);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int litho_server_destroy(litho_server* server) {
+  return guarded([&] {
+    if (server == nullptr) return;
+    delete server->s;
+    delete server;
+  });
+}
+
+int litho_server_submit(litho_server* server, int plan_index, const float* target, const float* theta0,
+                        float* out_mask, float* out_theta, long long* ticket) {
+  return guarded([&] {
+    require(server != nullptr && server->s != nullptr && ticket != nullptr, "null argument");
+    *ticket = server->s->submit(plan_index, target, theta0, out_mask, out_theta);
+  });
+}
+
+int litho_server_wait(litho_server* server, long long ticket, double timeout_s, litho_server_result* result) {
+  int timed_out = 0;
+  const int rc = guarded([&] {
+    require(server != nullptr && server->s != nullptr, "null server");
+    litho::ServedResult r;
+    if (!server->s->wait(ticket, timeout_s, &r)) {
+      timed_out = 1;
+      return;
+    }
+    if (r.error != 0) throw litho::Error(r.error, "server: request failed: " + server->s->error_message());
+    if (result != nullptr) {
+      fill_info(r.result, &result->info);
+      result->queued_ms = r.queued_ms;
+      result->service_ms = r.service_ms;
+    }
+  });
+  if (rc == LITHO_OK && timed_out) {
+    g_last_error = "request not finished";
+    return LITHO_ETIMEDOUT;
+  }
+  return rc;
+}
+
+size_t litho_server_request_bytes(const litho_server* server, int plan_index) {
+  if (server == nullptr || server->s == nullptr || plan_index < 0) return 0;
+  try {
+    return server->s->request_bytes(plan_index);
+  } catch (...) {
+    return 0;
+  }
 }
 
 int litho_ctm_optimize(litho_plan* plan, const float* target, const float* theta0, const litho_ctm_options* options,

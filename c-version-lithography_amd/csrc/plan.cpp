@@ -123,6 +123,48 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   LITHO_HIP(hipGetLastError());
 }
 
+EvalGraph::EvalGraph(Plan& plan, int batch, const EvalRequest& rq) : plan_(plan), batch_(batch), rq_(rq) {
+  if (batch <= 0 || batch > plan.capacity) throw Error(1, "batch exceeds the plan capacity");
+  if (!plan.ctx->graphs) return;
+  hipStream_t s = plan.ctx->stream;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  bool ok = true;
+  try {
+    run_eval(plan, batch, rq);
+  } catch (const Error&) {
+    ok = false;
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (!ok || e != hipSuccess || g == nullptr) {
+    if (g != nullptr) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (!ok) throw Error(3, "kernel launch failed while capturing the evaluation graph");
+    return;
+  }
+  graph_ = g;
+  if (hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    exec_ = nullptr;
+  }
+}
+
+EvalGraph::~EvalGraph() {
+  if (exec_ != nullptr) (void)hipGraphExecDestroy(exec_);
+  if (graph_ != nullptr) (void)hipGraphDestroy(graph_);
+}
+
+void EvalGraph::launch() {
+  if (exec_ != nullptr) {
+    LITHO_HIP(hipGraphLaunch(exec_, plan_.ctx->stream));
+  } else {
+    run_eval(plan_, batch_, rq_);
+  }
+}
+
 void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, double* stage_ms) {
   if (repeats <= 0) throw Error(1, "profile: repeats must be positive");
   hipStream_t s = plan.ctx->stream;

@@ -75,6 +75,7 @@ struct HostPinned {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool graphs = true;    // replay repeated evaluations from a captured hipGraph
   DevMem scratch;        // grow-only workspace of the level-set and tiling calls
   // At least `bytes` of workspace.  Growing it waits for the stream first, so
   // work still queued on the old buffer finishes before it is freed.
@@ -97,6 +98,8 @@ struct Plan {
   int col_blocks = 0;
   DevMem h, tw, row_freq, col_freq, row_of_u;
   int pruned = 0;                            // window-pruned column kernels in use
+  int tuned_chunk = 0;                       // batch size chosen by autotune_chunk (0 = not tuned)
+  int tuned_limit = 0;                       // the limit it was tuned for
   DevMem rt, mhat, t, v, y, p;
   DevMem img_partials, grad_partials, red;   // red: [capacity][4] = pe, bg, |g|^2, g.d
   DevMem coef;                               // [capacity] float, K1 step coefficients
@@ -144,6 +147,28 @@ struct EvalRequest {
 };
 
 void run_eval(Plan& plan, int batch, const EvalRequest& rq);
+
+// The launch sequence of run_eval(plan, batch, rq) captured once and replayed:
+// repeated evaluations on fixed buffers (the optimizer loops) pay one graph
+// launch instead of ~10 kernel launches.  The captured sequence has kernel
+// nodes only (no memset or copy nodes).  When the context disables graphs, or
+// capture fails, launch() issues the same kernels directly.
+class EvalGraph {
+ public:
+  EvalGraph(Plan& plan, int batch, const EvalRequest& rq);
+  ~EvalGraph();
+  EvalGraph(const EvalGraph&) = delete;
+  EvalGraph& operator=(const EvalGraph&) = delete;
+  void launch();
+  bool captured() const { return exec_ != nullptr; }
+
+ private:
+  Plan& plan_;
+  int batch_;
+  EvalRequest rq_;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+};
 
 // Average HIP-event time (ms) of the three launch groups of one evaluation on
 // the plan's stream: [0] forward half (3 kernels), [1] fused column

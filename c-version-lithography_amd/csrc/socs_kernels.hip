@@ -54,19 +54,39 @@ struct RowCfg {
 #ifndef LITHO_TW_LDS
 #define LITHO_TW_LDS 1
 #endif
+// threads of the column kernels for 256 <= N < 1024; 0 = about eight owned
+// elements per thread (LINES * N / 8, rounded up to whole waves)
+#ifndef LITHO_COL_NT_MID
+#define LITHO_COL_NT_MID 512
+#endif
+
+// register budget of the column kernels: 0 = bounded by the block size only,
+// 1 = at least the waves per SIMD that the LDS-limited block count provides
+#ifndef LITHO_COL_OCC_RULE
+#define LITHO_COL_OCC_RULE 0
+#endif
+
+constexpr int col_nt_mid(int lines, int n) {
+  return LITHO_COL_NT_MID > 0 ? LITHO_COL_NT_MID
+                              : ((lines * n / 8 + 63) / 64 * 64 > 1024 ? 1024 : (lines * n / 8 + 63) / 64 * 64);
+}
 
 template <int N>
 struct ColCfg {
   static constexpr int LINES = N >= 256 ? 8 : 16;
-  static constexpr int NT = N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? 256 : 64);
+  static constexpr int NT = N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? col_nt_mid(LINES, N) : 64);
   static constexpr int LS = fft::padded_stride(N);    // line stride (float2), swizzled lines
   static constexpr int ELEMS = LINES * N;
   static constexpr int PER = (ELEMS + NT - 1) / NT;     // owned elements per thread
-  // one block per CU (its LDS): allow the registers of NT/256 waves per SIMD
-  static constexpr int MAX_WAVES_PER_EU = NT >= 512 ? NT / 256 : 1;
   // twiddle table copied to LDS when it fits beside the lines (160 KB per CU)
   static constexpr bool TW_LDS =
       LITHO_TW_LDS && (size_t)LINES * LS * 8 + (size_t)N * 8 + (size_t)NT / 64 * 16 <= 160 * 1024;
+  static constexpr size_t LDS_BYTES = (size_t)LINES * LS * 8 + (TW_LDS ? (size_t)N * 8 : 0) + (size_t)NT / 64 * 16;
+  // register budget: as many waves per SIMD as the blocks that fit in one CU's LDS bring
+  static constexpr int BLOCKS_PER_CU = 160 * 1024 / LDS_BYTES > 0 ? (int)(160 * 1024 / LDS_BYTES) : 1;
+  static constexpr int LDS_WAVES = BLOCKS_PER_CU * NT / 256 > 8 ? 8 : (BLOCKS_PER_CU * NT / 256 > 0 ? BLOCKS_PER_CU * NT / 256 : 1);
+  static constexpr int MIN_WAVES_PER_EU = LITHO_COL_OCC_RULE == 1 ? LDS_WAVES : 1;
+  static constexpr int MAX_WAVES_PER_EU = LITHO_COL_OCC_RULE == 1 ? 8 : (NT >= 512 ? NT / 256 : 1);
 };
 
 // Twiddle source of the pruned column kernels: the LDS copy or the global
@@ -215,7 +235,7 @@ __device__ __forceinline__ void load_window_columns(float2* buf, const float2* _
 }
 
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_col_image_adj(SocsArgs a, ImageArgs im) {
   using C = ColCfg<N>;
   __shared__ float2 buf[C::LINES * C::LS];
@@ -409,7 +429,7 @@ k_final_row_inv(SocsArgs a) {
 
 // ---------------------------------------------------------------- K8
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_final_col_inv(SocsArgs a, GradArgs ga) {
   using C = ColCfg<N>;
   __shared__ float2 buf[C::LINES * C::LS];
@@ -474,7 +494,7 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
 // fft_columns.hpp.  Element ownership follows the last inverse stage:
 // column q % LINES, row i + r*TM with q = tid + s*NT, i = q / LINES.
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
   using C = ColCfg<N>;
   using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;
@@ -585,7 +605,7 @@ k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
 
 // ---------------------------------------------------------------- K8 pruned
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(1, ColCfg<N>::MAX_WAVES_PER_EU)))
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
   using C = ColCfg<N>;
   using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;

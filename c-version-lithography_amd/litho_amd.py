@@ -14,7 +14,10 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "liblitho_amd.so"
 
-OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, ERUNTIME = 0, 1, 2, 3, 4, 5
+OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED, ERUNTIME, ETIMEDOUT = 0, 1, 2, 3, 4, 5, 6
+
+
+_P_ = C.c_void_p
 
 
 class LithoError(RuntimeError):
@@ -31,6 +34,19 @@ class CtmOptions(C.Structure):
 
 class LsmOptions(C.Structure):
     _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("cfl", C.c_double), ("curvature_weight", C.c_double)]
+
+
+class CtmInfo(C.Structure):
+    _fields_ = [("evaluations", C.c_int), ("iterations", C.c_int), ("final_pe", C.c_double),
+                ("final_grad_norm", C.c_double), ("status", C.c_int)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("ctm", CtmOptions), ("memory_budget_bytes", C.c_double)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("info", CtmInfo), ("queued_ms", C.c_double), ("service_ms", C.c_double)]
 
 
 class Rect(C.Structure):
@@ -78,9 +94,8 @@ class TileLoss(C.Structure):
                 ("core_pe_loss", C.c_double), ("core_bg_penalty_raw", C.c_double)]
 
 
-class CtmInfo(C.Structure):
-    _fields_ = [("evaluations", C.c_int), ("iterations", C.c_int), ("final_pe", C.c_double),
-                ("final_grad_norm", C.c_double), ("status", C.c_int)]
+class BatchTile(C.Structure):
+    _fields_ = [("plan_index", C.c_int), ("target", _P_), ("theta0", _P_), ("out_mask", _P_), ("out_theta", _P_)]
 
 
 _P = C.c_void_p
@@ -97,6 +112,7 @@ SIGNATURES = {
     "litho_ctx_destroy": (_I, [_P]),
     "litho_ctx_synchronize": (_I, [_P]),
     "litho_ctx_stream": (_P, [_P]),
+    "litho_ctx_set_graphs": (_I, [_P, _I]),
     "litho_socs_window": (_I, [_I, _D, _D, _D, _D, _D, _IP, _IP, _IP, _IP, _IP, _DP]),
     "litho_socs_precompute": (_I, [_P, _I, _D, _D, _D, _D, _D, _I, _DP, _DP, _DP]),
     "litho_plan_create": (_I, [_P, _I, _I, _I, _I, _I, _I, _DP, _DP, _D, _I, C.POINTER(_P)]),
@@ -106,6 +122,13 @@ SIGNATURES = {
     "litho_pe_gradient": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _D, _D, _P, _P, _P, _DP, _DP]),
     "litho_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _P, _P, _DP, _DP]),
     "litho_ctm_optimize": (_I, [_P, _P, _P, C.POINTER(CtmOptions), _P, _P, _DP, _I, C.POINTER(CtmInfo)]),
+    "litho_batch_ctm": (_I, [C.POINTER(_P), _I, C.POINTER(BatchTile), _I, C.POINTER(CtmOptions), _I,
+                             C.POINTER(CtmInfo), _IP]),
+    "litho_server_create": (_I, [C.POINTER(_P), _I, C.POINTER(ServerOptions), C.POINTER(_P)]),
+    "litho_server_destroy": (_I, [_P]),
+    "litho_server_submit": (_I, [_P, _I, _P, _P, _P, _P, C.POINTER(C.c_longlong)]),
+    "litho_server_wait": (_I, [_P, C.c_longlong, _D, C.POINTER(ServerResult)]),
+    "litho_server_request_bytes": (C.c_size_t, [_P, _I]),
     "litho_initialize_theta": (_I, [_P, _P, _P, C.c_size_t]),
     "litho_profile_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _I, _DP]),
     "litho_ls_terms": (_I, [_P, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P]),
@@ -179,6 +202,9 @@ class Context:
     def synchronize(self):
         check(self.lib.litho_ctx_synchronize(self.h))
 
+    def set_graphs(self, enable: bool):
+        check(self.lib.litho_ctx_set_graphs(self.h, 1 if enable else 0))
+
     def __del__(self):
         try:
             self.close()
@@ -231,6 +257,75 @@ class Context:
     def partition_assemble(self, local, tiles, tile_size, out):
         h, w = out.shape
         check(self.lib.litho_partition_assemble(self.h, ptr(local), tiles, len(tiles), tile_size, h, w, ptr(out)))
+
+
+def batch_ctm(plans, tiles, method="lbfgs", iterations=30, learning_rate=0.9, history=10, tol=0.0,
+              threshold=0.25, alpha=50.0, chunk=0):
+    """litho_batch_ctm.  tiles: list of (plan_index, target, out_mask[, theta0[, out_theta]]) torch tensors.
+    Returns (list of info dicts, chunk used per plan)."""
+    lib = load()
+    hs = (_P * len(plans))(*[p.h for p in plans])
+    arr = (BatchTile * len(tiles))()
+    for i, t in enumerate(tiles):
+        pi, tg, om = t[0], t[1], t[2]
+        t0 = t[3] if len(t) > 3 else None
+        ot = t[4] if len(t) > 4 else None
+        arr[i] = BatchTile(pi, ptr(tg), ptr(t0), ptr(om), ptr(ot))
+    o = CtmOptions({"gradient_descent": 0, "lbfgs": 1}[method], iterations, learning_rate, history, tol,
+                   threshold, alpha)
+    info = (CtmInfo * len(tiles))()
+    used = (C.c_int * len(plans))()
+    check(lib.litho_batch_ctm(hs, len(plans), arr, len(tiles), C.byref(o), chunk, info, used))
+    out = [dict(evaluations=r.evaluations, iterations=r.iterations, final_pe=r.final_pe,
+                final_grad_norm=r.final_grad_norm, status=r.status) for r in info]
+    return out, list(used)
+
+
+class Server:
+    """litho_server: continuous-batching CTM over several plans (tile sizes)."""
+
+    def __init__(self, plans, method="gradient_descent", iterations=20, learning_rate=0.9, history=10, tol=0.0,
+                 threshold=0.25, alpha=50.0, memory_budget=64.0 * 2**30):
+        self.lib = load()
+        self.plans = list(plans)
+        hs = (_P * len(plans))(*[p.h for p in plans])
+        o = ServerOptions(CtmOptions({"gradient_descent": 0, "lbfgs": 1}[method], iterations, learning_rate,
+                                     history, tol, threshold, alpha), memory_budget)
+        h = _P()
+        check(self.lib.litho_server_create(hs, len(plans), C.byref(o), C.byref(h)))
+        self.h = h
+
+    def submit(self, plan_index, target, out_mask, theta0=None, out_theta=None) -> int:
+        t = C.c_longlong()
+        check(self.lib.litho_server_submit(self.h, plan_index, ptr(target), ptr(theta0), ptr(out_mask),
+                                           ptr(out_theta), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """dict(evaluations, ..., queued_ms, service_ms), or None on timeout."""
+        r = ServerResult()
+        code = self.lib.litho_server_wait(self.h, ticket, timeout, C.byref(r))
+        if code == ETIMEDOUT:
+            return None
+        check(code)
+        i = r.info
+        return dict(evaluations=i.evaluations, iterations=i.iterations, final_pe=i.final_pe,
+                    final_grad_norm=i.final_grad_norm, status=i.status, queued_ms=r.queued_ms,
+                    service_ms=r.service_ms)
+
+    def request_bytes(self, plan_index) -> int:
+        return int(self.lib.litho_server_request_bytes(self.h, plan_index))
+
+    def close(self):
+        if self.h:
+            check(self.lib.litho_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def partition_build(height, width, grid_rows, grid_cols, halo, guard):

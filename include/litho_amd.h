@@ -32,6 +32,7 @@ extern "C" {
 #define LITHO_EDEVICE 3       /* HIP runtime error */
 #define LITHO_EUNSUPPORTED 4  /* tile size without a compiled FFT */
 #define LITHO_ERUNTIME 5      /* std::runtime_error in the reference */
+#define LITHO_ETIMEDOUT 6     /* litho_server_wait: the request is not finished yet */
 
 typedef struct litho_ctx litho_ctx;
 typedef struct litho_plan litho_plan;
@@ -48,6 +49,9 @@ int litho_ctx_destroy(litho_ctx* ctx);
 int litho_ctx_synchronize(litho_ctx* ctx);
 /* The context's hipStream_t, for callers that order their own work on it. */
 void* litho_ctx_stream(litho_ctx* ctx);
+/* Optimizer loops replay each repeated evaluation from a captured hipGraph
+ * (default on); 0 launches the same kernels one by one.  Results are identical. */
+int litho_ctx_set_graphs(litho_ctx* ctx, int enable);
 
 /* ---------------------------------------------------------------- optics
  * Compact SOCS frequency window of the reference optics
@@ -137,6 +141,60 @@ typedef struct {
 int litho_ctm_optimize(litho_plan* plan, const float* target, const float* theta0,
                        const litho_ctm_options* options, float* out_mask, float* out_theta,
                        double* history, int history_capacity, litho_ctm_info* info);
+
+/* Batched CTM over independent tiles of several sizes: the reference's one
+ * CTM_Optimizer per tile (demo_CTM_parallel_benchmark.cpp, OpenMP over tiles).
+ * Tiles of one size share plans[plan_index]; they are solved `chunk` at a time,
+ * each chunk one batched optimizer run whose tiles keep their own trajectory
+ * (identical to solving each tile alone).  chunk = 0 chooses the chunk per
+ * tile size by timing batched evaluations (cached in the plan).  All plans must
+ * share one context.  info (host, num_tiles, nullable); chunk_used (host,
+ * num_plans, nullable) receives the chunk each plan ran with. */
+typedef struct {
+  int plan_index;
+  const float* target;   /* device N x N */
+  const float* theta0;   /* device N x N, nullable: initialize_theta(target) */
+  float* out_mask;       /* device N x N */
+  float* out_theta;      /* device N x N, nullable */
+} litho_batch_tile;
+
+int litho_batch_ctm(litho_plan* const* plans, int num_plans, const litho_batch_tile* tiles, int num_tiles,
+                    const litho_ctm_options* options, int chunk, litho_ctm_info* info, int* chunk_used);
+
+/* Served CTM with continuous batching (no counterpart in the reference, which
+ * solves one tile per optimizer object): a worker thread keeps one optimizer
+ * pool per plan (tile size) with plan capacity slots, admits queued requests in
+ * arrival order whenever a slot is free and the request state in flight stays
+ * within memory_budget_bytes, runs one batched round of every busy pool at a
+ * time and releases each request when its own optimizer finishes.  A request's
+ * outputs equal those of solving it alone.  All plans must share one context,
+ * which the server's worker then owns until litho_server_destroy (no other
+ * calls on that context meanwhile).  Request buffers (device N x N) must stay
+ * valid until the request is waited for. */
+typedef struct litho_server litho_server;
+
+typedef struct {
+  litho_ctm_options ctm;
+  double memory_budget_bytes;
+} litho_server_options;
+
+typedef struct {
+  litho_ctm_info info;
+  double queued_ms;      /* submit -> admission */
+  double service_ms;     /* admission -> completion */
+} litho_server_result;
+
+int litho_server_create(litho_plan* const* plans, int num_plans, const litho_server_options* options,
+                        litho_server** out);
+/* Finishes every admitted and queued request, then stops the worker. */
+int litho_server_destroy(litho_server* server);
+int litho_server_submit(litho_server* server, int plan_index, const float* target, const float* theta0,
+                        float* out_mask, float* out_theta, long long* ticket);
+/* Waits up to timeout_s seconds (negative: no limit).  LITHO_OK: done, result
+ * filled and the ticket released; LITHO_ETIMEDOUT: not finished yet. */
+int litho_server_wait(litho_server* server, long long ticket, double timeout_s, litho_server_result* result);
+/* Device bytes one request of this plan counts against the budget. */
+size_t litho_server_request_bytes(const litho_server* server, int plan_index);
 
 /* Timing of one CTM grad-eval by HIP events on the context stream (no reference
  * counterpart; the reference times with std::chrono, demo_CTM.cpp:72-98).
