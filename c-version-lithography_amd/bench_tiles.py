@@ -5,7 +5,7 @@
       per side -> N = 640 / 896 / 1152 (SOCS windows 67 / 97 / 123), K = 24,
       30 L-BFGS iterations (m = 10), seeded Manhattan targets.  Plain: every
       tile solved alone (litho_ctm_optimize, one tile per call).  Batched:
-      litho_batch_ctm with autotuned chunks.  Prints one JSON line with
+      litho_batch_ctm with autotuned chunks, one context (stream) per size.  Prints one JSON line with
       grad-evals/s of both and their ratio.
 
   python c-version-lithography_amd/bench_tiles.py served [--requests 160] [--iterations 20] [--load 0.7]
@@ -61,12 +61,12 @@ def make_plan(ctx, n, capacity):
 
 def batched(args):
     import torch
-    ctx = la.Context(0)
     sizes = [512 + 2 * 64, 768 + 2 * 64, 1024 + 2 * 64]
     per = args.tiles // 3
+    ctxs = [la.Context(0) for _ in sizes]     # one stream per tile size: the groups run side by side
     plans, wins = [], []
     t0 = time.perf_counter()
-    for n in sizes:
+    for ctx, n in zip(ctxs, sizes):
         p, d = make_plan(ctx, n, per)
         plans.append(p)
         wins.append(d["rows"])
@@ -110,18 +110,19 @@ def batched(args):
             "identical_masks": same}
     for p in plans:
         p.close()
-    ctx.close()
+    for ctx in ctxs:
+        ctx.close()
     print(json.dumps(line), flush=True)
 
 
 def served(args):
     import torch
-    ctx = la.Context(0)
     sizes = [256 + 128, 512 + 128, 1024 + 128, 2048 + 128]
     caps = [16, 16, 8, 4]
+    ctxs = [la.Context(0) for _ in sizes]     # one worker and stream per tile size
     plans, wins = [], []
     t0 = time.perf_counter()
-    for n, c in zip(sizes, caps):
+    for ctx, n, c in zip(ctxs, sizes, caps):
         p, d = make_plan(ctx, n, c)
         plans.append(p)
         wins.append(d["rows"])
@@ -193,7 +194,8 @@ def served(args):
             "identical_masks_alone_vs_mixed": same}
     for p in plans:
         p.close()
-    ctx.close()
+    for ctx in ctxs:
+        ctx.close()
     print(json.dumps(line), flush=True)
 
 

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
+#include <thread>
 
 #include "opt_kernels.hpp"
 
@@ -76,6 +78,42 @@ int autotune_chunk(Plan& plan, const std::vector<const float*>& targets, int lim
   return pick;
 }
 
+namespace {
+
+// The tiles of one plan through a refilling pool of c slots.
+void run_plan_group(Plan& plan, const std::vector<int>& ids, const BatchTile* tiles, const CtmOptions& opt,
+                    int chunk, std::vector<CtmTileResult>& res, int* chunk_used) {
+  int c = chunk;
+  if (c == 0) {
+    std::vector<const float*> tg;
+    for (int i : ids) tg.push_back(tiles[i].target);
+    c = autotune_chunk(plan, tg, (int)ids.size(), opt);
+  }
+  c = std::max(1, std::min({c, plan.capacity, (int)ids.size()}));
+  *chunk_used = c;
+  // c slots; a slot whose tile finished takes the next queued tile at once
+  CtmPool pool(plan, c, opt);
+  size_t next = 0;
+  while (next < ids.size() || pool.busy() > 0) {
+    while (next < ids.size()) {
+      const BatchTile& t = tiles[ids[next]];
+      CtmJob j;
+      j.id = ids[next];
+      j.target = t.target;
+      j.theta0 = t.theta0;
+      j.out_mask = t.out_mask;
+      j.out_theta = t.out_theta;
+      if (!pool.admit(j)) break;
+      ++next;
+    }
+    pool.step();
+    for (auto& f : pool.take_finished()) res[(size_t)f.first] = std::move(f.second);
+  }
+  LITHO_HIP(hipStreamSynchronize(plan.ctx->stream));
+}
+
+}  // namespace
+
 std::vector<CtmTileResult> run_batch(const std::vector<Plan*>& plans, const BatchTile* tiles, int num_tiles,
                                      const CtmOptions& opt, int chunk, std::vector<int>* chunk_used) {
   validate_ctm_options(opt);
@@ -83,49 +121,47 @@ std::vector<CtmTileResult> run_batch(const std::vector<Plan*>& plans, const Batc
   if (num_tiles <= 0 || tiles == nullptr) throw Error(1, "batch: no tiles");
   if (chunk < 0) throw Error(1, "batch: chunk must be >= 0 (0 = autotune)");
   for (const Plan* p : plans)
-    if (p == nullptr || p->ctx != plans[0]->ctx) throw Error(1, "batch: plans must share one context");
+    if (p == nullptr) throw Error(1, "batch: null plan");
   for (int i = 0; i < num_tiles; ++i) {
     const BatchTile& t = tiles[i];
     if (t.plan < 0 || t.plan >= (int)plans.size()) throw Error(1, "batch: tile plan index out of range");
     if (t.target == nullptr || t.out_mask == nullptr) throw Error(1, "batch: tile target and output are required");
   }
-  hipStream_t s = plans[0]->ctx->stream;
   std::vector<CtmTileResult> res(num_tiles);
-  if (chunk_used != nullptr) chunk_used->assign(plans.size(), 0);
-  for (int pi = 0; pi < (int)plans.size(); ++pi) {
-    Plan& plan = *plans[pi];
-    std::vector<int> ids;
-    for (int i = 0; i < num_tiles; ++i)
-      if (tiles[i].plan == pi) ids.push_back(i);
-    if (ids.empty()) continue;
-    int c = chunk;
-    if (c == 0) {
-      std::vector<const float*> tg;
-      for (int i : ids) tg.push_back(tiles[i].target);
-      c = autotune_chunk(plan, tg, (int)ids.size(), opt);
+  std::vector<int> used(plans.size(), 0);
+  // plans on one context run one after another on its stream; distinct
+  // contexts run side by side, one host thread each
+  std::vector<Context*> ctxs;
+  for (Plan* p : plans)
+    if (std::find(ctxs.begin(), ctxs.end(), p->ctx) == ctxs.end()) ctxs.push_back(p->ctx);
+  auto run_ctx = [&](Context* c) {
+    LITHO_HIP(hipSetDevice(c->device));
+    for (int pi = 0; pi < (int)plans.size(); ++pi) {
+      if (plans[pi]->ctx != c) continue;
+      std::vector<int> ids;
+      for (int i = 0; i < num_tiles; ++i)
+        if (tiles[i].plan == pi) ids.push_back(i);
+      if (!ids.empty()) run_plan_group(*plans[pi], ids, tiles, opt, chunk, res, &used[pi]);
     }
-    c = std::max(1, std::min({c, plan.capacity, (int)ids.size()}));
-    if (chunk_used != nullptr) (*chunk_used)[pi] = c;
-    // c slots; a slot whose tile finished takes the next queued tile at once
-    CtmPool pool(plan, c, opt);
-    size_t next = 0;
-    while (next < ids.size() || pool.busy() > 0) {
-      while (next < ids.size()) {
-        const BatchTile& t = tiles[ids[next]];
-        CtmJob j;
-        j.id = ids[next];
-        j.target = t.target;
-        j.theta0 = t.theta0;
-        j.out_mask = t.out_mask;
-        j.out_theta = t.out_theta;
-        if (!pool.admit(j)) break;
-        ++next;
-      }
-      pool.step();
-      for (auto& f : pool.take_finished()) res[(size_t)f.first] = std::move(f.second);
-    }
+  };
+  if (ctxs.size() == 1) {
+    run_ctx(ctxs[0]);
+  } else {
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> err(ctxs.size());
+    for (size_t k = 0; k < ctxs.size(); ++k)
+      th.emplace_back([&, k] {
+        try {
+          run_ctx(ctxs[k]);
+        } catch (...) {
+          err[k] = std::current_exception();
+        }
+      });
+    for (auto& t : th) t.join();
+    for (auto& e : err)
+      if (e) std::rethrow_exception(e);
   }
-  LITHO_HIP(hipStreamSynchronize(s));
+  if (chunk_used != nullptr) *chunk_used = used;
   return res;
 }
 

@@ -5,7 +5,8 @@
 // one captured graph; a slot whose tile finished takes the next queued tile at
 // the next round, so batches stay full until the queue drains.  The chunk size
 // per tile size is chosen by timing batched evaluations (autotune) unless the
-// caller fixes it.
+// caller fixes it.  Plans on distinct contexts (streams) are driven by one host
+// thread each, so their rounds overlap on the GPU.
 #pragma once
 #include <vector>
 

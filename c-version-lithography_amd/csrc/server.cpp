@@ -9,7 +9,7 @@ Server::Server(const std::vector<Plan*>& plans, const ServerOptions& opt) : plan
   validate_ctm_options(opt.ctm);
   if (plans.empty()) throw Error(1, "server: no plans");
   for (const Plan* p : plans)
-    if (p == nullptr || p->ctx != plans[0]->ctx) throw Error(1, "server: plans must share one context");
+    if (p == nullptr) throw Error(1, "server: null plan");
   if (!(opt.memory_budget > 0.0)) throw Error(1, "server: memory budget must be positive");
   for (Plan* p : plans) {
     pools_.push_back(std::make_unique<CtmPool>(*p, p->capacity, opt.ctm));
@@ -18,7 +18,10 @@ Server::Server(const std::vector<Plan*>& plans, const ServerOptions& opt) : plan
   }
   for (size_t b : bytes_)
     if ((double)b > opt.memory_budget) throw Error(2, "server: one request exceeds the memory budget");
-  worker_ = std::thread([this] { work(); });
+  std::vector<Context*> ctxs;
+  for (Plan* p : plans)
+    if (std::find(ctxs.begin(), ctxs.end(), p->ctx) == ctxs.end()) ctxs.push_back(p->ctx);
+  for (Context* c : ctxs) workers_.emplace_back([this, c] { work(c); });
 }
 
 Server::~Server() {
@@ -27,7 +30,8 @@ Server::~Server() {
     stop_ = true;
   }
   cv_work_.notify_all();
-  if (worker_.joinable()) worker_.join();
+  for (auto& w : workers_)
+    if (w.joinable()) w.join();
 }
 
 size_t Server::request_bytes(int plan_index) const { return bytes_.at((size_t)plan_index); }
@@ -53,7 +57,7 @@ long long Server::submit(int plan_index, const float* target, const float* theta
     p.submitted = Clock::now();
     queue_.push_back(p);
   }
-  cv_work_.notify_one();
+  cv_work_.notify_all();
   return t;
 }
 
@@ -70,27 +74,34 @@ bool Server::wait(long long ticket, double timeout_s, ServedResult* out) {
   return true;
 }
 
-void Server::work() {
+void Server::work(Context* ctx) {
+  std::vector<int> mine;
+  for (int i = 0; i < (int)plans_.size(); ++i)
+    if (plans_[i]->ctx == ctx) mine.push_back(i);
+  auto is_mine = [&](int plan) { return std::find(mine.begin(), mine.end(), plan) != mine.end(); };
+  auto busy = [&] {
+    for (int i : mine)
+      if (pools_[(size_t)i]->busy() > 0) return true;
+    return false;
+  };
+  auto queued = [&] {
+    for (const auto& q : queue_)
+      if (is_mine(q.plan)) return true;
+    return false;
+  };
   try {
-    LITHO_HIP(hipSetDevice(plans_[0]->ctx->device));
-    hipStream_t s = plans_[0]->ctx->stream;
+    LITHO_HIP(hipSetDevice(ctx->device));
     for (;;) {
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_work_.wait(lk, [&] {
-          if (!queue_.empty()) return true;
-          for (auto& p : pools_)
-            if (p->busy() > 0) return true;
-          return stop_;
-        });
-        bool busy = false;
-        for (auto& p : pools_) busy = busy || p->busy() > 0;
-        if (stop_ && queue_.empty() && !busy) break;
+        cv_work_.wait(lk, [&] { return queued() || busy() || stop_ || error_ != 0; });
+        if (error_ != 0) break;
+        if (stop_ && !queued() && !busy()) break;
         // admission in arrival order: a request waits only for its own size's slots or the budget
         const auto now = Clock::now();
         for (auto it = queue_.begin(); it != queue_.end();) {
           const double b = (double)bytes_[(size_t)it->plan];
-          if (in_flight_ + b <= opt_.memory_budget && pools_[(size_t)it->plan]->admit(it->job)) {
+          if (is_mine(it->plan) && in_flight_ + b <= opt_.memory_budget && pools_[(size_t)it->plan]->admit(it->job)) {
             in_flight_ += b;
             running_[it->ticket] = Running{it->plan, it->submitted, now};
             it = queue_.erase(it);
@@ -98,16 +109,22 @@ void Server::work() {
             ++it;
           }
         }
+        // nothing admitted and nothing running: the budget is held by other sizes
+        if (!busy()) {
+          cv_work_.wait_for(lk, std::chrono::milliseconds(1));
+          continue;
+        }
       }
       // one round of every busy pool (outside the lock: submissions continue meanwhile)
       std::vector<std::pair<long long, CtmTileResult>> fin;
-      for (auto& p : pools_) {
-        if (p->busy() == 0) continue;
-        p->step();
-        for (auto& f : p->take_finished()) fin.push_back(std::move(f));
+      for (int i : mine) {
+        CtmPool& p = *pools_[(size_t)i];
+        if (p.busy() == 0) continue;
+        p.step();
+        for (auto& f : p.take_finished()) fin.push_back(std::move(f));
       }
       if (fin.empty()) continue;
-      LITHO_HIP(hipStreamSynchronize(s));    // outputs of the finished requests are written
+      LITHO_HIP(hipStreamSynchronize(ctx->stream));    // outputs of the finished requests are written
       const auto now = Clock::now();
       {
         std::lock_guard<std::mutex> lk(mu_);
@@ -123,6 +140,7 @@ void Server::work() {
         }
       }
       cv_done_.notify_all();
+      cv_work_.notify_all();     // freed budget may let another size admit
     }
   } catch (const Error& e) {
     std::lock_guard<std::mutex> lk(mu_);
@@ -130,10 +148,12 @@ void Server::work() {
     error_msg_ = e.what();
   } catch (const std::exception& e) {
     std::lock_guard<std::mutex> lk(mu_);
-    error_ = 5;
-    error_msg_ = e.what();
+    if (error_ == 0) {
+      error_ = 5;
+      error_msg_ = e.what();
+    }
   }
-  // fail whatever is still pending so that no waiter blocks forever
+  // after a failure, fail whatever is still pending so that no waiter blocks forever
   std::lock_guard<std::mutex> lk(mu_);
   if (error_ != 0) {
     for (auto& q : queue_) done_[q.ticket].error = error_;
@@ -142,6 +162,7 @@ void Server::work() {
     running_.clear();
   }
   cv_done_.notify_all();
+  cv_work_.notify_all();
 }
 
 }  // namespace litho

@@ -1,9 +1,11 @@
 // Served CTM with continuous batching: requests (one tile each, any of the
-// plans' sizes) arrive at any time; a worker thread keeps one optimizer pool
-// per tile size, admits queued requests into free slots at every round (in
-// arrival order, within a device-memory budget) and releases each request as
-// soon as its own optimizer finishes.  A request's result does not depend on
-// what it shares its batches with.
+// plans' sizes) arrive at any time; worker threads keep one optimizer pool per
+// tile size, admit queued requests into free slots at every round (in arrival
+// order, within a device-memory budget shared by all sizes) and release each
+// request as soon as its own optimizer finishes.  Plans that share a context
+// are served by one worker on its stream; each further context gets its own
+// worker, so the rounds of different sizes overlap on the GPU.  A request's
+// result does not depend on what it shares its batches with.
 #pragma once
 #include <chrono>
 #include <condition_variable>
@@ -54,7 +56,7 @@ class Server {
     int plan;
     Clock::time_point submitted, admitted;
   };
-  void work();
+  void work(Context* ctx);
 
   std::vector<Plan*> plans_;
   ServerOptions opt_;
@@ -70,7 +72,7 @@ class Server {
   bool stop_ = false;
   int error_ = 0;
   std::string error_msg_;
-  std::thread worker_;
+  std::vector<std::thread> workers_;
 
  public:
   const std::string& error_message() const { return error_msg_; }

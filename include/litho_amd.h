@@ -147,9 +147,11 @@ int litho_ctm_optimize(litho_plan* plan, const float* target, const float* theta
  * Tiles of one size share plans[plan_index]; they are solved `chunk` at a time,
  * each chunk one batched optimizer run whose tiles keep their own trajectory
  * (identical to solving each tile alone).  chunk = 0 chooses the chunk per
- * tile size by timing batched evaluations (cached in the plan).  All plans must
- * share one context.  info (host, num_tiles, nullable); chunk_used (host,
- * num_plans, nullable) receives the chunk each plan ran with. */
+ * tile size by timing batched evaluations (cached in the plan).  Plans that
+ * share a context run one after another on its stream; plans on distinct
+ * contexts run concurrently (one host thread per context).  info (host,
+ * num_tiles, nullable); chunk_used (host, num_plans, nullable) receives the
+ * chunk each plan ran with. */
 typedef struct {
   int plan_index;
   const float* target;   /* device N x N */
@@ -165,11 +167,13 @@ int litho_batch_ctm(litho_plan* const* plans, int num_plans, const litho_batch_t
  * solves one tile per optimizer object): a worker thread keeps one optimizer
  * pool per plan (tile size) with plan capacity slots, admits queued requests in
  * arrival order whenever a slot is free and the request state in flight stays
- * within memory_budget_bytes, runs one batched round of every busy pool at a
- * time and releases each request when its own optimizer finishes.  A request's
- * outputs equal those of solving it alone.  All plans must share one context,
- * which the server's worker then owns until litho_server_destroy (no other
- * calls on that context meanwhile).  Request buffers (device N x N) must stay
+ * within memory_budget_bytes, runs batched rounds of the busy pools and
+ * releases each request when its own optimizer finishes.  A request's
+ * outputs equal those of solving it alone.  Plans that share a context are
+ * served by one worker thread on its stream; each further context gets a
+ * worker of its own (their rounds overlap on the GPU); the memory budget is
+ * shared.  The workers own the plans' contexts until litho_server_destroy (no
+ * other calls on them meanwhile).  Request buffers (device N x N) must stay
  * valid until the request is waited for. */
 typedef struct litho_server litho_server;
 

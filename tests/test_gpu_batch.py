@@ -88,3 +88,26 @@ def test_batch_rejects_bad_plan_index(gpu_ctx):
     with pytest.raises(la.LithoError):
         la.batch_ctm([plan], [(1, t, t)])
     plan.close()
+
+
+def test_batch_across_contexts_equals_single():
+    """Plans on distinct contexts run concurrently (one host thread each)."""
+    import torch
+    la = litho_amd()
+    sizes = [64, 96]
+    ctxs = [la.Context(0) for _ in sizes]
+    plans = [la.Plan.from_socs(c, socs(n), capacity=2) for c, n in zip(ctxs, sizes)]
+    tiles = []
+    for i in range(6):
+        t = dev(targets().manhattan_target(sizes[i % 2], 90 + i))
+        tiles.append((i % 2, t, torch.zeros_like(t)))
+    info, used = la.batch_ctm(plans, tiles, method="lbfgs", iterations=5, threshold=0.225, alpha=50.0, chunk=2)
+    torch.cuda.synchronize()
+    for (pi, t, out), inf in zip(tiles, info):
+        ref = torch.zeros_like(t)
+        r = plans[pi].ctm_optimize(t, ref, method="lbfgs", iterations=5, threshold=0.225, alpha=50.0)
+        assert r["evaluations"] == inf["evaluations"] and torch.equal(out, ref)
+    for p in plans:
+        p.close()
+    for c in ctxs:
+        c.close()

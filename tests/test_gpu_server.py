@@ -99,3 +99,29 @@ def test_server_wait_timeout_and_bad_ticket():
     srv.close()
     plan.close()
     ctx.close()
+
+
+def test_server_across_contexts_equals_alone():
+    """Plans on distinct contexts: one worker per context, results unchanged."""
+    import torch
+    la = litho_amd()
+    sizes = [64, 96]
+    ctxs = [la.Context(0) for _ in sizes]
+    plans = [la.Plan.from_socs(c, socs(n), capacity=2) for c, n in zip(ctxs, sizes)]
+    reqs = []
+    for i in range(6):
+        t = dev(targets().manhattan_target(sizes[i % 2], 70 + i))
+        reqs.append((i % 2, t, torch.zeros_like(t)))
+    refs = [alone(plans[pi], t, "gradient_descent", 5) for pi, t, _ in reqs]
+    torch.cuda.synchronize()
+    srv = la.Server(plans, iterations=5, threshold=0.225, alpha=50.0)
+    tickets = [srv.submit(pi, t, out) for pi, t, out in reqs]
+    results = [srv.wait(tk) for tk in tickets]
+    srv.close()
+    for (pi, t, out), r, (ref, ref_out) in zip(reqs, results, refs):
+        assert r["final_pe"] == ref["final_pe"]
+        assert torch.equal(out, ref_out)
+    for p in plans:
+        p.close()
+    for c in ctxs:
+        c.close()
