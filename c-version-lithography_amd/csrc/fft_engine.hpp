@@ -31,6 +31,8 @@ constexpr Radices factorize(int n) {
   if (rem == 3) f.r[f.count++] = 8;
   if (rem == 2) f.r[f.count++] = 4;
   if (rem == 1) f.r[f.count++] = 2;
+  // odd part: 3 * 3 as one radix-9 pass (two radix-3 layers in registers), then the primes
+  while (n % 9 == 0) { n /= 9; f.r[f.count++] = 9; }
   const int odd[6] = {3, 5, 7, 11, 13, 17};
   for (int i = 0; i < 6; ++i)
     while (n % odd[i] == 0) { n /= odd[i]; f.r[f.count++] = odd[i]; }
@@ -183,6 +185,11 @@ struct Dft<8, DIR> {
 template <int DIR>
 struct Dft<16, DIR> {
   __device__ __forceinline__ static void run(float2* v) { DftAB<4, 4, DIR>::run(v); }
+};
+
+template <int DIR>
+struct Dft<9, DIR> {
+  __device__ __forceinline__ static void run(float2* v) { DftAB<3, 3, DIR>::run(v); }
 };
 
 // odd radix: direct O(R^2) with compile-time constants, using the symmetric

@@ -71,10 +71,25 @@ constexpr int col_nt_mid(int lines, int n) {
                               : ((lines * n / 8 + 63) / 64 * 64 > 1024 ? 1024 : (lines * n / 8 + 63) / 64 * 64);
 }
 
+// columns per block of the column kernels for N >= 1024
+#ifndef LITHO_COL_LINES_LARGE
+#define LITHO_COL_LINES_LARGE 8
+#endif
+
+// largest radix of the factorisation: 11, 13 and 17 run as direct O(R^2)
+// butterflies whose registers crowd out the 1024-thread, 8-column blocks
+constexpr int max_radix(int n) {
+  const fft::Radices f = fft::factorize(n);
+  int m = 0;
+  for (int i = 0; i < f.count; ++i) m = f.r[i] > m ? f.r[i] : m;
+  return m;
+}
+
 template <int N>
 struct ColCfg {
-  static constexpr int LINES = N >= 256 ? 8 : 16;
-  static constexpr int NT = N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? col_nt_mid(LINES, N) : 64);
+  static constexpr bool BIG_ODD = N >= 1024 && max_radix(N) > 16;
+  static constexpr int LINES = BIG_ODD ? 4 : (N >= 1024 ? LITHO_COL_LINES_LARGE : (N >= 256 ? 8 : 16));
+  static constexpr int NT = BIG_ODD ? 512 : (N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? col_nt_mid(LINES, N) : 64));
   static constexpr int LS = fft::padded_stride(N);    // line stride (float2), swizzled lines
   static constexpr int ELEMS = LINES * N;
   static constexpr int PER = (ELEMS + NT - 1) / NT;     // owned elements per thread

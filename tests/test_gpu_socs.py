@@ -61,10 +61,27 @@ def rel_l2(a, ref):
     return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
 
 
+def window_kernels(n, k):
+    """Random complex kernels on the true SOCS window of the optics at size n:
+    the forward / adjoint identities hold for any kernels, so these pin the
+    pruned transforms at tile sizes whose SVD would be slow here."""
+    r, c, npts, wsum, y0, x0 = op.source_summary(n, **OPT)
+    rng = np.random.default_rng(n + k)
+    kern = rng.standard_normal((k, r, c)) + 1j * rng.standard_normal((k, r, c))
+    w = np.sort(rng.random(k) + 0.05)[::-1] * wsum / k
+    # scaled so that a clear mask images to I = sum_k w_k |H_k(DC)|^2 / sum w = 1, as the optics do
+    dc = kern[:, n // 2 - y0, n // 2 - x0]
+    kern = kern / np.sqrt(np.sum(w * np.abs(dc) ** 2) / wsum)
+    return op.SocsKernels(n=n, win_y0=y0, win_x0=x0, kernels=kern, weights=w, weight_sum=wsum)
+
+
 def make_case(kind, n, k=24, seed=0):
     if kind == "synthetic":
         s = Synth(n)
         cache = s.cache
+    elif kind == "window":
+        s = window_kernels(n, k)
+        cache = im.Cache.from_socs(s)
     else:
         s = socs(n, k)
         cache = im.Cache.from_socs(s)
@@ -73,7 +90,8 @@ def make_case(kind, n, k=24, seed=0):
     return s, cache, tgt
 
 
-CASES = [("synthetic", 20, 2), ("synthetic", 24, 2), ("socs", 64, 8), ("socs", 128, 24), ("socs", 512, 24)]
+CASES = [("synthetic", 20, 2), ("synthetic", 24, 2), ("socs", 64, 8), ("socs", 128, 24), ("socs", 512, 24),
+         ("window", 384, 6), ("window", 640, 6), ("window", 896, 6), ("window", 1152, 6), ("window", 2176, 4)]
 
 
 @pytest.mark.parametrize("kind,n,k", CASES)
