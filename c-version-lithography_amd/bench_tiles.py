@@ -20,6 +20,16 @@
       p99 latency (arrival -> result) of both, and whether every request's mask
       is the same in both runs.
 
+  python c-version-lithography_amd/bench_tiles.py fullchip [--chip 8192] [--grid 16] [--updates 20]
+      Full-chip Micro optimisation of a seeded Manhattan chip: 16 x 16 Micros
+      of Core 512 + halo 64 (tile 640), guard 40, K = 24, PE loss, Average
+      Guard fusion 0.5 / 0.5, guard-only CEPT.  Full-chip CTM (5 iterations,
+      lr 0.9), the best mask thresholded at 0.5, then full-chip LSM from that
+      binary mask (WENO5 / Godunov, CFL 0.5, curvature 0.01, re-init every 20
+      updates, Binary render).  Prints one JSON line with LSM updates/s (every
+      update = one RK2 step of every Micro + the batched exchange) and CTM
+      iterations/s.
+
 Inputs are resident in HBM before the timed regions; the SOCS precompute and
 plan creation are set-up, outside the timers.
 """
@@ -199,6 +209,59 @@ def served(args):
     print(json.dumps(line), flush=True)
 
 
+def fullchip(args):
+    import torch
+    ctx = la.Context(0)
+    core = args.chip // args.grid
+    halo, guard = 64, 40
+    ts, tiles = la.partition_build(args.chip, args.chip, args.grid, args.grid, halo, guard)
+    t0 = time.perf_counter()
+    plan, d = make_plan(ctx, ts, len(tiles))
+    target = torch.from_numpy(tg.manhattan_target(args.chip, 8192)).cuda()
+    torch.cuda.synchronize()
+    print(f"set-up: {time.perf_counter() - t0:.1f} s; {len(tiles)} Micros of {ts} (core {core}), window {d['rows']}",
+          file=sys.stderr, flush=True)
+    opts = la.exchange_options(loss_type="PE", enable_cept=True, cept_guard_only=True, enable_grad_fusion=True,
+                               fusion_mode="Average", fusion_weight_owner=0.5, fusion_weight_neighbor=0.5,
+                               threshold=THR, alpha=ALPHA)
+    gmask = torch.zeros_like(target)
+    plan.fullchip_ctm(tiles, target, target, opts, 1, 0.9, gmask)          # warm: first launches
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r_ctm = plan.fullchip_ctm(tiles, target, target, opts, args.ctm_iterations, 0.9, gmask)
+    torch.cuda.synchronize()
+    t_ctm = time.perf_counter() - t0
+    binary = (gmask >= 0.5).float()
+    print(f"CTM: {args.ctm_iterations} iterations in {t_ctm:.3f} s, best loss {r_ctm['best_loss']:.1f} "
+          f"(iteration {r_ctm['best_iteration']})", file=sys.stderr, flush=True)
+    lmask = torch.zeros_like(target)
+    lphi = torch.zeros_like(target)
+    plan.fullchip_lsm(tiles, binary, target, opts, 1, lmask, lphi, cfl=0.5, curvature_weight=0.01,
+                      reinit_interval=20)                                   # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r_lsm = plan.fullchip_lsm(tiles, binary, target, opts, args.updates, lmask, lphi, cfl=0.5, curvature_weight=0.01,
+                              reinit_interval=20)
+    torch.cuda.synchronize()
+    t_lsm = time.perf_counter() - t0
+    h = r_lsm["history"]
+    print(f"LSM: {args.updates} updates in {t_lsm:.3f} s; loss {h[0][0]:.1f} -> {h[-1][0]:.1f}, "
+          f"best {r_lsm['best_loss']:.1f} (update {r_lsm['best_iteration']}), dt {h[1][3]:.3f}..{h[1][4]:.3f}",
+          file=sys.stderr, flush=True)
+    px = len(tiles) * ts * ts
+    line = {"metric": "full-chip LSM updates/s (RK2 WENO5/Godunov of every Micro + batched exchange)",
+            "value": round(args.updates / t_lsm, 3), "unit": "updates/s", "ms_per_update": round(1e3 * t_lsm / args.updates, 2),
+            "ctm_iterations_per_s": round(args.ctm_iterations / t_ctm, 3),
+            "chip": args.chip, "micros": len(tiles), "tile": ts, "core": core, "halo": halo, "guard": guard,
+            "k": K, "window": d["rows"], "micro_pixels": px,
+            "stencil_GBps_at_40B_per_px": round(40 * px / (t_lsm / args.updates) / 1e9, 1),
+            "lsm_loss_first_last": [round(h[0][0], 3), round(h[-1][0], 3)],
+            "ctm_best_loss": round(r_ctm["best_loss"], 3)}
+    plan.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -209,11 +272,18 @@ def main():
     v.add_argument("--requests", type=int, default=160)
     v.add_argument("--iterations", type=int, default=20)
     v.add_argument("--load", type=float, default=0.7)
+    f = sub.add_parser("fullchip")
+    f.add_argument("--chip", type=int, default=8192)
+    f.add_argument("--grid", type=int, default=16)
+    f.add_argument("--updates", type=int, default=20)
+    f.add_argument("--ctm-iterations", type=int, default=5)
     args = ap.parse_args()
     if args.cmd == "batched":
         batched(args)
-    else:
+    elif args.cmd == "served":
         served(args)
+    else:
+        fullchip(args)
 
 
 if __name__ == "__main__":
