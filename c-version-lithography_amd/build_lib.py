@@ -17,10 +17,11 @@ CSRC = PKG / "csrc"
 BUILD = PKG.parent / "_build" / "litho_amd"
 LIB = PKG / "liblitho_amd.so"
 BENCH = PKG / "litho_bench"
+CPP_TEST = PKG / "test_litho_cpp"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
@@ -72,6 +73,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     if p.returncode != 0:
         raise RuntimeError(f"litho_bench link failed:\n{p.stdout}\n{p.stderr}")
     os.replace(btmp, BENCH)
+    # C++ tests of the reference call surface (include/litho_cpp.hpp)
+    ttmp = CPP_TEST.with_suffix(".tmp")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-x", "hip", str(PKG.parent / "tests" / "cpp" / "test_litho_cpp.cpp"), "-o",
+           str(ttmp), f"--offload-arch={ARCH}", "-I", str(PKG.parent / "include"), "-L", str(PKG), "-llitho_amd",
+           f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"test_litho_cpp build failed:\n{p.stdout}\n{p.stderr}")
+    os.replace(ttmp, CPP_TEST)
     return LIB
 
 

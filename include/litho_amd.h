@@ -22,6 +22,12 @@
 
 #include <stddef.h>
 
+/* The library is built with hidden symbol visibility: exactly these entry
+ * points are exported. */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -360,6 +366,10 @@ int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
 
 #ifdef __cplusplus
 }
+#endif
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
 #endif
 
 #endif /* LITHO_AMD_H */
