@@ -16,16 +16,64 @@
 // q / LINES, so consecutive lanes touch consecutive columns (coalesced rows).
 // Precondition (checked on the host): every window frequency u satisfies
 // u < N/R_1 or u >= N - N/R_1.
+//
+// Twiddles: the first and last stages read the N-entry table at r*i (a few
+// distinct i per 32 lanes, so few bank conflicts).  The middle stages read
+// per-pass tables in LDS, laid out [r-1][k] so that lanes with consecutive k
+// read consecutive bank slots; indexing the N-entry table at r*k*STEP instead
+// puts up to 32 lanes on one bank when STEP is a multiple of 32.
 #pragma once
 #include "fft_engine.hpp"
 
 namespace litho {
 namespace fft {
 
-template <int N, int LINES, int LS, int NT>
+// Layout of the middle-stage tables of a column transform of length n: the
+// inverse stages 1..M-2 (radix r[j], span r[0]*...*r[j-1]) then the forward
+// stages 1..M-2 (radix r[M-1-j], span r[M-1]*...*r[M-j]).  Stage s of radix R
+// and span P holds w(r, k) = e^{-2 pi i r k n/(P R) / n} at off[s] + (r-1)*P + k.
+struct MidTables {
+  int count;
+  int R[24];
+  int P[24];
+  int off[25];
+};
+constexpr MidTables mid_tables(int n) {
+  const Radices f = factorize(n);
+  MidTables t{};
+  const int m = f.count;
+  int p = f.r[0];
+  for (int j = 1; j + 1 < m; ++j) {
+    t.R[t.count] = f.r[j];
+    t.P[t.count] = p;
+    t.off[t.count + 1] = t.off[t.count] + (f.r[j] - 1) * p;
+    ++t.count;
+    p *= f.r[j];
+  }
+  p = f.r[m - 1];
+  for (int j = 1; j + 1 < m; ++j) {
+    t.R[t.count] = f.r[m - 1 - j];
+    t.P[t.count] = p;
+    t.off[t.count + 1] = t.off[t.count] + (f.r[m - 1 - j] - 1) * p;
+    ++t.count;
+    p *= f.r[m - 1 - j];
+  }
+  return t;
+}
+
+// Twiddle pointers of one column transform: the N-entry table (LDS or global)
+// and the middle-stage tables (LDS).
+struct ColTw {
+  const float2* full;
+  const float2* mid;
+};
+
+template <int N, int LINES, int LS, int NT, int SH>
 struct ColumnFFT {
   static constexpr Radices F = factorize(N);
   static constexpr int M = F.count;
+  static constexpr MidTables MT = mid_tables(N);
+  static constexpr int MID_ENTRIES = MT.off[MT.count];   // float2 entries of the middle-stage tables
   static_assert(M >= 2, "pruned column transforms need at least two radix stages");
   static constexpr int R1 = F.r[0];
   static constexpr int RM = F.r[M - 1];
@@ -35,6 +83,26 @@ struct ColumnFFT {
   static constexpr int BM = LINES * TM;      // radix-RM butterflies per block
   static constexpr int PER1 = (B1 + NT - 1) / NT;
   static constexpr int PERM = (BM + NT - 1) / NT;
+
+  // Fill the middle-stage tables from the N-entry table tw (e^{-2 pi i j / N}
+  // at j).  Ends without a barrier: the caller barriers before the first use.
+  __device__ __forceinline__ static void fill_mid_tables(float2* mid, const float2* __restrict__ tw) {
+    if constexpr (MID_ENTRIES > 0)
+    for (int e = threadIdx.x; e < MID_ENTRIES; e += NT) {
+      int off = 0, p = MT.P[0], step = N / (MT.P[0] * MT.R[0]);
+#pragma unroll
+      for (int j = 1; j < MT.count; ++j) {
+        if (e >= MT.off[j]) {
+          off = MT.off[j];
+          p = MT.P[j];
+          step = N / (MT.P[j] * MT.R[j]);
+        }
+      }
+      const int loc = e - off;
+      const int r = 1 + loc / p, k = loc - (loc / p) * p;
+      mid[e] = tw[r * k * step];
+    }
+  }
 
   // ---- inverse stage 1 from HBM: src = the window rows of one spectrum
   // (row i at src + i*N), row_of_u[u] = window row of frequency u or -1.
@@ -50,7 +118,7 @@ struct ColumnFFT {
         const bool in = x < N;
         const float2 a = (in && ra >= 0) ? src[(size_t)ra * N + x] : make_float2(0.f, 0.f);
         const float2 c = (in && rb >= 0) ? src[(size_t)rb * N + x] : make_float2(0.f, 0.f);
-        store_two_tap<0>(buf + line * LS, i * R1, sw(i * R1), a, c);
+        store_two_tap<0>(buf + line * LS, i * R1, swz<SH>(i * R1), a, c);
       }
     }
   }
@@ -58,25 +126,25 @@ struct ColumnFFT {
   __device__ __forceinline__ static void store_two_tap(float2* line, int o, int so, float2 a, float2 c) {
     if constexpr (R < R1) {
       // out[r] = a + c * w^{(R1-1) r}, w = e^{+2 pi i / R1}
-      line[sw_at<sw_uniform_write<1, R1>(), 1>(o, so, R)] = cadd(a, twiddle_ct<(R1 - 1) * R, R1, +1>(c));
+      line[sw_at<SH, sw_uniform_write<SH, 1, R1>(), 1>(o, so, R)] = cadd(a, twiddle_ct<(R1 - 1) * R, R1, +1>(c));
       store_two_tap<R + 1>(line, o, so, a, c);
     }
   }
 
   // ---- middle stages (LDS in place)
   template <int IDX, int P>
-  __device__ __forceinline__ static void inv_middle(float2* buf, const float2* __restrict__ tw) {
+  __device__ __forceinline__ static void inv_middle(float2* buf, const float2* mid) {
     if constexpr (IDX < M - 1) {
-      stockham_pass<N, F.r[IDX], P, LINES, LS, NT, +1>(buf, tw);
-      inv_middle<IDX + 1, P * F.r[IDX]>(buf, tw);
+      stockham_pass<N, F.r[IDX], P, LINES, LS, NT, +1, SH>(buf, PassTwiddles<P>{mid + MT.off[IDX - 1]});
+      inv_middle<IDX + 1, P * F.r[IDX]>(buf, mid);
     }
   }
   // forward stages use the reversed radix list: stage j has radix F.r[M-1-j]
   template <int J, int P>
-  __device__ __forceinline__ static void fwd_middle(float2* buf, const float2* __restrict__ tw) {
+  __device__ __forceinline__ static void fwd_middle(float2* buf, const float2* mid) {
     if constexpr (J < M - 1) {
-      stockham_pass<N, F.r[M - 1 - J], P, LINES, LS, NT, -1>(buf, tw);
-      fwd_middle<J + 1, P * F.r[M - 1 - J]>(buf, tw);
+      stockham_pass<N, F.r[M - 1 - J], P, LINES, LS, NT, -1, SH>(buf, PassTwiddles<P>{mid + MT.off[M - 3 + J]});
+      fwd_middle<J + 1, P * F.r[M - 1 - J]>(buf, mid);
     }
   }
 
@@ -85,21 +153,21 @@ struct ColumnFFT {
   // Ends after the LDS reads; the caller must barrier before reusing buf.
   __device__ __forceinline__ static void inverse_to_regs(float2* buf, const float2* __restrict__ src,
                                                          const int* __restrict__ row_of_u, int x0,
-                                                         const float2* __restrict__ tw, float2 (&v)[PERM][RM]) {
+                                                         ColTw tw, float2 (&v)[PERM][RM]) {
     inv_first_from_global(buf, src, row_of_u, x0);
     __syncthreads();
-    inv_middle<1, R1>(buf, tw);
+    inv_middle<1, R1>(buf, tw.mid);
 #pragma unroll
     for (int s = 0; s < PERM; ++s) {
       const int q = threadIdx.x + s * NT;
       if (BM % NT == 0 || q < BM) {
         const int line = q % LINES, i = q / LINES;
         const float2* p = buf + line * LS;
-        const int si = sw(i);
+        const int si = swz<SH>(i);
 #pragma unroll
-        for (int r = 0; r < RM; ++r) v[s][r] = p[sw_at<sw_uniform_read<TM>(), TM>(i, si, r)];
+        for (int r = 0; r < RM; ++r) v[s][r] = p[sw_at<SH, sw_uniform_read<SH, TM>(), TM>(i, si, r)];
 #pragma unroll
-        for (int r = 1; r < RM; ++r) v[s][r] = cmulc(v[s][r], tw[r * i]);
+        for (int r = 1; r < RM; ++r) v[s][r] = cmulc(v[s][r], tw.full[r * i]);
         Dft<RM, +1>::run(v[s]);
       }
     }
@@ -107,8 +175,8 @@ struct ColumnFFT {
 
   // ---- forward transform of registers v (same ownership) with pruned output:
   // dst row ra / rb receives frequency u = i / (R1-1)*T1 + i of column x.
-  __device__ __forceinline__ static void forward_from_regs(float2* buf, float2 (&v)[PERM][RM],
-                                                           const float2* __restrict__ tw, float2* __restrict__ dst,
+  __device__ __forceinline__ static void forward_from_regs(float2* buf, float2 (&v)[PERM][RM], ColTw tw,
+                                                           float2* __restrict__ dst,
                                                            const int* __restrict__ row_of_u, int x0) {
 #pragma unroll
     for (int s = 0; s < PERM; ++s) {
@@ -117,13 +185,13 @@ struct ColumnFFT {
         const int line = q % LINES, i = q / LINES;
         Dft<RM, -1>::run(v[s]);
         float2* d = buf + line * LS;
-        const int so = sw(i * RM);
+        const int so = swz<SH>(i * RM);
 #pragma unroll
-        for (int r = 0; r < RM; ++r) d[sw_at<sw_uniform_write<1, RM>(), 1>(i * RM, so, r)] = v[s][r];
+        for (int r = 0; r < RM; ++r) d[sw_at<SH, sw_uniform_write<SH, 1, RM>(), 1>(i * RM, so, r)] = v[s][r];
       }
     }
     __syncthreads();
-    fwd_middle<1, RM>(buf, tw);
+    fwd_middle<1, RM>(buf, tw.mid);
 #pragma unroll
     for (int s = 0; s < PER1; ++s) {
       const int q = threadIdx.x + s * NT;
@@ -132,11 +200,11 @@ struct ColumnFFT {
         const int x = x0 + line;
         float2 w[R1];
         const float2* p = buf + line * LS;
-        const int si = sw(i);
+        const int si = swz<SH>(i);
 #pragma unroll
-        for (int r = 0; r < R1; ++r) w[r] = p[sw_at<sw_uniform_read<T1>(), T1>(i, si, r)];
+        for (int r = 0; r < R1; ++r) w[r] = p[sw_at<SH, sw_uniform_read<SH, T1>(), T1>(i, si, r)];
 #pragma unroll
-        for (int r = 1; r < R1; ++r) w[r] = cmul(w[r], tw[r * i]);
+        for (int r = 1; r < R1; ++r) w[r] = cmul(w[r], tw.full[r * i]);
         float2 o0 = w[0], o1 = w[0];
         sum_two_outputs<1>(w, o0, o1);
         const int ra = row_of_u[i], rb = row_of_u[(R1 - 1) * T1 + i];

@@ -241,49 +241,59 @@ struct Dft {
 };
 
 // ------------------------------------------------------------------ LDS layout
-// Element p of a line lives at sw(p) = p + p/16: one pad slot per 16 complex
-// values, so the strided Stockham writes (stride R = 2..16) and the contiguous
-// reads both hit 16 distinct 8-byte bank slots per 16 lanes.  Line strides are
-// padded_stride(N) = N + N/16 + 2, i.e. 2 (mod 16) slots, which also spreads
-// lanes that address the same element of neighbouring lines.
-#ifndef LITHO_LDS_SWIZZLE
-#define LITHO_LDS_SWIZZLE 1
-#endif
-#if LITHO_LDS_SWIZZLE
-__host__ __device__ constexpr int sw(int p) { return p + (p >> 4); }
-__host__ __device__ constexpr int padded_stride(int n) { return n + (n >> 4) + 2; }
-#else
-__host__ __device__ constexpr int sw(int p) { return p; }
-__host__ __device__ constexpr int padded_stride(int n) { return n + 1; }
-#endif
+// gfx950 serves a 64-bit LDS access 32 lanes at a time across 32 eight-byte
+// bank slots; lanes that hit the same slot modulo 32 at different addresses
+// serialise.  Element p of a line lives at swz<SH>(p) = p + (p >> SH): one pad
+// slot per 2^SH complex values, so strided Stockham writes land on distinct
+// slots.  Line strides are padded_stride<SH>(N) = N + N/2^SH + 4.  The shift is
+// chosen per kernel family from a replay of its accesses under this bank
+// model: 4 for the column kernels (lanes spread across 8 lines), 5 for the row
+// kernels (lanes along one line).
+template <int SH>
+__host__ __device__ constexpr int swz(int p) { return p + (p >> SH); }
+template <int SH>
+__host__ __device__ constexpr int padded_stride(int n) { return n + (n >> SH) + 4; }
 
-// Swizzled address of element base + r*S (r = 0..R-1) given sw(base).  When the
-// shifted elements never straddle a 16-slot boundary differently for different
-// lanes, the offset is the compile-time constant r*S + (r*S)/16 and folds into
-// the LDS instruction's immediate; otherwise it is computed per element.
-//   reads  base + r*T           : uniform when T % 16 == 0
-//   writes (i-k)*R + k + r*P    : uniform when P % 16 == 0, or 16 % P == 0 and
-//                                 (R*P % 16 == 0 or 16 % (R*P) == 0)
-// ((i-k)*R is a multiple of R*P; with P | 16 the 16-slot blocks are unions of
+// Swizzled address of element base + r*S (r = 0..R-1) given swz(base).  When the
+// shifted elements never straddle a 2^SH-slot boundary differently for
+// different lanes, the offset is the compile-time constant r*S + (r*S >> SH)
+// and folds into the LDS instruction's immediate; otherwise it is computed per
+// element.  With B = 2^SH:
+//   reads  base + r*T           : uniform when T % B == 0
+//   writes (i-k)*R + k + r*P    : uniform when P % B == 0, or B % P == 0 and
+//                                 (R*P % B == 0 or B % (R*P) == 0)
+// ((i-k)*R is a multiple of R*P; with P | B the B-slot blocks are unions of
 // P-blocks, so k never moves an element across a block boundary.)
-template <int S>
-__host__ __device__ constexpr bool sw_uniform_read() { return S % 16 == 0; }
-template <int P, int R>
+template <int SH, int S>
+__host__ __device__ constexpr bool sw_uniform_read() { return S % (1 << SH) == 0; }
+template <int SH, int P, int R>
 __host__ __device__ constexpr bool sw_uniform_write() {
-  return P % 16 == 0 || (16 % P == 0 && ((R * P) % 16 == 0 || 16 % (R * P) == 0));
+  constexpr int B = 1 << SH;
+  return P % B == 0 || (B % P == 0 && ((R * P) % B == 0 || B % (R * P) == 0));
 }
-template <bool UNIFORM, int S>
+template <int SH, bool UNIFORM, int S>
 __device__ __forceinline__ int sw_at(int base, int sw_base, int r) {
-  if constexpr (UNIFORM && LITHO_LDS_SWIZZLE) return sw_base + r * S + ((r * S) >> 4);
-  else return sw(base + r * S);
+  if constexpr (UNIFORM) return sw_base + r * S + ((r * S) >> SH);
+  else return swz<SH>(base + r * S);
 }
+
+// Twiddle sources of a Stockham pass: w(r, k, step) = e^{-2 pi i r k step / N}.
+struct TableTwiddles {          // the N-entry table (HBM / L2 or an LDS copy)
+  const float2* t;
+  __device__ __forceinline__ float2 operator()(int r, int k, int step) const { return t[r * k * step]; }
+};
+template <int P>
+struct PassTwiddles {           // one pass's (R-1) x P table: lanes with consecutive k read consecutive slots
+  const float2* t;
+  __device__ __forceinline__ float2 operator()(int r, int k, int) const { return t[(r - 1) * P + k]; }
+};
 
 // ------------------------------------------------------------------ passes
 // One Stockham pass of radix R and span P over LINES lines of length N with
-// line stride LS (in float2) and NT threads.  Input position i + r*(N/R);
-// output position (i - k)*R + k + r*P with k = i mod P.
-template <int N, int R, int P, int LINES, int LS, int NT, int DIR>
-__device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restrict__ tw) {
+// line stride LS (in float2), NT threads and swizzle shift SH.  Input position
+// i + r*(N/R); output position (i - k)*R + k + r*P with k = i mod P.
+template <int N, int R, int P, int LINES, int LS, int NT, int DIR, int SH, class TW>
+__device__ __forceinline__ void stockham_pass(float2* buf, const TW& tw) {
   constexpr int T = N / R;
   constexpr int TOTAL = T * LINES;
   constexpr int PER = (TOTAL + NT - 1) / NT;
@@ -296,9 +306,9 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
       const int line = q / T;
       const int i = q - line * T;
       const float2* src = buf + line * LS;
-      const int si = sw(i);
+      const int si = swz<SH>(i);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[s][r] = src[sw_at<sw_uniform_read<T>(), T>(i, si, r)];
+      for (int r = 0; r < R; ++r) v[s][r] = src[sw_at<SH, sw_uniform_read<SH, T>(), T>(i, si, r)];
     }
   }
   __syncthreads();
@@ -313,37 +323,37 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restr
         constexpr int STEP = N / (P * R);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const float2 w = tw[r * k * STEP];
+          const float2 w = tw(r, k, STEP);
           v[s][r] = DIR < 0 ? cmul(v[s][r], w) : cmulc(v[s][r], w);
         }
       }
       Dft<R, DIR>::run(v[s]);
       float2* dst = buf + line * LS;
       const int o = (i - k) * R + k;
-      const int so = sw(o);
+      const int so = swz<SH>(o);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[sw_at<sw_uniform_write<P, R>(), P>(o, so, r)] = v[s][r];
+      for (int r = 0; r < R; ++r) dst[sw_at<SH, sw_uniform_write<SH, P, R>(), P>(o, so, r)] = v[s][r];
     }
   }
   __syncthreads();
 }
 
-template <int N, int IDX, int P, int LINES, int LS, int NT, int DIR>
+template <int N, int IDX, int P, int LINES, int LS, int NT, int DIR, int SH>
 __device__ __forceinline__ void run_passes(float2* buf, const float2* __restrict__ tw) {
   constexpr Radices f = factorize(N);
   if constexpr (IDX < f.count) {
     constexpr int R = f.r[IDX];
-    stockham_pass<N, R, P, LINES, LS, NT, DIR>(buf, tw);
-    run_passes<N, IDX + 1, P * R, LINES, LS, NT, DIR>(buf, tw);
+    stockham_pass<N, R, P, LINES, LS, NT, DIR, SH>(buf, TableTwiddles{tw});
+    run_passes<N, IDX + 1, P * R, LINES, LS, NT, DIR, SH>(buf, tw);
   }
 }
 
 // Transform LINES lines in LDS in place.  The caller must __syncthreads()
 // after writing the input; the routine ends with a barrier.
-template <int N, int LINES, int LS, int NT, int DIR>
+template <int N, int LINES, int LS, int NT, int DIR, int SH>
 __device__ __forceinline__ void fft_lines(float2* buf, const float2* __restrict__ tw) {
   static_assert(factorize(N).count > 0, "unsupported FFT length");
-  run_passes<N, 0, 1, LINES, LS, NT, DIR>(buf, tw);
+  run_passes<N, 0, 1, LINES, LS, NT, DIR, SH>(buf, tw);
 }
 
 }  // namespace fft

@@ -45,7 +45,8 @@ template <int N>
 struct RowCfg {
   static constexpr int LINES = N >= 1024 ? 4 : (N >= 256 ? 8 : 16);
   static constexpr int NT = N >= 512 ? 256 : (N >= 128 ? 128 : 64);
-  static constexpr int LS = fft::padded_stride(N);  // line stride (float2), swizzled lines
+  static constexpr int SH = 5;                          // swizzle shift: lanes run along one line
+  static constexpr int LS = fft::padded_stride<SH>(N);  // line stride (float2), swizzled lines
 };
 
 #ifndef LITHO_COL_NT_LARGE
@@ -90,13 +91,18 @@ struct ColCfg {
   static constexpr bool BIG_ODD = N >= 1024 && max_radix(N) > 16;
   static constexpr int LINES = BIG_ODD ? 4 : (N >= 1024 ? LITHO_COL_LINES_LARGE : (N >= 256 ? 8 : 16));
   static constexpr int NT = BIG_ODD ? 512 : (N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? col_nt_mid(LINES, N) : 64));
-  static constexpr int LS = fft::padded_stride(N);    // line stride (float2), swizzled lines
+  // swizzle shift: lanes spread over LINES lines; the four-column blocks of the
+  // large odd sizes behave like the row kernels
+  static constexpr int SH = BIG_ODD ? 5 : 4;
+  static constexpr int LS = fft::padded_stride<SH>(N);  // line stride (float2), swizzled lines
   static constexpr int ELEMS = LINES * N;
   static constexpr int PER = (ELEMS + NT - 1) / NT;     // owned elements per thread
-  // twiddle table copied to LDS when it fits beside the lines (160 KB per CU)
-  static constexpr bool TW_LDS =
-      LITHO_TW_LDS && (size_t)LINES * LS * 8 + (size_t)N * 8 + (size_t)NT / 64 * 16 <= 160 * 1024;
-  static constexpr size_t LDS_BYTES = (size_t)LINES * LS * 8 + (TW_LDS ? (size_t)N * 8 : 0) + (size_t)NT / 64 * 16;
+  // middle-stage twiddle tables of the pruned column transforms (always in LDS)
+  static constexpr int MID = fft::mid_tables(N).off[fft::mid_tables(N).count];
+  static constexpr size_t FIXED_BYTES = (size_t)LINES * LS * 8 + (size_t)(MID > 0 ? MID : 1) * 8 + (size_t)NT / 64 * 16;
+  // N-entry twiddle table copied to LDS when it fits beside the lines (160 KB per CU)
+  static constexpr bool TW_LDS = LITHO_TW_LDS && FIXED_BYTES + (size_t)N * 8 <= 160 * 1024;
+  static constexpr size_t LDS_BYTES = FIXED_BYTES + (TW_LDS ? (size_t)N * 8 : 0);
   // register budget: as many waves per SIMD as the blocks that fit in one CU's LDS bring
   static constexpr int BLOCKS_PER_CU = 160 * 1024 / LDS_BYTES > 0 ? (int)(160 * 1024 / LDS_BYTES) : 1;
   static constexpr int LDS_WAVES = BLOCKS_PER_CU * NT / 256 > 8 ? 8 : (BLOCKS_PER_CU * NT / 256 > 0 ? BLOCKS_PER_CU * NT / 256 : 1);
@@ -104,19 +110,21 @@ struct ColCfg {
   static constexpr int MAX_WAVES_PER_EU = LITHO_COL_OCC_RULE == 1 ? 8 : (NT >= 512 ? NT / 256 : 1);
 };
 
-// Twiddle source of the pruned column kernels: the LDS copy or the global
-// table.  The opaque zero offset taken per call stops the compiler from
-// hoisting the (loop-invariant) twiddle loads out of the K loops, which would
-// keep ~60 twiddle registers live across the loop and spill.
+// Twiddle sources of the pruned column kernels: the N-entry table (LDS copy or
+// global) and the middle-stage tables in LDS.  The opaque zero offset taken
+// per call stops the compiler from hoisting the (loop-invariant) twiddle loads
+// out of the K loops, which would keep ~60 twiddle registers live across the
+// loop and spill.
 template <int N>
 struct ColTwiddles {
   float2* lds;
   const float2* glob;
-  __device__ __forceinline__ const float2* get() const {
+  float2* mid;
+  __device__ __forceinline__ fft::ColTw get() const {
     int z = 0;
     asm volatile("" : "+s"(z));
-    if constexpr (ColCfg<N>::TW_LDS) return lds + z;
-    return glob + z;
+    if constexpr (ColCfg<N>::TW_LDS) return {lds + z, mid + z};
+    return {glob + z, mid + z};
   }
 };
 
@@ -157,17 +165,17 @@ k_row_fwd(SocsArgs a, int mode, const float* base, const float* __restrict__ dir
       }
       m2[h] = m;
     }
-    buf[l * C::LS + fft::sw(x)] = make_float2(m2[0], m2[1]);
+    buf[l * C::LS + fft::swz<C::SH>(x)] = make_float2(m2[0], m2[1]);
   }
   __syncthreads();
-  fft::fft_lines<N, PAIRS, C::LS, C::NT, -1>(buf, a.tw);
+  fft::fft_lines<N, PAIRS, C::LS, C::NT, -1, C::SH>(buf, a.tw);
   // unpack X_a[v] = (Z[v] + conj Z[-v]) / 2, X_b[v] = (Z[v] - conj Z[-v]) / (2i)
   float2* rt = a.rt + (size_t)b * a.wc * N;
   for (int e = threadIdx.x; e < PAIRS * a.wc; e += C::NT) {
     const int l = e / a.wc, j = e - l * a.wc;
     const int v = a.col_freq[j];
     const int vn = v == 0 ? 0 : N - v;
-    const float2 z = buf[l * C::LS + fft::sw(v)], zc = cconj(buf[l * C::LS + fft::sw(vn)]);
+    const float2 z = buf[l * C::LS + fft::swz<C::SH>(v)], zc = cconj(buf[l * C::LS + fft::swz<C::SH>(vn)]);
     const float2 xa = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
     const float2 xb = make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
     const int y = row0 + 2 * l;
@@ -189,15 +197,15 @@ k_col_fwd(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, y = e - l * N;
     const int j = j0 + l;
-    buf[l * C::LS + fft::sw(y)] = j < a.wc ? rt[(size_t)j * N + y] : make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::swz<C::SH>(y)] = j < a.wc ? rt[(size_t)j * N + y] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, -1, C::SH>(buf, a.tw);
   float2* mh = a.mhat + (size_t)b * a.wr * a.wc;
   for (int e = threadIdx.x; e < C::LINES * a.wr; e += C::NT) {
     const int l = e / a.wr, i = e - l * a.wr;
     const int j = j0 + l;
-    if (j < a.wc) mh[(size_t)i * a.wc + j] = buf[l * C::LS + fft::sw(a.row_freq[i])];
+    if (j < a.wc) mh[(size_t)i * a.wc + j] = buf[l * C::LS + fft::swz<C::SH>(a.row_freq[i])];
   }
 }
 
@@ -213,7 +221,7 @@ k_row_inv(SocsArgs a) {
   const int q0 = blockIdx.x * C::LINES;
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
-    buf[l * C::LS + fft::sw(x)] = make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::swz<C::SH>(x)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   const float2* mh = a.mhat + (size_t)b * a.wr * a.wc;
@@ -222,16 +230,16 @@ k_row_inv(SocsArgs a) {
     const int q = q0 + l;
     if (q < rows) {
       const int i = q % a.wr;
-      buf[l * C::LS + fft::sw(a.col_freq[j])] = cmul(a.h[(size_t)q * a.wc + j], mh[(size_t)i * a.wc + j]);
+      buf[l * C::LS + fft::swz<C::SH>(a.col_freq[j])] = cmul(a.h[(size_t)q * a.wc + j], mh[(size_t)i * a.wc + j]);
     }
   }
   __syncthreads();
-  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1, C::SH>(buf, a.tw);
   float2* t = a.t + (size_t)b * rows * N;
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
     const int q = q0 + l;
-    if (q < rows) t[(size_t)q * N + x] = buf[l * C::LS + fft::sw(x)];
+    if (q < rows) t[(size_t)q * N + x] = buf[l * C::LS + fft::swz<C::SH>(x)];
   }
 }
 
@@ -245,7 +253,7 @@ __device__ __forceinline__ void load_window_columns(float2* buf, const float2* _
   for (int e = threadIdx.x; e < wr * C::LINES; e += C::NT) {
     const int i = e / C::LINES, c = e - i * C::LINES;
     const int x = x0 + c;
-    if (x < N) buf[c * C::LS + fft::sw(row_freq[i])] = src[(size_t)i * N + x];
+    if (x < N) buf[c * C::LS + fft::swz<C::SH>(row_freq[i])] = src[(size_t)i * N + x];
   }
 }
 
@@ -267,19 +275,19 @@ k_col_image_adj(SocsArgs a, ImageArgs im) {
   for (int s = 0; s < C::PER; ++s) {
     acc[s] = 0.f;
     const int e = threadIdx.x + s * C::NT;
-    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::sw(e / C::LINES)] = make_float2(0.f, 0.f);
+    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   // ---- pass 1: aerial image
   for (int k = 0; k < a.k; ++k) {
     load_window_columns<N>(buf, t + (size_t)k * a.wr * N, a.wr, a.row_freq, x0);
     __syncthreads();
-    fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+    fft::fft_lines<N, C::LINES, C::LS, C::NT, +1, C::SH>(buf, a.tw);
 #pragma unroll
     for (int s = 0; s < C::PER; ++s) {
       const int e = threadIdx.x + s * C::NT;
       if (e < C::ELEMS) {
-        float2* p = buf + (e % C::LINES) * C::LS + fft::sw(e / C::LINES);
+        float2* p = buf + (e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES);
         const float2 z = *p;
         acc[s] = fmaf(z.x, z.x, fmaf(z.y, z.y, acc[s]));
         *p = make_float2(0.f, 0.f);
@@ -349,29 +357,29 @@ k_col_image_adj(SocsArgs a, ImageArgs im) {
   for (int k = 0; k < a.k; ++k) {
     load_window_columns<N>(buf, t + (size_t)k * a.wr * N, a.wr, a.row_freq, x0);
     __syncthreads();
-    fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+    fft::fft_lines<N, C::LINES, C::LS, C::NT, +1, C::SH>(buf, a.tw);
 #pragma unroll
     for (int s = 0; s < C::PER; ++s) {
       const int e = threadIdx.x + s * C::NT;
       if (e < C::ELEMS) {
-        float2* p = buf + (e % C::LINES) * C::LS + fft::sw(e / C::LINES);
+        float2* p = buf + (e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES);
         const float2 z = *p;
         *p = make_float2(z.x * g[s], z.y * g[s]);
       }
     }
     __syncthreads();
-    fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
+    fft::fft_lines<N, C::LINES, C::LS, C::NT, -1, C::SH>(buf, a.tw);
     float2* vk = v + (size_t)k * a.wr * N;
     for (int e = threadIdx.x; e < a.wr * C::LINES; e += C::NT) {
       const int i = e / C::LINES, c = e - i * C::LINES;
       const int x = x0 + c;
-      if (x < N) vk[(size_t)i * N + x] = buf[c * C::LS + fft::sw(a.row_freq[i])];
+      if (x < N) vk[(size_t)i * N + x] = buf[c * C::LS + fft::swz<C::SH>(a.row_freq[i])];
     }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < C::PER; ++s) {
       const int e = threadIdx.x + s * C::NT;
-      if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::sw(e / C::LINES)] = make_float2(0.f, 0.f);
+      if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
   }
@@ -391,15 +399,15 @@ k_row_adj(SocsArgs a) {
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
     const int q = q0 + l;
-    buf[l * C::LS + fft::sw(x)] = q < rows ? v[(size_t)q * N + x] : make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::swz<C::SH>(x)] = q < rows ? v[(size_t)q * N + x] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft::fft_lines<N, C::LINES, C::LS, C::NT, -1>(buf, a.tw);
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, -1, C::SH>(buf, a.tw);
   float2* yk = a.y + (size_t)b * rows * a.wc;
   for (int e = threadIdx.x; e < C::LINES * a.wc; e += C::NT) {
     const int l = e / a.wc, j = e - l * a.wc;
     const int q = q0 + l;
-    if (q < rows) yk[(size_t)q * a.wc + j] = cmulc(buf[l * C::LS + fft::sw(a.col_freq[j])], a.h[(size_t)q * a.wc + j]);
+    if (q < rows) yk[(size_t)q * a.wc + j] = cmulc(buf[l * C::LS + fft::swz<C::SH>(a.col_freq[j])], a.h[(size_t)q * a.wc + j]);
   }
 }
 
@@ -415,7 +423,7 @@ k_final_row_inv(SocsArgs a) {
   const int i0 = blockIdx.x * C::LINES;
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
-    buf[l * C::LS + fft::sw(x)] = make_float2(0.f, 0.f);
+    buf[l * C::LS + fft::swz<C::SH>(x)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   const float2* yk = a.y + (size_t)b * rows * a.wc;
@@ -429,16 +437,16 @@ k_final_row_inv(SocsArgs a) {
         s.x += u.x;
         s.y += u.y;
       }
-      buf[l * C::LS + fft::sw(a.col_freq[j])] = s;
+      buf[l * C::LS + fft::swz<C::SH>(a.col_freq[j])] = s;
     }
   }
   __syncthreads();
-  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1, C::SH>(buf, a.tw);
   float2* p = a.p + (size_t)b * a.wr * N;
   for (int e = threadIdx.x; e < C::LINES * N; e += C::NT) {
     const int l = e / N, x = e - l * N;
     const int i = i0 + l;
-    if (i < a.wr) p[(size_t)i * N + x] = buf[l * C::LS + fft::sw(x)];
+    if (i < a.wr) p[(size_t)i * N + x] = buf[l * C::LS + fft::swz<C::SH>(x)];
   }
 }
 
@@ -456,12 +464,12 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
 #pragma unroll
   for (int s = 0; s < C::PER; ++s) {
     const int e = threadIdx.x + s * C::NT;
-    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::sw(e / C::LINES)] = make_float2(0.f, 0.f);
+    if (e < C::ELEMS) buf[(e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   load_window_columns<N>(buf, a.p + (size_t)b * a.wr * N, a.wr, a.row_freq, x0);
   __syncthreads();
-  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1>(buf, a.tw);
+  fft::fft_lines<N, C::LINES, C::LS, C::NT, +1, C::SH>(buf, a.tw);
   double nrm = 0.0, dot = 0.0;
   const int* rect = ga.rect != nullptr ? ga.rect + 4 * b : nullptr;
 #pragma unroll
@@ -471,7 +479,7 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
     const int x = x0 + c;
     if (e < C::ELEMS && x < N) {
       const size_t gi = tile + (size_t)y * N + x;
-      const float gm = ga.scale * buf[c * C::LS + fft::sw(y)].x;
+      const float gm = ga.scale * buf[c * C::LS + fft::swz<C::SH>(y)].x;
       if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gm;
       if (ga.theta != nullptr) {
         float gt = -0.5f * gm * sinf(ga.theta[gi]);
@@ -512,10 +520,11 @@ template <int N>
 __global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
   using C = ColCfg<N>;
-  using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;
+  using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT, C::SH>;
   __shared__ float2 buf[C::LINES * C::LS];
   __shared__ double red[2 * C::NT / 64];
   __shared__ float2 tws[C::TW_LDS ? N : 1];
+  __shared__ float2 mid[CF::MID_ENTRIES > 0 ? CF::MID_ENTRIES : 1];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
   const int x0 = blockIdx.x * C::LINES;
@@ -524,9 +533,10 @@ k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
   const float2* t = a.t + (size_t)b * rows * N;
   if constexpr (C::TW_LDS) {
     for (int e = threadIdx.x; e < N; e += C::NT) tws[e] = a.tw[e];
-    __syncthreads();
   }
-  const ColTwiddles<N> twd{tws, a.tw};
+  CF::fill_mid_tables(mid, a.tw);
+  __syncthreads();
+  const ColTwiddles<N> twd{tws, a.tw, mid};
   float acc[CF::PERM][CF::RM];
   float2 v[CF::PERM][CF::RM];
 #pragma unroll
@@ -623,15 +633,18 @@ template <int N>
 __global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
 k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
   using C = ColCfg<N>;
-  using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT>;
+  using CF = fft::ColumnFFT<N, C::LINES, C::LS, C::NT, C::SH>;
   __shared__ float2 buf[C::LINES * C::LS];
   __shared__ double red[2 * C::NT / 64];
+  __shared__ float2 mid[CF::MID_ENTRIES > 0 ? CF::MID_ENTRIES : 1];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
   const int x0 = blockIdx.x * C::LINES;
   const size_t tile = (size_t)b * N * N;
+  CF::fill_mid_tables(mid, a.tw);
+  __syncthreads();
   float2 v[CF::PERM][CF::RM];
-  CF::inverse_to_regs(buf, a.p + (size_t)b * a.wr * N, a.row_of_u, x0, a.tw, v);
+  CF::inverse_to_regs(buf, a.p + (size_t)b * a.wr * N, a.row_of_u, x0, fft::ColTw{a.tw, mid}, v);
   double nrm = 0.0, dot = 0.0;
   const int* rect = ga.rect != nullptr ? ga.rect + 4 * b : nullptr;
 #pragma unroll
