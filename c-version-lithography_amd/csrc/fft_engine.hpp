@@ -42,6 +42,41 @@ constexpr Radices factorize(int n) {
 
 constexpr bool supported(int n) { return n >= 2 && factorize(n).count > 0; }
 
+// Per-stage twiddle tables of the line transforms (rows and the pruned
+// columns): every stage s >= 1 of the inverse order (radix r[s], span
+// P = r[0]*...*r[s-1]) at index s-1, then every stage j >= 1 of the forward
+// order (radix r[M-1-j], span r[M-1]*...*r[M-j]) at index M-2+j.  Stage
+// table t of radix R and span P holds w(r, k) = e^{-2 pi i r k / (P R)} at
+// off[t] + (r-1)*P + k, so lanes with consecutive k read consecutive entries.
+struct StageTables {
+  int count;
+  int R[24];
+  int P[24];
+  int off[25];
+};
+constexpr StageTables stage_tables(int n) {
+  const Radices f = factorize(n);
+  StageTables t{};
+  const int m = f.count;
+  int p = f.r[0];
+  for (int j = 1; j < m; ++j) {
+    t.R[t.count] = f.r[j];
+    t.P[t.count] = p;
+    t.off[t.count + 1] = t.off[t.count] + (f.r[j] - 1) * p;
+    ++t.count;
+    p *= f.r[j];
+  }
+  p = f.r[m - 1];
+  for (int j = 1; j < m; ++j) {
+    t.R[t.count] = f.r[m - 1 - j];
+    t.P[t.count] = p;
+    t.off[t.count + 1] = t.off[t.count] + (f.r[m - 1 - j] - 1) * p;
+    ++t.count;
+    p *= f.r[m - 1 - j];
+  }
+  return t;
+}
+
 // ------------------------------------------------------- compile-time constants
 constexpr double kPi = 3.14159265358979323846264338327950288;
 

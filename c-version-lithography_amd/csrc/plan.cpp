@@ -70,6 +70,25 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     p->row_of_u.alloc(rou.size() * sizeof(int));
     LITHO_HIP(hipMemcpy(p->row_of_u.ptr, rou.data(), p->row_of_u.bytes, hipMemcpyHostToDevice));
 
+    // the same for the window columns and the row transforms; the real-row
+    // forward (two rows per complex line) also reads frequency N - v, so the
+    // upper bound is strict
+    std::vector<int> cou(n, -1);
+    for (int j = 0; j < wc; ++j) cou[cf[j]] = j;
+    bool row_prunable = r1 > 0;
+    if (row_prunable) {
+      const int t1 = n / r1;
+      for (int v : cf) row_prunable = row_prunable && (v < t1 || v > n - t1);
+    }
+    const char* rpath = std::getenv("LITHO_ROW_PATH");
+    p->row_pruned = row_prunable && !(rpath != nullptr && std::string(rpath) == "generic") ? 1 : 0;
+    p->col_of_u.alloc(cou.size() * sizeof(int));
+    LITHO_HIP(hipMemcpy(p->col_of_u.ptr, cou.data(), p->col_of_u.bytes, hipMemcpyHostToDevice));
+    const std::vector<float2> rtw = row_stage_twiddles(n);
+    p->rtw.alloc((rtw.empty() ? 1 : rtw.size()) * sizeof(float2));
+    if (!rtw.empty())
+      LITHO_HIP(hipMemcpy(p->rtw.ptr, rtw.data(), rtw.size() * sizeof(float2), hipMemcpyHostToDevice));
+
     p->h.alloc(hk.size() * sizeof(float2));
     p->tw.alloc(tw.size() * sizeof(float2));
     p->row_freq.alloc(rf.size() * sizeof(int));
@@ -92,7 +111,8 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     p->coef.alloc(c * sizeof(float));
     p->host_red = new HostPinned(c * 4 * sizeof(double));
     p->host_coef = new HostPinned(c * sizeof(float));
-    p->device_bytes = p->h.bytes + p->tw.bytes + p->row_freq.bytes + p->col_freq.bytes + p->rt.bytes +
+    p->device_bytes = p->h.bytes + p->tw.bytes + p->rtw.bytes + p->row_of_u.bytes + p->col_of_u.bytes +
+                      p->row_freq.bytes + p->col_freq.bytes + p->rt.bytes +
                       p->mhat.bytes + p->t.bytes + p->v.bytes + p->y.bytes + p->p.bytes + p->img_partials.bytes +
                       p->grad_partials.bytes + p->red.bytes + p->coef.bytes;
   } catch (...) {

@@ -96,8 +96,9 @@ struct Plan {
   double weight_sum = 0.0;
   const SocsOps* ops = nullptr;
   int col_blocks = 0;
-  DevMem h, tw, row_freq, col_freq, row_of_u;
+  DevMem h, tw, rtw, row_freq, col_freq, row_of_u, col_of_u;
   int pruned = 0;                            // window-pruned column kernels in use
+  int row_pruned = 0;                        // window-pruned row kernels in use
   int tuned_chunk = 0;                       // batch size chosen by autotune_chunk (0 = not tuned)
   int tuned_limit = 0;                       // the limit it was tuned for
   DevMem rt, mhat, t, v, y, p;
@@ -122,6 +123,9 @@ struct Plan {
     a.active = nullptr;
     a.row_of_u = row_of_u.as<int>();
     a.pruned = pruned;
+    a.col_of_u = col_of_u.as<int>();
+    a.rtw = rtw.as<float2>();
+    a.row_pruned = row_pruned;
     return a;
   }
 };

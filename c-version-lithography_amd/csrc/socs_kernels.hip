@@ -29,8 +29,11 @@
 //   (per-block FP64 partials are summed by launch_reduce_fixed, opt_kernels.hip)
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "fft_columns.hpp"
 #include "fft_engine.hpp"
+#include "fft_rows.hpp"
 #include "opt_kernels.hpp"
 #include "socs_kernels.hpp"
 
@@ -47,6 +50,29 @@ struct RowCfg {
   static constexpr int NT = N >= 512 ? 256 : (N >= 128 ? 128 : 64);
   static constexpr int SH = 5;                          // swizzle shift: lanes run along one line
   static constexpr int LS = fft::padded_stride<SH>(N);  // line stride (float2), swizzled lines
+};
+
+// Pruned row kernels (fft_rows.hpp): LINES lines per block for the kernels
+// over all K * W_r spectrum rows, WLINES for the kernels over the W_r or W_c
+// window lines, which are too few to fill the chip with wide blocks.
+#ifndef LITHO_ROWP_LINES_LARGE
+#define LITHO_ROWP_LINES_LARGE 2
+#endif
+#ifndef LITHO_ROWP_WLINES_LARGE
+#define LITHO_ROWP_WLINES_LARGE 1
+#endif
+// about one radix-16 butterfly per thread, in whole waves, 64..256 threads
+constexpr int row_threads(int lines, int n) {
+  return (lines * n / 16 + 63) / 64 * 64 < 64 ? 64 : ((lines * n / 16 + 63) / 64 * 64 > 256 ? 256 : (lines * n / 16 + 63) / 64 * 64);
+}
+template <int N>
+struct RowPCfg {
+  static constexpr int SH = 5;
+  static constexpr int LS = fft::padded_stride<SH>(N);
+  static constexpr int LINES = N >= 1024 ? LITHO_ROWP_LINES_LARGE : (N >= 256 ? 4 : 8);
+  static constexpr int NT = row_threads(LINES, N);
+  static constexpr int WLINES = N >= 1024 ? LITHO_ROWP_WLINES_LARGE : (N >= 256 ? 2 : 4);
+  static constexpr int WNT = row_threads(WLINES, N);
 };
 
 #ifndef LITHO_COL_NT_LARGE
@@ -450,6 +476,169 @@ k_final_row_inv(SocsArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- pruned rows
+// K1, K2, K3, K6 and K7 over the pruned line transforms of fft_rows.hpp: the
+// spectra are nonzero (or needed) only at the window frequencies, so the
+// inverse transforms start from two taps per butterfly and the forward
+// transforms finish with two outputs per butterfly; first and last stages
+// read and write HBM directly.  K1 additionally needs N - v in the computed
+// set for every window column v (strict upper bound, checked on the host).
+template <int N>
+__global__ void __launch_bounds__(RowPCfg<N>::NT)
+k_row_fwd_pruned(SocsArgs a, int mode, const float* base, const float* __restrict__ dir,
+                 const float* __restrict__ coef, float* theta_out, float* __restrict__ mask_out) {
+  using C = RowPCfg<N>;
+  using RF = fft::RowFFT<N, C::LINES, C::LS, C::NT, C::SH>;
+  constexpr int T1 = RF::T1;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ float2 zc[C::LINES * 2 * T1];   // computed frequencies: u < T1 at u, u >= N - T1 at u - (N - 2 T1)
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const size_t tile = (size_t)b * N * N;
+  const int row0 = blockIdx.x * 2 * C::LINES;
+  const float cf = (dir != nullptr) ? coef[b] : 0.f;
+  RF::forward(
+      buf, a.rtw,
+      [&](int l, int x) {
+        float m2[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int y = row0 + 2 * l + h;
+          float m = 0.f;
+          if (y < N) {
+            const size_t g = tile + (size_t)y * N + x;
+            float v = base[g];
+            if (mode == 1) {
+              if (dir != nullptr) v = v + cf * dir[g];
+              if (theta_out != nullptr) theta_out[g] = v;
+              m = 0.5f * (1.0f + cosf(v));
+            } else {
+              m = v;
+            }
+            if (mask_out != nullptr) mask_out[g] = m;
+          }
+          m2[h] = m;
+        }
+        return make_float2(m2[0], m2[1]);
+      },
+      [&](int l, int u, float2 z) { zc[l * 2 * T1 + (u < T1 ? u : u - (N - 2 * T1))] = z; });
+  __syncthreads();
+  // unpack X_a[v] = (Z[v] + conj Z[-v]) / 2, X_b[v] = (Z[v] - conj Z[-v]) / (2i)
+  float2* rt = a.rt + (size_t)b * a.wc * N;
+  for (int e = threadIdx.x; e < C::LINES * a.wc; e += C::NT) {
+    const int l = e / a.wc, j = e - l * a.wc;
+    const int v = a.col_freq[j];
+    const int vn = v == 0 ? 0 : N - v;
+    const float2 z = zc[l * 2 * T1 + (v < T1 ? v : v - (N - 2 * T1))];
+    const float2 zn = cconj(zc[l * 2 * T1 + (vn < T1 ? vn : vn - (N - 2 * T1))]);
+    const float2 xa = make_float2(0.5f * (z.x + zn.x), 0.5f * (z.y + zn.y));
+    const float2 xb = make_float2(0.5f * (z.y - zn.y), -0.5f * (z.x - zn.x));
+    const int y = row0 + 2 * l;
+    if (y < N) rt[(size_t)j * N + y] = xa;
+    if (y + 1 < N) rt[(size_t)j * N + y + 1] = xb;
+  }
+}
+
+// K2: the W_c row-spectrum columns (stored as lines along y) -> mask spectrum
+// window rows.  Needs the window rows to fit the pruning (the column flag).
+template <int N>
+__global__ void __launch_bounds__(RowPCfg<N>::WNT)
+k_col_fwd_pruned(SocsArgs a) {
+  using C = RowPCfg<N>;
+  using RF = fft::RowFFT<N, C::WLINES, C::LS, C::WNT, C::SH>;
+  __shared__ float2 buf[C::WLINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const float2* rt = a.rt + (size_t)b * a.wc * N;
+  float2* mh = a.mhat + (size_t)b * a.wr * a.wc;
+  const int j0 = blockIdx.x * C::WLINES;
+  RF::forward(
+      buf, a.rtw,
+      [&](int l, int y) { return j0 + l < a.wc ? rt[(size_t)(j0 + l) * N + y] : make_float2(0.f, 0.f); },
+      [&](int l, int u, float2 z) {
+        const int i = a.row_of_u[u];
+        if (j0 + l < a.wc && i >= 0) mh[(size_t)i * a.wc + j0 + l] = z;
+      });
+}
+
+// K3: T_k rows = inverse row transforms of H~_k x M^ over the window columns.
+template <int N>
+__global__ void __launch_bounds__(RowPCfg<N>::NT)
+k_row_inv_pruned(SocsArgs a) {
+  using C = RowPCfg<N>;
+  using RF = fft::RowFFT<N, C::LINES, C::LS, C::NT, C::SH>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int rows = a.k * a.wr;
+  const int q0 = blockIdx.x * C::LINES;
+  const float2* mh = a.mhat + (size_t)b * a.wr * a.wc;
+  float2* t = a.t + (size_t)b * rows * N;
+  RF::inverse(
+      buf, a.rtw,
+      [&](int l, int u) {
+        const int q = q0 + l, j = a.col_of_u[u];
+        if (q >= rows || j < 0) return make_float2(0.f, 0.f);
+        return cmul(a.h[(size_t)q * a.wc + j], mh[(size_t)(q % a.wr) * a.wc + j]);
+      },
+      [&](int l, int x, float2 z) {
+        if (q0 + l < rows) t[(size_t)(q0 + l) * N + x] = z;
+      });
+}
+
+// K6: Y_k = conj(H~_k) x forward row transforms of V_k, window columns only.
+template <int N>
+__global__ void __launch_bounds__(RowPCfg<N>::NT)
+k_row_adj_pruned(SocsArgs a) {
+  using C = RowPCfg<N>;
+  using RF = fft::RowFFT<N, C::LINES, C::LS, C::NT, C::SH>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int rows = a.k * a.wr;
+  const int q0 = blockIdx.x * C::LINES;
+  const float2* v = a.v + (size_t)b * rows * N;
+  float2* yk = a.y + (size_t)b * rows * a.wc;
+  RF::forward(
+      buf, a.rtw,
+      [&](int l, int x) { return q0 + l < rows ? v[(size_t)(q0 + l) * N + x] : make_float2(0.f, 0.f); },
+      [&](int l, int u, float2 z) {
+        const int q = q0 + l, j = a.col_of_u[u];
+        if (q < rows && j >= 0) yk[(size_t)q * a.wc + j] = cmulc(z, a.h[(size_t)q * a.wc + j]);
+      });
+}
+
+// K7: P rows = inverse row transforms of sum_k Y_k (k in increasing order).
+template <int N>
+__global__ void __launch_bounds__(RowPCfg<N>::WNT)
+k_final_row_inv_pruned(SocsArgs a) {
+  using C = RowPCfg<N>;
+  using RF = fft::RowFFT<N, C::WLINES, C::LS, C::WNT, C::SH>;
+  __shared__ float2 buf[C::WLINES * C::LS];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int rows = a.k * a.wr;
+  const int i0 = blockIdx.x * C::WLINES;
+  const float2* yk = a.y + (size_t)b * rows * a.wc;
+  float2* p = a.p + (size_t)b * a.wr * N;
+  RF::inverse(
+      buf, a.rtw,
+      [&](int l, int u) {
+        const int i = i0 + l, j = a.col_of_u[u];
+        float2 s = make_float2(0.f, 0.f);
+        if (i >= a.wr || j < 0) return s;
+        for (int k = 0; k < a.k; ++k) {
+          const float2 w = yk[((size_t)k * a.wr + i) * a.wc + j];
+          s.x += w.x;
+          s.y += w.y;
+        }
+        return s;
+      },
+      [&](int l, int x, float2 z) {
+        if (i0 + l < a.wr) p[(size_t)(i0 + l) * N + x] = z;
+      });
+}
+
 // ---------------------------------------------------------------- K8
 template <int N>
 __global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
@@ -698,12 +887,47 @@ struct SocsImpl {
   static void forward_half(const SocsArgs& a, int batch, int mode, const float* base, const float* dir,
                            const float* coef, float* theta_out, float* mask_out, hipStream_t s) {
     using R = RowCfg<N>;
-    dim3 g1((N + 2 * R::LINES - 1) / (2 * R::LINES), batch);
-    hipLaunchKernelGGL(k_row_fwd<N>, g1, dim3(R::NT), 0, s, a, mode, base, dir, coef, theta_out, mask_out);
-    dim3 g2((a.wc + R::LINES - 1) / R::LINES, batch);
-    hipLaunchKernelGGL(k_col_fwd<N>, g2, dim3(R::NT), 0, s, a);
-    dim3 g3((a.k * a.wr + R::LINES - 1) / R::LINES, batch);
-    hipLaunchKernelGGL(k_row_inv<N>, g3, dim3(R::NT), 0, s, a);
+    using P = RowPCfg<N>;
+    constexpr bool kPrunable = fft::factorize(N).count >= 2;
+    // K1: mask rows -> row spectra at the window columns
+    bool done = false;
+    if constexpr (kPrunable) {
+      if (a.row_pruned) {
+        dim3 g1((N + 2 * P::LINES - 1) / (2 * P::LINES), batch);
+        hipLaunchKernelGGL(k_row_fwd_pruned<N>, g1, dim3(P::NT), 0, s, a, mode, base, dir, coef, theta_out, mask_out);
+        done = true;
+      }
+    }
+    if (!done) {
+      dim3 g1((N + 2 * R::LINES - 1) / (2 * R::LINES), batch);
+      hipLaunchKernelGGL(k_row_fwd<N>, g1, dim3(R::NT), 0, s, a, mode, base, dir, coef, theta_out, mask_out);
+    }
+    // K2: columns -> mask spectrum window
+    done = false;
+    if constexpr (kPrunable) {
+      if (a.pruned) {
+        dim3 g2((a.wc + P::WLINES - 1) / P::WLINES, batch);
+        hipLaunchKernelGGL(k_col_fwd_pruned<N>, g2, dim3(P::WNT), 0, s, a);
+        done = true;
+      }
+    }
+    if (!done) {
+      dim3 g2((a.wc + R::LINES - 1) / R::LINES, batch);
+      hipLaunchKernelGGL(k_col_fwd<N>, g2, dim3(R::NT), 0, s, a);
+    }
+    // K3: T_k rows
+    done = false;
+    if constexpr (kPrunable) {
+      if (a.row_pruned) {
+        dim3 g3((a.k * a.wr + P::LINES - 1) / P::LINES, batch);
+        hipLaunchKernelGGL(k_row_inv_pruned<N>, g3, dim3(P::NT), 0, s, a);
+        done = true;
+      }
+    }
+    if (!done) {
+      dim3 g3((a.k * a.wr + R::LINES - 1) / R::LINES, batch);
+      hipLaunchKernelGGL(k_row_inv<N>, g3, dim3(R::NT), 0, s, a);
+    }
   }
 
   static void image_adjoint(const SocsArgs& a, const ImageArgs& im, int batch, hipStream_t s) {
@@ -720,11 +944,24 @@ struct SocsImpl {
 
   static void adjoint_tail(const SocsArgs& a, const GradArgs& ga, int batch, hipStream_t s) {
     using R = RowCfg<N>;
+    using P = RowPCfg<N>;
     using C = ColCfg<N>;
-    dim3 g6((a.k * a.wr + R::LINES - 1) / R::LINES, batch);
-    hipLaunchKernelGGL(k_row_adj<N>, g6, dim3(R::NT), 0, s, a);
-    dim3 g7((a.wr + R::LINES - 1) / R::LINES, batch);
-    hipLaunchKernelGGL(k_final_row_inv<N>, g7, dim3(R::NT), 0, s, a);
+    bool done = false;
+    if constexpr (fft::factorize(N).count >= 2) {
+      if (a.row_pruned) {
+        dim3 g6((a.k * a.wr + P::LINES - 1) / P::LINES, batch);
+        hipLaunchKernelGGL(k_row_adj_pruned<N>, g6, dim3(P::NT), 0, s, a);
+        dim3 g7((a.wr + P::WLINES - 1) / P::WLINES, batch);
+        hipLaunchKernelGGL(k_final_row_inv_pruned<N>, g7, dim3(P::WNT), 0, s, a);
+        done = true;
+      }
+    }
+    if (!done) {
+      dim3 g6((a.k * a.wr + R::LINES - 1) / R::LINES, batch);
+      hipLaunchKernelGGL(k_row_adj<N>, g6, dim3(R::NT), 0, s, a);
+      dim3 g7((a.wr + R::LINES - 1) / R::LINES, batch);
+      hipLaunchKernelGGL(k_final_row_inv<N>, g7, dim3(R::NT), 0, s, a);
+    }
     dim3 g8(col_blocks(), batch);
     if constexpr (fft::factorize(N).count >= 2) {
       if (a.pruned) {
@@ -750,6 +987,20 @@ static const SocsOps* make_ops() {
 int first_radix(int n) {
   const fft::Radices f = fft::factorize(n);
   return f.count >= 2 ? f.r[0] : 0;
+}
+
+std::vector<float2> row_stage_twiddles(int n) {
+  const fft::Radices f = fft::factorize(n);
+  if (f.count < 2) return {};
+  const fft::StageTables t = fft::stage_tables(n);
+  std::vector<float2> out((size_t)t.off[t.count]);
+  for (int st = 0; st < t.count; ++st)
+    for (int r = 1; r < t.R[st]; ++r)
+      for (int k = 0; k < t.P[st]; ++k) {
+        const double ang = -2.0 * fft::kPi * (double)r * (double)k / ((double)t.P[st] * (double)t.R[st]);
+        out[(size_t)t.off[st] + (size_t)(r - 1) * t.P[st] + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+      }
+  return out;
 }
 
 const SocsOps* socs_ops_for(int n) {

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace litho {
 
 // Tile sizes with compiled FFT instantiations: the CPU-test sizes, the Micro
@@ -29,7 +31,10 @@ struct SocsArgs {
   float2* p;                      // [b][wr][N]
   const int* active;              // [b] nonzero = evaluate tile b; null = all tiles
   const int* row_of_u;            // [N] window row of column frequency u, or -1
-  int pruned;                     // window fits the pruned column transforms
+  int pruned;                     // window rows fit the pruned column transforms
+  const int* col_of_u;            // [N] window column of row frequency u, or -1
+  const float2* rtw;              // stage twiddle tables of the row transforms (row_stage_twiddles)
+  int row_pruned;                 // window columns fit the pruned row transforms
 };
 
 struct ImageArgs {
@@ -66,6 +71,10 @@ struct SocsOps {
 const SocsOps* socs_ops_for(int n);
 // First radix of the column factorisation of n (0 if n has a single stage).
 int first_radix(int n);
+// Stage twiddle tables of the row transforms of length n (layout of
+// fft::stage_tables), computed in FP64 and rounded to FP32; empty if n has a
+// single stage.
+std::vector<float2> row_stage_twiddles(int n);
 void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s);
 
 }  // namespace litho
