@@ -198,7 +198,6 @@ struct CtmPool::Impl {
   }
   virtual bool admit(const CtmJob& j) = 0;
   virtual void step() = 0;
-  virtual size_t slot_state_bytes() const = 0;
 };
 
 // ------------------------------------------------------------ gradient descent
@@ -225,7 +224,6 @@ struct GdPool final : CtmPool::Impl {
     rq.ga.theta = x.as<float>();
     graph = std::make_unique<EvalGraph>(plan, S, rq);
   }
-  size_t slot_state_bytes() const override { return 5 * n2 * sizeof(float); }
 
   bool admit(const CtmJob& j) override {
     const int b = find_free();
@@ -300,33 +298,38 @@ struct GdPool final : CtmPool::Impl {
 
 // ------------------------------------------------------------------ L-BFGS
 struct LbfgsPool final : CtmPool::Impl {
-  DevMem x, xp, g, gp, d, mask, hs, hy, pair_part, dir_part, vec_red;
-  HostPinned hvec;
+  DevMem x, xp, g, gp, d, hs, hy, pair_part, dir_part, vec_red, dg_red;
+  HostPinned hvec, hdg;
   std::vector<LbfgsTile> st;
+  // g . d of a new direction arrives with the evaluation of its first trial
+  // (step 1): the line search starts once both are on the host
+  std::vector<char> pending_dg0;
   std::unique_ptr<EvalGraph> graph;
   const LineSearchParams lp;
   const int m_max;
 
   LbfgsPool(Plan& p, int slots, const CtmOptions& o)
-      : Impl(p, slots, o), hvec((size_t)slots * kPairDots * sizeof(double)), st(slots), m_max(o.history) {
+      : Impl(p, slots, o), hvec((size_t)slots * kPairDots * sizeof(double)), hdg((size_t)slots * sizeof(double)),
+        st(slots), pending_dg0(slots, 0), m_max(o.history) {
     alloc_slots(x, S, n2);
     alloc_slots(xp, S, n2);
     alloc_slots(g, S, n2);
     alloc_slots(gp, S, n2);
     alloc_slots(d, S, n2);
-    alloc_slots(mask, S, n2);
-    alloc_slots(hs, S, n2, kMaxHist + 1);
-    alloc_slots(hy, S, n2, kMaxHist + 1);
+    hs.alloc((size_t)S * hist_tile_stride(n2) * sizeof(float));
+    hy.alloc((size_t)S * hist_tile_stride(n2) * sizeof(float));
     pair_part.alloc((size_t)S * kOptBlocks * kPairDots * sizeof(double));
     dir_part.alloc((size_t)S * kOptBlocks * sizeof(double));
     vec_red.alloc((size_t)S * kPairDots * sizeof(double));
+    dg_red.alloc((size_t)S * sizeof(double));
     int* meta = pm.meta();
     double* coefs = pm.coef();
     for (int i = 0; i < S * kMetaStride; ++i) meta[i] = -1;
     for (int i = 0; i < S * kCoefStride; ++i) coefs[i] = 0.0;
     // every evaluation reads the same buffers: theta = xp + step * d for the
     // active slots, gradient to g, g . d for the line search
-    EvalRequest rq = base_request(mask.as<float>(), g.as<float>());
+    // (the masks of the trials are not kept: the result is rebuilt from theta)
+    EvalRequest rq = base_request(nullptr, g.as<float>());
     rq.base = xp.as<float>();
     rq.dir = d.as<float>();
     rq.coef = pm.d_step();
@@ -335,7 +338,6 @@ struct LbfgsPool final : CtmPool::Impl {
     rq.ga.dir = d.as<float>();
     graph = std::make_unique<EvalGraph>(plan, S, rq);
   }
-  size_t slot_state_bytes() const override { return (size_t)(6 + 2 * (kMaxHist + 1)) * n2 * sizeof(float); }
 
   bool admit(const CtmJob& j) override {
     const int b = find_free();
@@ -343,6 +345,7 @@ struct LbfgsPool final : CtmPool::Impl {
     load_slot(b, j, xp.as<float>());
     LITHO_HIP(hipMemsetAsync(d.as<float>() + b * n2, 0, n2 * sizeof(float), s));   // first trial: x = xp + 0 * d
     st[b] = LbfgsTile{};
+    pending_dg0[b] = 0;
     return true;
   }
 
@@ -490,14 +493,18 @@ struct LbfgsPool final : CtmPool::Impl {
     pm.upload_meta_coef(s);
     launch_lbfgs_direction(g.as<float>(), hs.as<float>(), hy.as<float>(), pm.d_meta(), pm.d_coef(), n2, d.as<float>(),
                            S, dir_m, dir_part.as<double>(), s);
-    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, vec_red.as<double>(), 1, S, s);
-    LITHO_HIP(hipMemcpyAsync(hvec.ptr, vec_red.ptr, (size_t)S * sizeof(double), hipMemcpyDeviceToHost, s));
-    LITHO_HIP(hipStreamSynchronize(s));
+    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, dg_red.as<double>(), 1, S, s);
+    LITHO_HIP(hipMemcpyAsync(hdg.ptr, dg_red.ptr, (size_t)S * sizeof(double), hipMemcpyDeviceToHost, s));
+    // No synchronisation here: the first trial (step 1) is evaluated right
+    // away and the line search starts when g . d arrives with its result.
     for (int b = 0; b < S; ++b) {
       if (!running[b] || st[b].phase != LbfgsTile::NEED_PAIR) continue;
-      st[b].step = 1.0;
-      st[b].k += 1;
-      begin_line_search(b, hvec.as<double>()[b]);
+      LbfgsTile& t = st[b];
+      t.step = 1.0;
+      t.k += 1;
+      t.a = t.step;
+      t.phase = LbfgsTile::BRACKET;
+      pending_dg0[b] = 1;
     }
   }
 
@@ -522,6 +529,14 @@ struct LbfgsPool final : CtmPool::Impl {
     for (int b = 0; b < S; ++b) {
       if (!flags[b]) continue;
       LbfgsTile& t = st[b];
+      if (pending_dg0[b]) {
+        // the direction's g . d (copied before this evaluation on the same
+        // stream); a non-descent direction ends the tile without counting
+        // the trial, as if it had not been evaluated
+        pending_dg0[b] = 0;
+        begin_line_search(b, hdg.as<double>()[b]);
+        if (t.phase == LbfgsTile::DONE) continue;
+      }
       const double f = red[4 * b + 0];
       const double gn2 = red[4 * b + 2];
       const double dg = red[4 * b + 3];
@@ -643,8 +658,8 @@ std::vector<std::pair<long long, CtmTileResult>> CtmPool::take_finished() {
   return out;
 }
 size_t CtmPool::slot_bytes(const Plan& plan, const CtmOptions& opt) {
-  const size_t n2 = plan.tile_elems() * sizeof(float);
-  return (opt.method == 0 ? 5 : 7 + 2 * (kMaxHist + 1)) * n2;
+  const size_t n2 = plan.tile_elems();
+  return (opt.method == 0 ? 5 * n2 : 6 * n2 + 2 * hist_tile_stride(n2)) * sizeof(float);
 }
 
 std::vector<CtmTileResult> run_ctm(Plan& plan, int batch, const float* targets, const float* theta0,

@@ -85,17 +85,18 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double* __
   }
 }
 
-__device__ __forceinline__ double dot2(float2 a, float2 b) {
-  return fma((double)a.x, (double)b.x, (double)a.y * (double)b.y);
+__device__ __forceinline__ double dot4(float4 a, float4 b) {
+  return fma((double)a.x, (double)b.x, fma((double)a.y, (double)b.y, fma((double)a.z, (double)b.z, (double)a.w * (double)b.w)));
 }
+__device__ __forceinline__ float4 sub4(float4 a, float4 b) { return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
 
 // L-BFGS pair update.  For tile b: s_new = x - xp, y_new = g - gp written to
 // slot new_slot of the history; FP64 dot products per block, NV = 4 (MB + 1):
 //   [4j+0] s_j . y_new   [4j+1] y_j . y_new   [4j+2] s_j . g   [4j+3] y_j . g
 // for the active slots j < m <= MB (slot index slots[j]), then the new pair at
 // position 4 MB.  With shift set, xp <- x and gp <- g afterwards (the accepted
-// point becomes the base of the next line search).  Vectors are read as float2:
-// n is even for every supported tile (N^2 with N even).
+// point becomes the base of the next line search).  Vectors are read as
+// float4: n is a multiple of 4 for every supported tile (N^2 with N even).
 template <int MB>
 __global__ void __launch_bounds__(kOptThreads)
 k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* __restrict__ g,
@@ -108,51 +109,51 @@ k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* _
   const int new_slot = meta[b * kMetaStride + 1];
   const int* slots = meta + b * kMetaStride + 2;
   const size_t tb = (size_t)b * n;
-  const size_t hist = (size_t)(kMaxHist + 1) * n;  // per-tile history stride
+  const size_t hist = hist_tile_stride(n);
   float* hs_b = hs + (size_t)b * hist;
   float* hy_b = hy + (size_t)b * hist;
   double acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  const float2* sp[MB > 0 ? MB : 1];
-  const float2* yp[MB > 0 ? MB : 1];
+  const float4* sp[MB > 0 ? MB : 1];
+  const float4* yp[MB > 0 ? MB : 1];
 #pragma unroll
   for (int j = 0; j < MB; ++j) {
     const int sl = j < m ? slots[j] : 0;
-    sp[j] = reinterpret_cast<const float2*>(hs_b + (size_t)sl * n);
-    yp[j] = reinterpret_cast<const float2*>(hy_b + (size_t)sl * n);
+    sp[j] = reinterpret_cast<const float4*>(hs_b + (size_t)sl * hist_slot_stride(n));
+    yp[j] = reinterpret_cast<const float4*>(hy_b + (size_t)sl * hist_slot_stride(n));
   }
-  float2* sn = reinterpret_cast<float2*>(hs_b + (size_t)new_slot * n);
-  float2* yn = reinterpret_cast<float2*>(hy_b + (size_t)new_slot * n);
-  const float2* x2 = reinterpret_cast<const float2*>(x + tb);
-  float2* xp2 = reinterpret_cast<float2*>(xp + tb);
-  const float2* g2 = reinterpret_cast<const float2*>(g + tb);
-  float2* gp2 = reinterpret_cast<float2*>(gp + tb);
-  const size_t n2 = n / 2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
-    const float2 xv = x2[i], xpv = xp2[i], gv = g2[i], gpv = gp2[i];
-    const float2 s = make_float2(xv.x - xpv.x, xv.y - xpv.y);
-    const float2 y = make_float2(gv.x - gpv.x, gv.y - gpv.y);
+  float4* sn = reinterpret_cast<float4*>(hs_b + (size_t)new_slot * hist_slot_stride(n));
+  float4* yn = reinterpret_cast<float4*>(hy_b + (size_t)new_slot * hist_slot_stride(n));
+  const float4* x4 = reinterpret_cast<const float4*>(x + tb);
+  float4* xp4 = reinterpret_cast<float4*>(xp + tb);
+  const float4* g4 = reinterpret_cast<const float4*>(g + tb);
+  float4* gp4 = reinterpret_cast<float4*>(gp + tb);
+  const size_t n4 = n / 4;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+    const float4 xv = x4[i], xpv = xp4[i], gv = g4[i], gpv = gp4[i];
+    const float4 s = sub4(xv, xpv);
+    const float4 y = sub4(gv, gpv);
     sn[i] = s;
     yn[i] = y;
     if (shift) {
-      xp2[i] = xv;
-      gp2[i] = gv;
+      xp4[i] = xv;
+      gp4[i] = gv;
     }
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
       if (j < m) {
-        const float2 sj = sp[j][i], yj = yp[j][i];
-        acc[4 * j + 0] += dot2(sj, y);
-        acc[4 * j + 1] += dot2(yj, y);
-        acc[4 * j + 2] += dot2(sj, gv);
-        acc[4 * j + 3] += dot2(yj, gv);
+        const float4 sj = sp[j][i], yj = yp[j][i];
+        acc[4 * j + 0] += dot4(sj, y);
+        acc[4 * j + 1] += dot4(yj, y);
+        acc[4 * j + 2] += dot4(sj, gv);
+        acc[4 * j + 3] += dot4(yj, gv);
       }
     }
-    acc[4 * MB + 0] += dot2(s, y);
-    acc[4 * MB + 1] += dot2(y, y);
-    acc[4 * MB + 2] += dot2(s, gv);
-    acc[4 * MB + 3] += dot2(y, gv);
+    acc[4 * MB + 0] += dot4(s, y);
+    acc[4 * MB + 1] += dot4(y, y);
+    acc[4 * MB + 2] += dot4(s, gv);
+    acc[4 * MB + 3] += dot4(y, gv);
   }
   block_reduce_store<NV>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x) * NV);
 }
@@ -171,38 +172,40 @@ k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, con
   const int* slots = meta + b * kMetaStride + 2;
   const double* cb = coef + (size_t)b * kCoefStride;
   const size_t tb = (size_t)b * n;
-  const size_t hist = (size_t)(kMaxHist + 1) * n;
+  const size_t hist = hist_tile_stride(n);
   const float* hs_b = hs + (size_t)b * hist;
   const float* hy_b = hy + (size_t)b * hist;
   const float cg = (float)cb[0];
   float cs[MB > 0 ? MB : 1], cy[MB > 0 ? MB : 1];
-  const float2* sp[MB > 0 ? MB : 1];
-  const float2* yp[MB > 0 ? MB : 1];
+  const float4* sp[MB > 0 ? MB : 1];
+  const float4* yp[MB > 0 ? MB : 1];
 #pragma unroll
   for (int j = 0; j < MB; ++j) {
     const int sl = j < m ? slots[j] : 0;
     cs[j] = j < m ? (float)cb[1 + j] : 0.f;
     cy[j] = j < m ? (float)cb[1 + kMaxHist + j] : 0.f;
-    sp[j] = reinterpret_cast<const float2*>(hs_b + (size_t)sl * n);
-    yp[j] = reinterpret_cast<const float2*>(hy_b + (size_t)sl * n);
+    sp[j] = reinterpret_cast<const float4*>(hs_b + (size_t)sl * hist_slot_stride(n));
+    yp[j] = reinterpret_cast<const float4*>(hy_b + (size_t)sl * hist_slot_stride(n));
   }
-  const float2* g2 = reinterpret_cast<const float2*>(g + tb);
-  float2* d2 = reinterpret_cast<float2*>(d + tb);
+  const float4* g4 = reinterpret_cast<const float4*>(g + tb);
+  float4* d4 = reinterpret_cast<float4*>(d + tb);
   double acc[1] = {0.0};
-  const size_t n2 = n / 2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
-    const float2 gv = g2[i];
-    float2 v = make_float2(cg * gv.x, cg * gv.y);
+  const size_t n4 = n / 4;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+    const float4 gv = g4[i];
+    float4 v = make_float4(cg * gv.x, cg * gv.y, cg * gv.z, cg * gv.w);
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
       if (j < m) {
-        const float2 sj = sp[j][i], yj = yp[j][i];
+        const float4 sj = sp[j][i], yj = yp[j][i];
         v.x = fmaf(cs[j], sj.x, fmaf(cy[j], yj.x, v.x));
         v.y = fmaf(cs[j], sj.y, fmaf(cy[j], yj.y, v.y));
+        v.z = fmaf(cs[j], sj.z, fmaf(cy[j], yj.z, v.z));
+        v.w = fmaf(cs[j], sj.w, fmaf(cy[j], yj.w, v.w));
       }
     }
-    d2[i] = v;
-    acc[0] += dot2(gv, v);
+    d4[i] = v;
+    acc[0] += dot4(gv, v);
   }
   block_reduce_store<1>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x));
 }

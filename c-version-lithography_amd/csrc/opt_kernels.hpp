@@ -8,11 +8,18 @@
 namespace litho {
 
 constexpr int kOptThreads = 256;
-constexpr int kOptBlocks = 1024;         // grid-stride blocks per tile of the L-BFGS vector kernels
+#ifndef LITHO_OPT_BLOCKS
+#define LITHO_OPT_BLOCKS 1024
+#endif
+constexpr int kOptBlocks = LITHO_OPT_BLOCKS;  // grid-stride blocks per tile of the L-BFGS vector kernels
 constexpr int kMaxHist = 12;             // largest supported L-BFGS history m
 constexpr int kPairDots = 4 * (kMaxHist + 1);
 constexpr int kMetaStride = 2 + kMaxHist;   // [m, new_slot, slots[kMaxHist]]
 constexpr int kCoefStride = 1 + 2 * kMaxHist;
+// Floats between consecutive L-BFGS history slots of one tile, and per tile
+// (kMaxHist + 1 slots: m pairs plus the spare that receives the next one).
+__host__ __device__ constexpr size_t hist_slot_stride(size_t n) { return n; }
+__host__ __device__ constexpr size_t hist_tile_stride(size_t n) { return (size_t)(kMaxHist + 1) * hist_slot_stride(n); }
 
 void launch_init_theta(const float* pattern, float* theta, size_t count, hipStream_t s);
 void launch_scale(const float* x, float* y, const float* coef, const int* active, size_t n, int batch,

@@ -56,10 +56,13 @@ struct RowCfg {
 // over all K * W_r spectrum rows, WLINES for the kernels over the W_r or W_c
 // window lines, which are too few to fill the chip with wide blocks.
 #ifndef LITHO_ROWP_LINES_LARGE
-#define LITHO_ROWP_LINES_LARGE 2
+#define LITHO_ROWP_LINES_LARGE 1
 #endif
 #ifndef LITHO_ROWP_WLINES_LARGE
 #define LITHO_ROWP_WLINES_LARGE 1
+#endif
+#ifndef LITHO_ROWP_LINES_MID
+#define LITHO_ROWP_LINES_MID 2
 #endif
 // about one radix-16 butterfly per thread, in whole waves, 64..256 threads
 constexpr int row_threads(int lines, int n) {
@@ -69,7 +72,7 @@ template <int N>
 struct RowPCfg {
   static constexpr int SH = 5;
   static constexpr int LS = fft::padded_stride<SH>(N);
-  static constexpr int LINES = N >= 1024 ? LITHO_ROWP_LINES_LARGE : (N >= 256 ? 4 : 8);
+  static constexpr int LINES = N >= 1024 ? LITHO_ROWP_LINES_LARGE : (N >= 256 ? LITHO_ROWP_LINES_MID : 8);
   static constexpr int NT = row_threads(LINES, N);
   static constexpr int WLINES = N >= 1024 ? LITHO_ROWP_WLINES_LARGE : (N >= 256 ? 2 : 4);
   static constexpr int WNT = row_threads(WLINES, N);
@@ -98,9 +101,14 @@ constexpr int col_nt_mid(int lines, int n) {
                               : ((lines * n / 8 + 63) / 64 * 64 > 1024 ? 1024 : (lines * n / 8 + 63) / 64 * 64);
 }
 
-// columns per block of the column kernels for N >= 1024
+// columns per block of the column kernels for N >= 1024 and for 640 <= N < 1024
+// (measured: 8 x 1024 threads best from 1152 up, 4 x 512 threads for
+// 640..896, 8 x 512 threads below 640 when several tiles share a launch)
 #ifndef LITHO_COL_LINES_LARGE
 #define LITHO_COL_LINES_LARGE 8
+#endif
+#ifndef LITHO_COL_LINES_MID
+#define LITHO_COL_LINES_MID 4
 #endif
 
 // largest radix of the factorisation: 11, 13 and 17 run as direct O(R^2)
@@ -115,7 +123,8 @@ constexpr int max_radix(int n) {
 template <int N>
 struct ColCfg {
   static constexpr bool BIG_ODD = N >= 1024 && max_radix(N) > 16;
-  static constexpr int LINES = BIG_ODD ? 4 : (N >= 1024 ? LITHO_COL_LINES_LARGE : (N >= 256 ? 8 : 16));
+  static constexpr int LINES =
+      BIG_ODD ? 4 : (N >= 1024 ? LITHO_COL_LINES_LARGE : (N >= 640 ? LITHO_COL_LINES_MID : (N >= 256 ? 8 : 16)));
   static constexpr int NT = BIG_ODD ? 512 : (N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? col_nt_mid(LINES, N) : 64));
   // swizzle shift: lanes spread over LINES lines; the four-column blocks of the
   // large odd sizes behave like the row kernels
