@@ -27,6 +27,18 @@ namespace {
 
 __device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
 
+// 1/x for the WENO weights: hardware reciprocal refined by two Newton-Raphson
+// steps r <- r + r(1 - x r), which square the relative error each time (well
+// below one ulp of double from the estimate's ~2^-26 start).  x lies in
+// [1e-198, ~1e300] here, so the range scaling of a full IEEE division is not
+// needed; the result differs from 1/x by at most an ulp or so.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+}
+
 // WENO5-JS combination of five one-sided differences (level_set_utils.cpp:56-111).
 __device__ __forceinline__ double weno5(double v1, double v2, double v3, double v4, double v5) {
   const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0, c16 = 1.0 / 6.0, c56 = 5.0 / 6.0;
@@ -41,24 +53,42 @@ __device__ __forceinline__ double weno5(double v1, double v2, double v3, double 
   const double s3 = cs1 * t3 * t3 + cs2 * e3 * e3;
   const double vmax2 = fmax(fmax(fmax(v1 * v1, v2 * v2), fmax(v3 * v3, v4 * v4)), v5 * v5);
   const double eps = 1e-6 * vmax2 + 1e-99;
-  const double a1 = 0.1 / ((s1 + eps) * (s1 + eps));
-  const double a2 = 0.6 / ((s2 + eps) * (s2 + eps));
-  const double a3 = 0.3 / ((s3 + eps) * (s3 + eps));
-  return (a1 * d1 + a2 * d2 + a3 * d3) / (a1 + a2 + a3);
+  // (s + eps)^2 >= 1e-198, and the weights stay below ~1e198
+  const double a1 = 0.1 * rcp_nr((s1 + eps) * (s1 + eps));
+  const double a2 = 0.6 * rcp_nr((s2 + eps) * (s2 + eps));
+  const double a3 = 0.3 * rcp_nr((s3 + eps) * (s3 + eps));
+  return (a1 * d1 + a2 * d2 + a3 * d3) * rcp_nr(a1 + a2 + a3);
 }
 
-// One line of the tile: value at clamped index j + o.
-struct Line {
-  const float* p;
-  int stride, last;
-  __device__ __forceinline__ double at(int j) const { return (double)p[(size_t)clampi(j, last) * stride]; }
+// phi of one Micro at absolute (yy, xx), clamped into the tile (the
+// reference's edge padding), read from global memory.
+struct GlobalPhi {
+  const float* t;
+  int rows, cols;
+  __device__ __forceinline__ double operator()(int yy, int xx) const {
+    return (double)t[(size_t)clampi(yy, rows - 1) * cols + clampi(xx, cols - 1)];
+  }
 };
 
+// The same from an LDS copy of rows [y0 - 3, y0 + TY + 3) x columns
+// [x0 - 3, x0 + TX + 3) filled with the clamped values: every stencil access
+// lies within 3 pixels of its output pixel.
+template <int W>
+struct LdsPhi {
+  const float* s;
+  int y0, x0;
+  __device__ __forceinline__ double operator()(int yy, int xx) const {
+    return (double)s[(yy - y0 + 3) * W + (xx - x0 + 3)];
+  }
+};
+
+// f(o) = phi at offset o in [-3, 3] along the line through the pixel.
 // minus: v_m = (f[j+m-2] - f[j+m-3]) / d, m = 0..4; plus: v_q = (f[j+4-q] - f[j+3-q]) / d.
-__device__ __forceinline__ void weno_pair(const Line& f, int j, double inv_d, double& minus, double& plus) {
+template <class F>
+__device__ __forceinline__ void weno_pair(const F& f, double inv_d, double& minus, double& plus) {
   double f_[7];
 #pragma unroll
-  for (int o = 0; o < 7; ++o) f_[o] = f.at(j + o - 3);    // f[j-3] .. f[j+3]
+  for (int o = 0; o < 7; ++o) f_[o] = f(o - 3);    // f[j-3] .. f[j+3]
   double d[6];
 #pragma unroll
   for (int o = 0; o < 6; ++o) d[o] = (f_[o + 1] - f_[o]) * inv_d;  // d[o] = f[j+o-2] - f[j+o-3]
@@ -82,34 +112,37 @@ __device__ __forceinline__ double godunov(double vn, double pxm, double pxp, dou
 }
 
 // |grad phi| normal term at (y, x): Vn |grad phi|.
-__device__ __forceinline__ double ls_normal_term(const float* tile, int rows, int cols, int y, int x, double vn,
-                                                 double inv_dx, double inv_dy) {
+template <class P>
+__device__ __forceinline__ double ls_normal_term(const P& phi, int y, int x, double vn, double inv_dx, double inv_dy) {
   if (vn == 0.0) return 0.0;
   double pxm, pxp, pym, pyp;
-  weno_pair(Line{tile + (size_t)y * cols, 1, cols - 1}, x, inv_dx, pxm, pxp);
-  weno_pair(Line{tile + x, cols, rows - 1}, y, inv_dy, pym, pyp);
+  weno_pair([&](int o) { return phi(y, x + o); }, inv_dx, pxm, pxp);
+  weno_pair([&](int o) { return phi(y + o, x); }, inv_dy, pym, pyp);
   return vn * godunov(vn, pxm, pxp, pym, pyp);
 }
 
-// _gradient_1d at index j of a line of length n (n >= 3) given a getter.
+// _gradient_1d at index j of a line of length n (n >= 3) given a getter;
+// inv_d2 = 1 / (2 d) (exact for the unit grid spacing of the reference).
 template <class G>
-__device__ __forceinline__ double grad1(const G& g, int j, int n, double d2) {
-  if (j == 0) return (-3.0 * g(0) + 4.0 * g(1) - g(2)) / d2;
-  if (j == n - 1) return (3.0 * g(n - 1) - 4.0 * g(n - 2) + g(n - 3)) / d2;
-  return (g(j + 1) - g(j - 1)) / d2;
+__device__ __forceinline__ double grad1(const G& g, int j, int n, double inv_d2) {
+  if (j == 0) return (-3.0 * g(0) + 4.0 * g(1) - g(2)) * inv_d2;
+  if (j == n - 1) return (3.0 * g(n - 1) - 4.0 * g(n - 2) + g(n - 3)) * inv_d2;
+  return (g(j + 1) - g(j - 1)) * inv_d2;
 }
 
-// evolve_kappa at (y, x) (level_set_utils.cpp:257-278).
-__device__ __forceinline__ double ls_kappa_term(const float* tile, int rows, int cols, int y, int x, double dx,
-                                                double dy, double b) {
-  const double dx2 = 2.0 * dx, dy2 = 2.0 * dy;
-  auto phi = [&](int yy, int xx) { return (double)tile[(size_t)yy * cols + xx]; };
-  auto px_at = [&](int yy, int xx) { return grad1([&](int k) { return phi(yy, k); }, xx, cols, dx2); };
-  auto py_at = [&](int yy, int xx) { return grad1([&](int k) { return phi(k, xx); }, yy, rows, dy2); };
+// evolve_kappa at (y, x) (level_set_utils.cpp:257-278).  The one-sided
+// differences at the tile edges keep every access inside the tile and
+// within 3 pixels of (y, x).
+template <class P>
+__device__ __forceinline__ double ls_kappa_term(const P& phi, int rows, int cols, int y, int x, double dx, double dy,
+                                                double b) {
+  const double inv_dx2 = 1.0 / (2.0 * dx), inv_dy2 = 1.0 / (2.0 * dy);
+  auto px_at = [&](int yy, int xx) { return grad1([&](int k) { return phi(yy, k); }, xx, cols, inv_dx2); };
+  auto py_at = [&](int yy, int xx) { return grad1([&](int k) { return phi(k, xx); }, yy, rows, inv_dy2); };
   const double px = px_at(y, x), py = py_at(y, x);
-  const double pxy = grad1([&](int k) { return px_at(k, x); }, y, rows, dy2);
-  const double pxx = grad1([&](int k) { return px_at(y, k); }, x, cols, dx2);
-  const double pyy = grad1([&](int k) { return py_at(k, x); }, y, rows, dy2);
+  const double pxy = grad1([&](int k) { return px_at(k, x); }, y, rows, inv_dy2);
+  const double pxx = grad1([&](int k) { return px_at(y, k); }, x, cols, inv_dx2);
+  const double pyy = grad1([&](int k) { return py_at(k, x); }, y, rows, inv_dy2);
   const double g2 = px * px + py * py;
   const double num = pxx * py * py - 2.0 * px * py * pxy + pyy * px * px;
   return b * (g2 > 2.220446049250313e-16 ? num / g2 : 0.0);
@@ -127,11 +160,11 @@ k_ls_terms(const float* __restrict__ phi, const float* __restrict__ vn, int rows
            double b, float* __restrict__ delta, float* __restrict__ kappa) {
   const int t = blockIdx.y;
   const size_t plane = (size_t)rows * cols;
-  const float* tile = phi + t * plane;
+  const GlobalPhi tile{phi + t * plane, rows, cols};
   for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < plane; p += (size_t)gridDim.x * blockDim.x) {
     const int y = (int)(p / cols), x = (int)(p - (size_t)y * cols);
     if (delta != nullptr)
-      delta[t * plane + p] = (float)ls_normal_term(tile, rows, cols, y, x, (double)vn[t * plane + p], 1.0 / dx, 1.0 / dy);
+      delta[t * plane + p] = (float)ls_normal_term(tile, y, x, (double)vn[t * plane + p], 1.0 / dx, 1.0 / dy);
     if (kappa != nullptr) kappa[t * plane + p] = (float)ls_kappa_term(tile, rows, cols, y, x, dx, dy, b);
   }
 }
@@ -182,48 +215,59 @@ void launch_lsm_dt(const float* grad, const int* rects, int batch, int n, double
 }
 
 // ---------------------------------------------------------------- RK2 stage
+// One block per 32 x 8 pixels of one Micro: the clamped 38 x 14 neighbourhood
+// of phi_in goes to LDS once, and every stencil of the block reads it there.
+constexpr int kStageTX = 32, kStageTY = 8;
+constexpr int kStageW = kStageTX + 6, kStageH = kStageTY + 6;
+
 template <int STAGE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kStageTX * kStageTY)
 k_lsm_stage(const float* __restrict__ phi_in, const float* base, const float* __restrict__ grad,
             const int* __restrict__ rects, const double* __restrict__ dt, int n, double dx, double dy, double b,
             float* out, float* __restrict__ mask_out) {
-  // out may alias base (an in-place update): both are touched at the thread's own pixel only
-  const int t = blockIdx.y;
+  // out may alias base (an in-place update): both are touched at the thread's
+  // own pixel only; phi_in is never written by this launch
+  __shared__ float sp[kStageH * kStageW];
+  const int t = blockIdx.z;
   const size_t plane = (size_t)n * n;
   const float* tile = phi_in + t * plane;
-  const int* r = rects + 4 * t;
-  const double step = dt[t];
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < plane; p += (size_t)gridDim.x * blockDim.x) {
-    const int y = (int)(p / n), x = (int)(p - (size_t)y * n);
-    double v;
-    if (in_rect(r, y, x)) {
-      const double vn = (double)grad[t * plane + p];
-      const double normal = ls_kappa_term(tile, n, n, y, x, dx, dy, b) -
-                            ls_normal_term(tile, n, n, y, x, vn, 1.0 / dx, 1.0 / dy);
-      const double cur = (double)tile[p];
-      if (STAGE == 1)
-        v = cur + step * normal;
-      else
-        v = 0.5 * (double)base[t * plane + p] + 0.5 * (cur + step * normal);
-    } else {
-      v = STAGE == 1 ? (double)tile[p] : (double)base[t * plane + p];
-    }
-    const float vf = (float)v;
-    out[t * plane + p] = vf;
-    if (mask_out != nullptr) mask_out[t * plane + p] = vf >= 0.f ? 1.f : 0.f;
+  const int x0 = blockIdx.x * kStageTX, y0 = blockIdx.y * kStageTY;
+  for (int e = threadIdx.x; e < kStageH * kStageW; e += kStageTX * kStageTY) {
+    const int ly = e / kStageW, lx = e - ly * kStageW;
+    sp[e] = tile[(size_t)clampi(y0 + ly - 3, n - 1) * n + clampi(x0 + lx - 3, n - 1)];
   }
+  __syncthreads();
+  const int x = x0 + (int)threadIdx.x % kStageTX, y = y0 + (int)threadIdx.x / kStageTX;
+  if (x >= n || y >= n) return;
+  const size_t p = (size_t)y * n + x;
+  const LdsPhi<kStageW> phi{sp, y0, x0};
+  const int* r = rects + 4 * t;
+  double v;
+  if (in_rect(r, y, x)) {
+    const double vn = (double)grad[t * plane + p];
+    const double normal = ls_kappa_term(phi, n, n, y, x, dx, dy, b) - ls_normal_term(phi, y, x, vn, 1.0 / dx, 1.0 / dy);
+    const double cur = phi(y, x);
+    if (STAGE == 1)
+      v = cur + dt[t] * normal;
+    else
+      v = 0.5 * (double)base[t * plane + p] + 0.5 * (cur + dt[t] * normal);
+  } else {
+    v = STAGE == 1 ? phi(y, x) : (double)base[t * plane + p];
+  }
+  const float vf = (float)v;
+  out[t * plane + p] = vf;
+  if (mask_out != nullptr) mask_out[t * plane + p] = vf >= 0.f ? 1.f : 0.f;
 }
 
 void launch_lsm_stage(int stage, const float* phi_in, const float* base, const float* grad, const int* rects,
                       const double* dt, int batch, int n, double dx, double dy, double b, float* out, float* mask_out,
                       hipStream_t s) {
-  const size_t plane = (size_t)n * n;
-  const int blocks = (int)((plane + 255) / 256 < 1024 ? (plane + 255) / 256 : 1024);
+  const dim3 grid((n + kStageTX - 1) / kStageTX, (n + kStageTY - 1) / kStageTY, batch);
   if (stage == 1)
-    hipLaunchKernelGGL(k_lsm_stage<1>, dim3(blocks, batch), dim3(256), 0, s, phi_in, base, grad, rects, dt, n, dx, dy,
+    hipLaunchKernelGGL(k_lsm_stage<1>, grid, dim3(kStageTX * kStageTY), 0, s, phi_in, base, grad, rects, dt, n, dx, dy,
                        b, out, mask_out);
   else
-    hipLaunchKernelGGL(k_lsm_stage<2>, dim3(blocks, batch), dim3(256), 0, s, phi_in, base, grad, rects, dt, n, dx, dy,
+    hipLaunchKernelGGL(k_lsm_stage<2>, grid, dim3(kStageTX * kStageTY), 0, s, phi_in, base, grad, rects, dt, n, dx, dy,
                        b, out, mask_out);
 }
 

@@ -11,7 +11,7 @@ namespace litho {
 namespace {
 
 void check_tile(int batch, int rows, int cols) {
-  if (batch <= 0) throw Error(1, "level set: batch must be positive");
+  if (batch <= 0 || batch > 65535) throw Error(1, "level set: batch must be in [1, 65535]");
   if (rows < 3 || cols < 3) throw Error(1, "level set: tiles need at least 3 rows and 3 columns");
   if (cols > 4096) throw Error(1, "level set: at most 4096 columns per tile");
 }
