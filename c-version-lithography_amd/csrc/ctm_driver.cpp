@@ -50,36 +50,26 @@ void validate_ctm_options(const CtmOptions& opt) {
 
 namespace {
 
-// Host-written parameters of the slots (active flags, step coefficients, L-BFGS
-// slot lists and direction coefficients), staged in one pinned block and
-// uploaded region by region with one transfer each.  The host rewrites a
-// region only after a stream synchronisation that follows its last upload.
+// Host-written parameters of the slots (active flags, step coefficients and
+// the L-BFGS need-a-new-direction flags), staged in one pinned block and
+// uploaded with one transfer per round.  The host rewrites the block only
+// after a stream synchronisation that follows its last upload.
 struct Params {
-  size_t off_step, off_meta, off_coef, bytes;
+  size_t off_step, off_need, bytes;
   HostPinned host;
   DevMem dev;
-  static size_t layout(int cap, size_t* step, size_t* meta, size_t* coef) {
-    *step = (size_t)cap * sizeof(int);
-    *meta = *step + (size_t)cap * sizeof(float);
-    *coef = (*meta + (size_t)cap * kMetaStride * sizeof(int) + 7) / 8 * 8;
-    return *coef + (size_t)cap * kCoefStride * sizeof(double);
-  }
   explicit Params(int cap)
-      : bytes(layout(cap, &off_step, &off_meta, &off_coef)), host(bytes), dev(bytes) {}
+      : off_step((size_t)cap * sizeof(int)), off_need(off_step + (size_t)cap * sizeof(float)),
+        bytes(off_need + (size_t)cap * sizeof(int)), host(bytes), dev(bytes) {}
   int* active() const { return host.as<int>(); }
   float* step() const { return reinterpret_cast<float*>(host.as<char>() + off_step); }
-  int* meta() const { return reinterpret_cast<int*>(host.as<char>() + off_meta); }
-  double* coef() const { return reinterpret_cast<double*>(host.as<char>() + off_coef); }
+  int* need() const { return reinterpret_cast<int*>(host.as<char>() + off_need); }
   const int* d_active() const { return dev.as<int>(); }
   const float* d_step() const { return reinterpret_cast<const float*>(dev.as<char>() + off_step); }
-  const int* d_meta() const { return reinterpret_cast<const int*>(dev.as<char>() + off_meta); }
-  const double* d_coef() const { return reinterpret_cast<const double*>(dev.as<char>() + off_coef); }
-  void upload(size_t off, size_t end, hipStream_t s) const {
-    LITHO_HIP(hipMemcpyAsync(dev.as<char>() + off, host.as<char>() + off, end - off, hipMemcpyHostToDevice, s));
+  const int* d_need() const { return reinterpret_cast<const int*>(dev.as<char>() + off_need); }
+  void upload_active_step(hipStream_t s) const {
+    LITHO_HIP(hipMemcpyAsync(dev.ptr, host.ptr, bytes, hipMemcpyHostToDevice, s));
   }
-  void upload_active_step(hipStream_t s) const { upload(0, off_meta, s); }
-  void upload_meta(hipStream_t s) const { upload(off_meta, off_coef, s); }
-  void upload_meta_coef(hipStream_t s) const { upload(off_meta, bytes, s); }
 };
 
 void copy_tile(float* dst, const float* src, size_t n, hipStream_t s) {
@@ -92,47 +82,12 @@ struct LbfgsTile {
   enum Phase { INIT, BRACKET, ZOOM, NEED_PAIR, DONE };
   Phase phase = INIT;
   int k = 0, trials = 0;
+  int pairs = 0;   // pair updates formed so far: the history holds at most min(pairs, m) pairs
   double f = 0, gnorm = 0, f0 = 0, dg0 = 0, step = 0, a = 0;
   double a_prev = 0, f_prev = 0, dg_prev = 0;
   double lo_a = 0, lo_f = 0, lo_g = 0, hi_a = 0, hi_f = 0, hi_g = 0;
   bool final_is_xp = false;
-  // history: chronological slot list, dot matrices (row/col = chronological)
-  std::vector<int> slots;
-  std::vector<std::vector<double>> sy, yy;   // sy[i][j] = s_i . y_j, yy symmetric
-  int spare = 0;
 };
-
-void solve_compact(const LbfgsTile& t, const std::vector<double>& p, const std::vector<double>& q, double* coef) {
-  // d = -gamma g - S u + gamma Y w,  w = R^{-1} p,  u = R^{-T}((D + gamma Y'Y) w - gamma q)
-  const int m = (int)t.slots.size();
-  for (int i = 0; i < kCoefStride; ++i) coef[i] = 0.0;
-  if (m == 0) {
-    coef[0] = -1.0;
-    return;
-  }
-  const double gamma = t.sy[m - 1][m - 1] / t.yy[m - 1][m - 1];
-  std::vector<double> w(m), r(m), u(m);
-  for (int i = m - 1; i >= 0; --i) {
-    double acc = p[i];
-    for (int j = i + 1; j < m; ++j) acc -= t.sy[i][j] * w[j];
-    w[i] = acc / t.sy[i][i];
-  }
-  for (int i = 0; i < m; ++i) {
-    double acc = t.sy[i][i] * w[i] - gamma * q[i];
-    for (int j = 0; j < m; ++j) acc += gamma * t.yy[i][j] * w[j];
-    r[i] = acc;
-  }
-  for (int i = 0; i < m; ++i) {
-    double acc = r[i];
-    for (int j = 0; j < i; ++j) acc -= t.sy[j][i] * u[j];
-    u[i] = acc / t.sy[i][i];
-  }
-  coef[0] = -gamma;
-  for (int j = 0; j < m; ++j) {
-    coef[1 + j] = -u[j];
-    coef[1 + kMaxHist + j] = gamma * w[j];
-  }
-}
 
 }  // namespace
 
@@ -157,6 +112,7 @@ struct CtmPool::Impl {
     for (int b = 0; b < S; ++b) {
       pm.active()[b] = 0;
       pm.step()[b] = 0.f;
+      pm.need()[b] = 0;
     }
   }
   virtual ~Impl() = default;
@@ -299,18 +255,22 @@ struct GdPool final : CtmPool::Impl {
 // ------------------------------------------------------------------ L-BFGS
 struct LbfgsPool final : CtmPool::Impl {
   DevMem x, xp, g, gp, d, hs, hy, pair_part, dir_part, vec_red, dg_red;
-  HostPinned hvec, hdg;
+  // device-resident history of every slot: [m, spare, slots] ints, S'Y and
+  // Y'Y (kMaxHist^2 doubles each) and the direction coefficients
+  DevMem d_meta, d_sy, d_yy, d_coef;
+  HostPinned hdg;
   std::vector<LbfgsTile> st;
   // g . d of a new direction arrives with the evaluation of its first trial
   // (step 1): the line search starts once both are on the host
   std::vector<char> pending_dg0;
+  int mb_pair = 0;
   std::unique_ptr<EvalGraph> graph;
   const LineSearchParams lp;
   const int m_max;
 
   LbfgsPool(Plan& p, int slots, const CtmOptions& o)
-      : Impl(p, slots, o), hvec((size_t)slots * kPairDots * sizeof(double)), hdg((size_t)slots * sizeof(double)),
-        st(slots), pending_dg0(slots, 0), m_max(o.history) {
+      : Impl(p, slots, o), hdg((size_t)slots * sizeof(double)), st(slots), pending_dg0(slots, 0),
+        m_max(o.history) {
     alloc_slots(x, S, n2);
     alloc_slots(xp, S, n2);
     alloc_slots(g, S, n2);
@@ -322,10 +282,13 @@ struct LbfgsPool final : CtmPool::Impl {
     dir_part.alloc((size_t)S * kOptBlocks * sizeof(double));
     vec_red.alloc((size_t)S * kPairDots * sizeof(double));
     dg_red.alloc((size_t)S * sizeof(double));
-    int* meta = pm.meta();
-    double* coefs = pm.coef();
-    for (int i = 0; i < S * kMetaStride; ++i) meta[i] = -1;
-    for (int i = 0; i < S * kCoefStride; ++i) coefs[i] = 0.0;
+    d_meta.alloc((size_t)S * kMetaStride * sizeof(int));
+    d_sy.alloc((size_t)S * kMaxHist * kMaxHist * sizeof(double));
+    d_yy.alloc((size_t)S * kMaxHist * kMaxHist * sizeof(double));
+    d_coef.alloc((size_t)S * kCoefStride * sizeof(double));
+    LITHO_HIP(hipMemsetAsync(d_meta.ptr, 0, d_meta.bytes, s));
+    LITHO_HIP(hipMemsetAsync(d_sy.ptr, 0, d_sy.bytes, s));
+    LITHO_HIP(hipMemsetAsync(d_yy.ptr, 0, d_yy.bytes, s));
     // every evaluation reads the same buffers: theta = xp + step * d for the
     // active slots, gradient to g, g . d for the line search
     // (the masks of the trials are not kept: the result is rebuilt from theta)
@@ -344,6 +307,7 @@ struct LbfgsPool final : CtmPool::Impl {
     if (b < 0) return false;
     load_slot(b, j, xp.as<float>());
     LITHO_HIP(hipMemsetAsync(d.as<float>() + b * n2, 0, n2 * sizeof(float), s));   // first trial: x = xp + 0 * d
+    LITHO_HIP(hipMemsetAsync(d_meta.as<int>() + b * kMetaStride, 0, 2 * sizeof(int), s));   // m = 0, spare slot 0
     st[b] = LbfgsTile{};
     pending_dg0[b] = 0;
     return true;
@@ -393,123 +357,48 @@ struct LbfgsPool final : CtmPool::Impl {
     }
   }
 
-  // (A) new directions for the slots that accepted a step
-  void update_directions() {
-    int* meta = pm.meta();
-    double* coefs = pm.coef();
-    const double eps = std::numeric_limits<double>::epsilon();
-    bool need_pair = false;
-    int max_m = 0;
+  // (A) new directions for the slots that accepted a step, formed on the
+  // device after the round's upload: pair dots, history update and
+  // compact-form coefficients, the direction and g . d, without a host round
+  // trip.  g . d is copied to the host ahead of the evaluation of the first
+  // trial (step 1), and the line search starts when both have arrived.
+  bool prepare_directions() {
+    int* need = pm.need();
+    bool any = false;
+    mb_pair = 0;
     for (int b = 0; b < S; ++b) {
-      int* mb = meta + b * kMetaStride;
-      for (int i = 0; i < kMetaStride; ++i) mb[i] = 0;
-      mb[0] = -1;
-      if (running[b] && st[b].phase == LbfgsTile::NEED_PAIR) {
-        need_pair = true;
-        const LbfgsTile& t = st[b];
-        mb[0] = (int)t.slots.size();
-        mb[1] = t.spare;
-        for (size_t j = 0; j < t.slots.size(); ++j) mb[2 + j] = t.slots[j];
-        max_m = std::max(max_m, mb[0]);
-      }
-    }
-    if (!need_pair) return;
-    const int pair_nv = 4 * (max_m + 1);   // dots per slot: 4 per old pair, then the new pair
-    pm.upload_meta(s);
-    launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(), hy.as<float>(),
-                      pm.d_meta(), n2, S, max_m, true, pair_part.as<double>(), s);
-    launch_reduce_fixed(pair_part.as<double>(), kOptBlocks, pair_nv, vec_red.as<double>(), kPairDots, S, s);
-    LITHO_HIP(hipMemcpyAsync(hvec.ptr, vec_red.ptr, (size_t)S * kPairDots * sizeof(double), hipMemcpyDeviceToHost, s));
-    LITHO_HIP(hipStreamSynchronize(s));
-    const double* dots = hvec.as<double>();
-    for (int b = 0; b < S; ++b) {
-      if (!running[b] || st[b].phase != LbfgsTile::NEED_PAIR) continue;
+      need[b] = running[b] && st[b].phase == LbfgsTile::NEED_PAIR ? 1 : 0;
+      if (!need[b]) continue;
+      any = true;
       LbfgsTile& t = st[b];
-      const double* v = dots + (size_t)b * kPairDots;
-      const int m = (int)t.slots.size();
-      const double ys = v[4 * max_m + 0], yy = v[4 * max_m + 1];
-      std::vector<double> p(m), q(m);
-      for (int j = 0; j < m; ++j) {
-        p[j] = v[4 * j + 2];
-        q[j] = v[4 * j + 3];
-      }
-      if (ys > eps * yy) {          // BFGSMat::add_correction condition
-        std::vector<double> col_sy(m), col_yy(m);
-        for (int j = 0; j < m; ++j) {
-          col_sy[j] = v[4 * j + 0];
-          col_yy[j] = v[4 * j + 1];
-        }
-        const int drop = m == m_max ? 1 : 0;
-        const int freed = drop ? t.slots[0] : -1;
-        std::vector<int> ns(t.slots.begin() + drop, t.slots.end());
-        ns.push_back(t.spare);
-        const int m2 = (int)ns.size();
-        std::vector<std::vector<double>> sy2(m2, std::vector<double>(m2, 0.0)), yy2 = sy2;
-        for (int i = 0; i < m2 - 1; ++i)
-          for (int j = 0; j < m2 - 1; ++j) {
-            sy2[i][j] = t.sy[i + drop][j + drop];
-            yy2[i][j] = t.yy[i + drop][j + drop];
-          }
-        for (int i = 0; i < m2 - 1; ++i) {
-          sy2[i][m2 - 1] = col_sy[i + drop];
-          yy2[i][m2 - 1] = col_yy[i + drop];
-          yy2[m2 - 1][i] = col_yy[i + drop];
-        }
-        sy2[m2 - 1][m2 - 1] = ys;
-        yy2[m2 - 1][m2 - 1] = yy;
-        std::vector<double> p2(m2), q2(m2);
-        for (int i = 0; i < m2 - 1; ++i) {
-          p2[i] = p[i + drop];
-          q2[i] = q[i + drop];
-        }
-        p2[m2 - 1] = v[4 * max_m + 2];
-        q2[m2 - 1] = v[4 * max_m + 3];
-        // spare slot for the next pair: the freed slot or an unused index
-        if (freed >= 0) {
-          t.spare = freed;
-        } else {
-          int cand = 0;
-          while (std::find(ns.begin(), ns.end(), cand) != ns.end()) ++cand;
-          t.spare = cand;
-        }
-        t.slots = ns;
-        t.sy = sy2;
-        t.yy = yy2;
-        p = p2;
-        q = q2;
-      }
-      solve_compact(t, p, q, coefs + (size_t)b * kCoefStride);
-    }
-    // rebuild meta with the committed slot lists for the direction kernel
-    int dir_m = 0;
-    for (int b = 0; b < S; ++b) {
-      int* mb = meta + b * kMetaStride;
-      if (!running[b] || st[b].phase != LbfgsTile::NEED_PAIR) continue;
-      mb[0] = (int)st[b].slots.size();
-      mb[1] = st[b].spare;
-      for (size_t j = 0; j < st[b].slots.size(); ++j) mb[2 + j] = st[b].slots[j];
-      dir_m = std::max(dir_m, mb[0]);
-    }
-    pm.upload_meta_coef(s);
-    launch_lbfgs_direction(g.as<float>(), hs.as<float>(), hy.as<float>(), pm.d_meta(), pm.d_coef(), n2, d.as<float>(),
-                           S, dir_m, dir_part.as<double>(), s);
-    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, dg_red.as<double>(), 1, S, s);
-    LITHO_HIP(hipMemcpyAsync(hdg.ptr, dg_red.ptr, (size_t)S * sizeof(double), hipMemcpyDeviceToHost, s));
-    // No synchronisation here: the first trial (step 1) is evaluated right
-    // away and the line search starts when g . d arrives with its result.
-    for (int b = 0; b < S; ++b) {
-      if (!running[b] || st[b].phase != LbfgsTile::NEED_PAIR) continue;
-      LbfgsTile& t = st[b];
+      mb_pair = std::max(mb_pair, std::min(t.pairs, m_max));
+      t.pairs += 1;
       t.step = 1.0;
       t.k += 1;
       t.a = t.step;
       t.phase = LbfgsTile::BRACKET;
       pending_dg0[b] = 1;
     }
+    return any;
+  }
+  // mb_pair bounds the history length before the update over the slots of
+  // the round (it selects the instantiation); one more pair after it
+  void launch_directions() {
+    const int mb_dir = std::min(mb_pair + 1, m_max);
+    launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(), hy.as<float>(),
+                      pm.d_need(), d_meta.as<int>(), n2, S, mb_pair, true, pair_part.as<double>(), s);
+    launch_reduce_fixed(pair_part.as<double>(), kOptBlocks, 4 * (mb_pair + 1), vec_red.as<double>(), kPairDots, S,
+                        s);
+    launch_lbfgs_update(S, pm.d_need(), vec_red.as<double>(), mb_pair, m_max, d_meta.as<int>(), d_sy.as<double>(),
+                        d_yy.as<double>(), d_coef.as<double>(), s);
+    launch_lbfgs_direction(g.as<float>(), hs.as<float>(), hy.as<float>(), pm.d_need(), d_meta.as<int>(),
+                           d_coef.as<double>(), n2, d.as<float>(), S, mb_dir, dir_part.as<double>(), s);
+    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, dg_red.as<double>(), 1, S, s);
+    LITHO_HIP(hipMemcpyAsync(hdg.ptr, dg_red.ptr, (size_t)S * sizeof(double), hipMemcpyDeviceToHost, s));
   }
 
   // (B) one batched evaluation of every slot with a pending trial
-  void evaluate() {
+  void evaluate(bool directions) {
     int* flags = pm.active();
     float* stepv = pm.step();
     bool any = false;
@@ -521,8 +410,9 @@ struct LbfgsPool final : CtmPool::Impl {
       stepv[b] = (ev && t.phase != LbfgsTile::INIT) ? (float)t.a : 0.f;
       any = any || ev;
     }
-    if (!any) return;
+    if (!any) return;   // a new direction always comes with its first trial
     pm.upload_active_step(s);
+    if (directions) launch_directions();
     graph->launch();
     const double* red = fetch_reductions(plan, S);
     std::vector<int> started;
@@ -620,8 +510,7 @@ struct LbfgsPool final : CtmPool::Impl {
 
   void step() override {
     if (busy() == 0) return;
-    update_directions();
-    evaluate();
+    evaluate(prepare_directions());
     // write out the slots that finished: x, or xp after a failed line search
     for (int b = 0; b < S; ++b) {
       if (!running[b] || st[b].phase != LbfgsTile::DONE) continue;

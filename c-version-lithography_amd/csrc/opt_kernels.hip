@@ -100,12 +100,12 @@ __device__ __forceinline__ float4 sub4(float4 a, float4 b) { return make_float4(
 template <int MB>
 __global__ void __launch_bounds__(kOptThreads)
 k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* __restrict__ g,
-             float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy, const int* __restrict__ meta,
-             size_t n, int shift, double* __restrict__ partials) {
+             float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy, const int* __restrict__ need,
+             const int* __restrict__ meta, size_t n, int shift, double* __restrict__ partials) {
   constexpr int NV = 4 * (MB + 1);
   const int b = blockIdx.y;
+  if (need[b] == 0) return;
   const int m = meta[b * kMetaStride + 0];
-  if (m < 0) return;
   const int new_slot = meta[b * kMetaStride + 1];
   const int* slots = meta + b * kMetaStride + 2;
   const size_t tb = (size_t)b * n;
@@ -164,11 +164,11 @@ k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* _
 template <int MB>
 __global__ void __launch_bounds__(kOptThreads)
 k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, const float* __restrict__ hy,
-                  const int* __restrict__ meta, const double* __restrict__ coef, size_t n, float* __restrict__ d,
-                  double* __restrict__ partials) {
+                  const int* __restrict__ need, const int* __restrict__ meta, const double* __restrict__ coef,
+                  size_t n, float* __restrict__ d, double* __restrict__ partials) {
   const int b = blockIdx.y;
+  if (need[b] == 0) return;
   const int m = meta[b * kMetaStride + 0];
-  if (m < 0) return;
   const int* slots = meta + b * kMetaStride + 2;
   const double* cb = coef + (size_t)b * kCoefStride;
   const size_t tb = (size_t)b * n;
@@ -210,22 +210,165 @@ k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, con
   block_reduce_store<1>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x));
 }
 
+// One 64-thread block per tile.  Restates LBFGS++'s BFGSMat::add_correction
+// (accept when s.y > eps y.y; the oldest pair leaves once m_max are held) on
+// the compact representation, then the Byrd-Nocedal-Schnabel direction
+//   d = -gamma g - S u + gamma Y w,  w = R^{-1} p,
+//   u = R^{-T}((D + gamma Y'Y) w - gamma q),  gamma = s.y / y.y of the newest pair,
+// with R the upper triangle of S'Y, D its diagonal, p = S'g, q = Y'g.  The
+// matrices are staged in LDS; the two triangular solves run on one lane with
+// register arrays (loops unrolled over kMaxHist).
+__global__ void __launch_bounds__(64)
+k_lbfgs_update(const int* __restrict__ need, const double* __restrict__ dots, int mb, int m_max,
+               int* __restrict__ meta, double* __restrict__ sy, double* __restrict__ yy, double* __restrict__ coef) {
+  constexpr int H = kMaxHist;
+  const int b = blockIdx.x;
+  if (need[b] == 0) return;
+  __shared__ double s0[H * H], y0[H * H], s1[H * H], y1[H * H];
+  __shared__ double v[kPairDots];
+  __shared__ int mt[kMetaStride];
+  const int tid = threadIdx.x;
+  double* sy_b = sy + (size_t)b * H * H;
+  double* yy_b = yy + (size_t)b * H * H;
+  for (int e = tid; e < H * H; e += 64) {
+    s0[e] = sy_b[e];
+    y0[e] = yy_b[e];
+  }
+  for (int e = tid; e < 4 * (mb + 1); e += 64) v[e] = dots[(size_t)b * kPairDots + e];
+  if (tid < kMetaStride) mt[tid] = meta[b * kMetaStride + tid];
+  __syncthreads();
+  const int m = mt[0];
+  const double ys = v[4 * mb + 0], yy_new = v[4 * mb + 1];
+  const bool accept = ys > 2.220446049250313e-16 * yy_new;
+  const int drop = accept && m == m_max ? 1 : 0;
+  const int m2 = accept ? m - drop + 1 : m;
+  // the updated S'Y (upper triangle) and Y'Y, chronological order
+  for (int e = tid; e < H * H; e += 64) {
+    const int i = e / H, j = e - i * H;
+    double a = s0[e], c = y0[e];
+    if (accept) {
+      a = 0.0;
+      c = 0.0;
+      if (i < m2 - 1 && j < m2 - 1) {
+        a = s0[(i + drop) * H + j + drop];
+        c = y0[(i + drop) * H + j + drop];
+      } else if (i < m2 - 1 && j == m2 - 1) {
+        a = v[4 * (i + drop) + 0];
+        c = v[4 * (i + drop) + 1];
+      } else if (i == m2 - 1 && j < m2 - 1) {
+        c = v[4 * (j + drop) + 1];
+      } else if (i == m2 - 1 && j == m2 - 1) {
+        a = ys;
+        c = yy_new;
+      }
+    }
+    s1[e] = a;
+    y1[e] = c;
+    sy_b[e] = a;
+    yy_b[e] = c;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  int* mt_g = meta + b * kMetaStride;
+  if (accept) {
+    const int freed = drop ? mt[2] : -1;
+    int slots[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) slots[i] = (i < m2 - 1) ? mt[2 + i + drop] : (i == m2 - 1 ? mt[1] : -1);
+    int spare = freed;
+    if (spare < 0) {   // the smallest slot index not in the list
+      spare = 0;
+      for (bool used = true; used;) {
+        used = false;
+#pragma unroll
+        for (int i = 0; i < H; ++i) used = used || slots[i] == spare;
+        if (used) ++spare;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+      if (i < m2) mt_g[2 + i] = slots[i];
+    mt_g[0] = m2;
+    mt_g[1] = spare;
+  }
+  double* cf = coef + (size_t)b * kCoefStride;
+  for (int i = 0; i < kCoefStride; ++i) cf[i] = 0.0;
+  if (m2 == 0) {
+    cf[0] = -1.0;
+    return;
+  }
+  double p[H], q[H], w[H], u[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const int src = accept ? (j < m2 - 1 ? j + drop : mb) : j;
+    p[j] = j < m2 ? v[4 * src + 2] : 0.0;
+    q[j] = j < m2 ? v[4 * src + 3] : 0.0;
+    w[j] = 0.0;
+    u[j] = 0.0;
+  }
+  const double gamma = s1[(m2 - 1) * H + m2 - 1] / y1[(m2 - 1) * H + m2 - 1];
+#pragma unroll
+  for (int i = H - 1; i >= 0; --i) {
+    if (i < m2) {
+      double acc = p[i];
+#pragma unroll
+      for (int j = i + 1; j < H; ++j)
+        if (j < m2) acc -= s1[i * H + j] * w[j];
+      w[i] = acc / s1[i * H + i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    if (i < m2) {
+      double acc = s1[i * H + i] * w[i] - gamma * q[i];
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        if (j < m2) acc += gamma * y1[i * H + j] * w[j];
+      u[i] = acc;   // r_i; replaced by u_i in increasing i below
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    if (i < m2) {
+      double acc = u[i];
+#pragma unroll
+      for (int j = 0; j < i; ++j) acc -= s1[j * H + i] * u[j];
+      u[i] = acc / s1[i * H + i];
+    }
+  }
+  cf[0] = -gamma;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j < m2) {
+      cf[1 + j] = -u[j];
+      cf[1 + kMaxHist + j] = gamma * w[j];
+    }
+  }
+}
+
+void launch_lbfgs_update(int batch, const int* need, const double* dots, int mb, int m_max, int* meta, double* sy,
+                         double* yy, double* coef, hipStream_t s) {
+  hipLaunchKernelGGL(k_lbfgs_update, dim3(batch), dim3(64), 0, s, need, dots, mb, m_max, meta, sy, yy, coef);
+}
+
 namespace {
 
 using PairLaunch = void (*)(dim3, hipStream_t, const float*, float*, const float*, float*, float*, float*, const int*,
-                            size_t, int, double*);
-using DirLaunch = void (*)(dim3, hipStream_t, const float*, const float*, const float*, const int*, const double*,
-                           size_t, float*, double*);
+                            const int*, size_t, int, double*);
+using DirLaunch = void (*)(dim3, hipStream_t, const float*, const float*, const float*, const int*, const int*,
+                           const double*, size_t, float*, double*);
 
 template <int MB>
 void pair_launch(dim3 grid, hipStream_t s, const float* x, float* xp, const float* g, float* gp, float* hs, float* hy,
-                 const int* meta, size_t n, int shift, double* partials) {
-  hipLaunchKernelGGL(k_lbfgs_pair<MB>, grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, meta, n, shift, partials);
+                 const int* need, const int* meta, size_t n, int shift, double* partials) {
+  hipLaunchKernelGGL(k_lbfgs_pair<MB>, grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, need, meta, n, shift,
+                     partials);
 }
 template <int MB>
-void dir_launch(dim3 grid, hipStream_t s, const float* g, const float* hs, const float* hy, const int* meta,
-                const double* coef, size_t n, float* d, double* partials) {
-  hipLaunchKernelGGL(k_lbfgs_direction<MB>, grid, dim3(kOptThreads), 0, s, g, hs, hy, meta, coef, n, d, partials);
+void dir_launch(dim3 grid, hipStream_t s, const float* g, const float* hs, const float* hy, const int* need,
+                const int* meta, const double* coef, size_t n, float* d, double* partials) {
+  hipLaunchKernelGGL(k_lbfgs_direction<MB>, grid, dim3(kOptThreads), 0, s, g, hs, hy, need, meta, coef, n, d,
+                     partials);
 }
 template <int... I>
 constexpr PairLaunch pair_table_entry(int m, std::integer_sequence<int, I...>) {
@@ -240,16 +383,17 @@ constexpr DirLaunch dir_table_entry(int m, std::integer_sequence<int, I...>) {
 
 }  // namespace
 
-void launch_lbfgs_pair(const float* x, float* xp, const float* g, float* gp, float* hs, float* hy, const int* meta,
-                       size_t n, int batch, int max_m, bool shift, double* partials, hipStream_t s) {
-  pair_table_entry(max_m, std::make_integer_sequence<int, kMaxHist + 1>{})(
-      dim3(kOptBlocks, batch), s, x, xp, g, gp, hs, hy, meta, n, shift ? 1 : 0, partials);
+void launch_lbfgs_pair(const float* x, float* xp, const float* g, float* gp, float* hs, float* hy, const int* need,
+                       const int* meta, size_t n, int batch, int mb, bool shift, double* partials, hipStream_t s) {
+  pair_table_entry(mb, std::make_integer_sequence<int, kMaxHist + 1>{})(
+      dim3(kOptBlocks, batch), s, x, xp, g, gp, hs, hy, need, meta, n, shift ? 1 : 0, partials);
 }
 
-void launch_lbfgs_direction(const float* g, const float* hs, const float* hy, const int* meta, const double* coef,
-                            size_t n, float* d, int batch, int max_m, double* partials, hipStream_t s) {
-  dir_table_entry(max_m, std::make_integer_sequence<int, kMaxHist + 1>{})(dim3(kOptBlocks, batch), s, g, hs, hy,
-                                                                          meta, coef, n, d, partials);
+void launch_lbfgs_direction(const float* g, const float* hs, const float* hy, const int* need, const int* meta,
+                            const double* coef, size_t n, float* d, int batch, int mb, double* partials,
+                            hipStream_t s) {
+  dir_table_entry(mb, std::make_integer_sequence<int, kMaxHist + 1>{})(dim3(kOptBlocks, batch), s, g, hs, hy, need,
+                                                                       meta, coef, n, d, partials);
 }
 
 // One block per (v, tile): thread t sums blocks t, t + 256, ... in order, then a
