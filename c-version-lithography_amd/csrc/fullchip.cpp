@@ -338,6 +338,8 @@ FullChipResult fullchip_lsm(Plan& plan, const litho_tile* tiles, int T, int heig
   if (iterations < 0) throw Error(1, "FullChipLSMOptions: iterations must be non-negative");
   if (reinit_interval < 0) throw Error(1, "FullChipLSMOptions: reinit_interval must be non-negative");
   if (!(lo.dx > 0.0) || !(lo.dy > 0.0)) throw Error(1, "FullChipLSMOptions: dx/dy must be positive");
+  if (lo.mask_type != 0 && lo.mask_type != 1) throw Error(1, "FullChipLSMOptions: mask_type must be Binary or Gray");
+  if (lo.mask_type == 1 && lo.render_scale <= 0) throw Error(1, "FullChipLSMOptions: render_scale must be positive");
   litho_exchange_options o = options;
   o.loss_region = 1;                            // Core + Guard (full_chip_lsm.h:22)
   o.cept_guard_only = 1;
@@ -363,6 +365,7 @@ FullChipResult fullchip_lsm(Plan& plan, const litho_tile* tiles, int T, int heig
     uint32_t* scratch = reinterpret_cast<uint32_t*>(ws);
     float* dist = reinterpret_cast<float*>(ws + (size_t)T * 2 * plane * sizeof(uint32_t));
     launch_reinit_sdf(b.grad.as<float>(), T, n, n, lo.dx, lo.dy, 0.5f, false, scratch, dist, nullptr, phi, masks, s);
+    if (lo.mask_type == 1) launch_render_mask(phi, T, n, n, 1, lo.render_scale, masks, s);
   }
   launch_extract(global_target, height, width, ex.table(), T, n, b.targets.as<float>(), s);
   ex.prepare(b.targets.as<float>(), o);

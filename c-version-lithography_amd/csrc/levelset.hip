@@ -271,6 +271,75 @@ void launch_lsm_stage(int stage, const float* phi_in, const float* base, const f
                        b, out, mask_out);
 }
 
+// ---------------------------------------------------------------- render
+// render_levelset_mask (levelset_mask_renderer.cpp:7-33).  Binary: phi >= 0.
+// Gray: the float32 phi upscaled by `scale` with pixel-centre bilinear
+// interpolation and replicated borders (cv::resize INTER_LINEAR: source
+// coordinate f = (d + 0.5) / scale - 0.5 in FP64 rounded to float32, floor
+// and fraction, clamped at both ends with a zero fraction), each output row
+// interpolated along x first and the two rows then along y, every product and
+// sum rounded separately in float32 (no contraction); the subpixels >= 0 of
+// each pixel's scale x scale block are counted and divided by scale^2.
+namespace {
+struct LinearTap {
+  int s0, s1;
+  float w0, w1;
+};
+__device__ __forceinline__ LinearTap linear_tap(int d, int n_src, double inv_scale) {
+  float f = (float)__dadd_rn(__dmul_rn((double)d + 0.5, inv_scale), -0.5);
+  int s = (int)floorf(f);
+  f = __fsub_rn(f, (float)s);
+  if (s < 0) {
+    s = 0;
+    f = 0.f;
+  }
+  if (s >= n_src - 1) {
+    s = n_src - 1;
+    f = 0.f;
+  }
+  return {s, s + 1 < n_src ? s + 1 : n_src - 1, __fsub_rn(1.f, f), f};
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256)
+k_render_mask(const float* __restrict__ phi, int rows, int cols, int type, int scale, float* __restrict__ mask) {
+  const int t = blockIdx.y;
+  const size_t plane = (size_t)rows * cols;
+  const float* tile = phi + t * plane;
+  const double inv_scale = 1.0 / (double)scale;
+  const double count = (double)scale * (double)scale;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < plane; p += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(p / cols), x = (int)(p - (size_t)y * cols);
+    float v;
+    if (type == 0) {
+      v = tile[p] >= 0.f ? 1.f : 0.f;
+    } else {
+      int inside = 0;
+      for (int sy = 0; sy < scale; ++sy) {
+        const LinearTap ty = linear_tap(y * scale + sy, rows, inv_scale);
+        const float* r0 = tile + (size_t)ty.s0 * cols;
+        const float* r1 = tile + (size_t)ty.s1 * cols;
+        for (int sx = 0; sx < scale; ++sx) {
+          const LinearTap tx = linear_tap(x * scale + sx, cols, inv_scale);
+          const float h0 = __fadd_rn(__fmul_rn(r0[tx.s0], tx.w0), __fmul_rn(r0[tx.s1], tx.w1));
+          const float h1 = __fadd_rn(__fmul_rn(r1[tx.s0], tx.w0), __fmul_rn(r1[tx.s1], tx.w1));
+          const float val = __fadd_rn(__fmul_rn(h0, ty.w0), __fmul_rn(h1, ty.w1));
+          inside += val >= 0.f ? 1 : 0;
+        }
+      }
+      v = (float)((double)inside / count);
+    }
+    mask[t * plane + p] = v;
+  }
+}
+
+void launch_render_mask(const float* phi, int batch, int rows, int cols, int type, int scale, float* mask,
+                        hipStream_t s) {
+  const size_t plane = (size_t)rows * cols;
+  const int blocks = (int)((plane + 255) / 256 < 1024 ? (plane + 255) / 256 : 1024);
+  hipLaunchKernelGGL(k_render_mask, dim3(blocks, batch), dim3(256), 0, s, phi, rows, cols, type, scale, mask);
+}
+
 // ---------------------------------------------------------------- chamfer SDF
 // OpenCV DIST_L2 / mask 5 semantics in 16-bit fixed point: horizontal and
 // vertical step 1, diagonal 1.4, knight move 2.1969 (round(w * 65536) of the

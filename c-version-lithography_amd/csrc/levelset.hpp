@@ -33,6 +33,12 @@ void launch_lsm_stage(int stage, const float* phi_in, const float* base, const f
                       const double* dt, int batch, int n, double dx, double dy, double b, float* out, float* mask_out,
                       hipStream_t s);
 
+// render_levelset_mask (levelset_mask_renderer.cpp:7-33) of batch rows x cols
+// tiles: type 0 Binary (phi >= 0), type 1 Gray (bilinear upscale by scale,
+// zero threshold, mean over each pixel's scale x scale block).
+void launch_render_mask(const float* phi, int batch, int rows, int cols, int type, int scale, float* mask,
+                        hipStream_t s);
+
 // Chamfer signed distance (level_set_utils.cpp:290-319 over OpenCV's
 // distanceTransform(DIST_L2, 5)): inside = phi > level (or phi >= level when
 // inclusive), sdf = dist_in dx - dist_out dx with the 5x5 chamfer distances in

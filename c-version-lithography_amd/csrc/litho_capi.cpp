@@ -82,6 +82,16 @@ litho::CtmOptions ctm_options(const litho_ctm_options& o) {
   c.alpha = o.alpha;
   return c;
 }
+litho::LsmOptions lsm_options(const litho_lsm_options& o) {
+  litho::LsmOptions l;
+  l.dx = o.dx;
+  l.dy = o.dy;
+  l.cfl = o.cfl;
+  l.curvature_weight = o.curvature_weight;
+  l.mask_type = o.mask_type;
+  l.render_scale = o.render_scale;
+  return l;
+}
 void fill_info(const litho::CtmTileResult& r, litho_ctm_info* info) {
   info->evaluations = r.evaluations;
   info->iterations = r.iterations;
@@ -462,17 +472,22 @@ int litho_ls_reinit_sdf(litho_ctx* ctx, int batch, int rows, int cols, const flo
   });
 }
 
+int litho_render_levelset_mask(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, int type, int scale,
+                               float* mask_out) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    litho::render_levelset_mask(c, batch, rows, cols, phi, type, scale, mask_out);
+    LITHO_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
 int litho_lsm_step(litho_ctx* ctx, int batch, int n, const float* phi, const float* mask_gradient,
                    const int* update_rects, const litho_lsm_options* options, int reinit, float* phi_out,
                    float* mask_out, double* dt_out) {
   return guarded([&] {
     require(options != nullptr, "null options");
     litho::Context& c = ctx_of(ctx);
-    litho::LsmOptions o;
-    o.dx = options->dx;
-    o.dy = options->dy;
-    o.cfl = options->cfl;
-    o.curvature_weight = options->curvature_weight;
+    const litho::LsmOptions o = lsm_options(*options);
     litho::lsm_step(c, batch, n, phi, mask_gradient, update_rects, o, reinit != 0, phi_out, mask_out, dt_out);
     LITHO_HIP(hipStreamSynchronize(c.stream));
   });
@@ -560,11 +575,7 @@ int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
     litho::Plan& p = plan_of(plan);
     require(options != nullptr && lsm != nullptr, "null options");
     set_device(p);
-    litho::LsmOptions lo;
-    lo.dx = lsm->dx;
-    lo.dy = lsm->dy;
-    lo.cfl = lsm->cfl;
-    lo.curvature_weight = lsm->curvature_weight;
+    const litho::LsmOptions lo = lsm_options(*lsm);
     const litho::FullChipResult r =
         litho::fullchip_lsm(p, tiles, num_tiles, height, width, initial_binary, global_target, *options, lo,
                             iterations, reinit_interval, global_mask_out, global_phi_out, history);

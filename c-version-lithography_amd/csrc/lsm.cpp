@@ -18,7 +18,22 @@ void check_tile(int batch, int rows, int cols) {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+void check_render(int type, int scale) {
+  if (type != 0 && type != 1) throw Error(1, "render_levelset_mask: type must be Binary (0) or Gray (1)");
+  if (type == 1 && scale <= 0) throw Error(1, "render_levelset_mask: render_scale must be positive");
+}
+
 }  // namespace
+
+void render_levelset_mask(Context& ctx, int batch, int rows, int cols, const float* phi, int type, int scale,
+                          float* mask) {
+  if (batch <= 0 || batch > 65535 || rows <= 0 || cols <= 0)
+    throw Error(1, "render_levelset_mask: empty tile or batch out of [1, 65535]");
+  if (phi == nullptr || mask == nullptr) throw Error(1, "render_levelset_mask: null buffer");
+  check_render(type, scale);
+  launch_render_mask(phi, batch, rows, cols, type, scale, mask, ctx.stream);
+  LITHO_HIP(hipGetLastError());
+}
 
 void ls_terms(Context& ctx, int batch, int rows, int cols, const float* phi, const float* vn, double dx, double dy,
               double b, float* delta, float* kappa) {
@@ -53,6 +68,7 @@ void lsm_step(Context& ctx, int batch, int n, const float* phi, const float* gra
   if (phi == nullptr || grad == nullptr || rects == nullptr || phi_out == nullptr)
     throw Error(1, "lsm_step: null buffer");
   if (!(o.dx > 0.0) || !(o.dy > 0.0)) throw Error(1, "lsm_step: dx/dy must be positive");
+  check_render(o.mask_type, o.render_scale);
   for (int b = 0; b < batch; ++b) {
     const int* r = rects + 4 * b;
     if (r[0] < 0 || r[1] > n || r[2] < 0 || r[3] > n || r[0] > r[1] || r[2] > r[3])
@@ -74,14 +90,19 @@ void lsm_step(Context& ctx, int batch, int n, const float* phi, const float* gra
   LITHO_HIP(hipMemcpyAsync(d_rects, rects, (size_t)batch * 4 * sizeof(int), hipMemcpyHostToDevice, s));
   launch_lsm_dt(grad, d_rects, batch, n, o.dx, o.dy, o.cfl, o.curvature_weight, part, d_dt, s);
   launch_lsm_stage(1, phi, phi, grad, d_rects, d_dt, batch, n, o.dx, o.dy, o.curvature_weight, phi1, nullptr, s);
+  // the Binary render comes with the last kernel that writes phi_out; the
+  // Gray render reads the finished phi_out
+  float* binary_mask = o.mask_type == 0 ? mask_out : nullptr;
   launch_lsm_stage(2, phi1, phi, grad, d_rects, d_dt, batch, n, o.dx, o.dy, o.curvature_weight, phi_out,
-                   reinit ? nullptr : mask_out, s);
+                   reinit ? nullptr : binary_mask, s);
   if (reinit) {
     uint32_t* scratch = reinterpret_cast<uint32_t*>(ws + off_ch);
     float* dist = reinterpret_cast<float*>(ws + off_ch + (size_t)batch * 2 * plane * sizeof(uint32_t));
     // signs = (phi >= 0) - 0.5: inside means phi_out >= 0 (full_chip_lsm.cpp:236-239)
-    launch_reinit_sdf(phi_out, batch, n, n, o.dx, o.dy, 0.f, true, scratch, dist, d_rects, phi_out, mask_out, s);
+    launch_reinit_sdf(phi_out, batch, n, n, o.dx, o.dy, 0.f, true, scratch, dist, d_rects, phi_out, binary_mask, s);
   }
+  if (o.mask_type == 1 && mask_out != nullptr)
+    launch_render_mask(phi_out, batch, n, n, 1, o.render_scale, mask_out, s);
   LITHO_HIP(hipGetLastError());
   if (dt_out != nullptr) {
     LITHO_HIP(hipMemcpyAsync(dt_out, d_dt, (size_t)batch * sizeof(double), hipMemcpyDeviceToHost, s));

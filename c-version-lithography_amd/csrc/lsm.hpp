@@ -10,7 +10,14 @@ namespace litho {
 
 struct LsmOptions {
   double dx = 1.0, dy = 1.0, cfl = 0.7, curvature_weight = 0.01;
+  int mask_type = 0;      // render of the masks: 0 Binary, 1 Gray (full_chip_lsm.h:14-15)
+  int render_scale = 4;   // Gray: upscale per direction
 };
+
+// render_levelset_mask(phi, type, scale) (levelset_mask_renderer.cpp:7-33) of
+// batch rows x cols tiles on the device.
+void render_levelset_mask(Context& ctx, int batch, int rows, int cols, const float* phi, int type, int scale,
+                          float* mask);
 
 void ls_terms(Context& ctx, int batch, int rows, int cols, const float* phi, const float* vn, double dx, double dy,
               double b, float* delta, float* kappa);
@@ -20,7 +27,7 @@ void ls_reinit_sdf(Context& ctx, int batch, int rows, int cols, const float* phi
 // One RK2 update (full_chip_lsm.cpp:215-241) of `batch` n x n tiles; rects: host
 // [batch][4] update windows {y0, y1, x0, x1}.  phi_out may alias phi.  With
 // reinit the update window is re-initialised to the chamfer SDF of the result.
-// mask_out (nullable) receives the Binary render (phi_out >= 0).  dt_out (host,
+// mask_out (nullable) receives the render of o.mask_type.  dt_out (host,
 // nullable) receives each tile's step; passing it synchronises the stream.
 void lsm_step(Context& ctx, int batch, int n, const float* phi, const float* grad, const int* rects,
               const LsmOptions& o, bool reinit, float* phi_out, float* mask_out, double* dt_out);

@@ -32,8 +32,15 @@ class CtmOptions(C.Structure):
                 ("threshold", C.c_double), ("alpha", C.c_double)]
 
 
+_MASK_TYPES = {"Binary": 0, "Gray": 1}
+
+
 class LsmOptions(C.Structure):
-    _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("cfl", C.c_double), ("curvature_weight", C.c_double)]
+    _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("cfl", C.c_double), ("curvature_weight", C.c_double),
+                ("mask_type", C.c_int), ("render_scale", C.c_int)]
+
+    def __init__(self, dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01, mask_type=0, render_scale=4):
+        super().__init__(dx, dy, cfl, curvature_weight, mask_type, render_scale)
 
 
 class CtmInfo(C.Structure):
@@ -134,6 +141,7 @@ SIGNATURES = {
     "litho_ls_terms": (_I, [_P, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P]),
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
     "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
+    "litho_render_levelset_mask": (_I, [_P, _I, _I, _I, _P, _I, _I, _P]),
     "litho_lsm_step": (_I, [_P, _I, _I, _P, _P, _IP, C.POINTER(LsmOptions), _I, _P, _P, _DP]),
     "litho_partition_build": (_I, [_I, _I, _I, _I, _I, _I, _IP, C.POINTER(Tile)]),
     "litho_partition_extract": (_I, [_P, _P, _I, _I, C.POINTER(Tile), _I, _I, _P]),
@@ -235,15 +243,21 @@ class Context:
         b, r, c = _shape3(phi)
         check(self.lib.litho_ls_reinit_sdf(self.h, b, r, c, ptr(phi), dx, dy, ptr(sdf)))
 
+    def render_levelset_mask(self, phi, mask_out, mask_type="Binary", scale=4):
+        """render_levelset_mask of (batch, rows, cols) tiles; mask_type "Binary" or "Gray"."""
+        b, r, c = _shape3(phi)
+        check(self.lib.litho_render_levelset_mask(self.h, b, r, c, ptr(phi), _MASK_TYPES[mask_type], scale,
+                                                  ptr(mask_out)))
+
     def lsm_step(self, phi, grad, rects, phi_out, mask_out=None, dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01,
-                 reinit=False):
+                 reinit=False, mask_type="Binary", render_scale=4):
         """One RK2 update of (batch, n, n) tiles; rects: (batch, 4) y0, y1, x0, x1.  Returns dt per tile."""
         b, r, c = _shape3(phi)
         if r != c:
             raise ValueError("lsm_step tiles are square")
         rc = np.ascontiguousarray(rects, dtype=np.int32).reshape(b, 4)
         dt = np.zeros(b)
-        o = LsmOptions(dx, dy, cfl, curvature_weight)
+        o = LsmOptions(dx, dy, cfl, curvature_weight, _MASK_TYPES[mask_type], render_scale)
         check(self.lib.litho_lsm_step(self.h, b, r, ptr(phi), ptr(grad), rc.ctypes.data_as(_IP), C.byref(o),
                                       1 if reinit else 0, ptr(phi_out), ptr(mask_out), _dp(dt)))
         return dt
@@ -420,11 +434,12 @@ class Plan:
         return dict(best_iteration=bi.value, best_loss=bl.value, history=hist)
 
     def fullchip_lsm(self, tiles, initial_binary, target, options, iterations, global_mask_out, global_phi_out=None,
-                     dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01, reinit_interval=20):
+                     dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01, reinit_interval=20, mask_type="Binary",
+                     render_scale=4):
         h, w = target.shape
         hist = np.zeros((iterations + 1, 5))
         bi, bl = C.c_int(), C.c_double()
-        lo = LsmOptions(dx, dy, cfl, curvature_weight)
+        lo = LsmOptions(dx, dy, cfl, curvature_weight, _MASK_TYPES[mask_type], render_scale)
         check(self.lib.litho_fullchip_lsm(self.h, tiles, len(tiles), h, w, ptr(initial_binary), ptr(target),
                                           C.byref(options), C.byref(lo), iterations, reinit_interval,
                                           ptr(global_mask_out), ptr(global_phi_out), _dp(hist), C.byref(bi),

@@ -245,7 +245,17 @@ typedef struct {
   double dx, dy;             /* FullChipLSMOptions dx / dy (full_chip_lsm.h:8-66) */
   double cfl;                /* CFL number */
   double curvature_weight;   /* b */
+  int mask_type;             /* render of the masks: 0 Binary, 1 Gray (full_chip_lsm.h:14) */
+  int render_scale;          /* Gray: upscale per direction (full_chip_lsm.h:15, default 4) */
 } litho_lsm_options;
+
+/* render_levelset_mask(phi, type, scale) (levelset_mask_renderer.cpp:7-33) of
+ * `batch` rows x cols tiles: type 0 Binary (phi >= 0); type 1 Gray: float32
+ * phi upscaled by `scale` with cv::resize INTER_LINEAR semantics (pixel-centre
+ * bilinear, replicated borders), subpixels >= 0 counted per pixel and divided
+ * by scale^2.  phi, mask_out: device batch x rows x cols. */
+int litho_render_levelset_mask(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, int type, int scale,
+                               float* mask_out);
 
 /* One RK2 level-set update of `batch` n x n Micros, the loop body of
  * FullChipLSMOptimizer::optimize (full_chip_lsm.cpp:208-242):
@@ -254,7 +264,7 @@ typedef struct {
  *   phi1 = phi + dt N(phi) (window);  phi' = phi/2 + (phi1 + dt N(phi1))/2 (window);
  *   with reinit != 0 the window is re-initialised to reinit_sdf((phi' >= 0) - 0.5).
  * update_rects: HOST array of batch x {y0, y1, x0, x1} (half-open, tile-local).
- * phi_out may alias phi.  mask_out (nullable): Binary render, phi_out >= 0.
+ * phi_out may alias phi.  mask_out (nullable): render of options->mask_type.
  * dt_out (HOST, nullable): the step of each tile; passing it waits for the step. */
 int litho_lsm_step(litho_ctx* ctx, int batch, int n, const float* phi, const float* mask_gradient,
                    const int* update_rects, const litho_lsm_options* options, int reinit, float* phi_out,
@@ -352,8 +362,8 @@ int litho_fullchip_ctm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
                        int iterations, double learning_rate, float* global_mask_out, float* local_masks_out,
                        double* history, int* best_iteration, double* best_loss);
 
-/* FullChipLSMOptimizer::optimize(initial_binary, global_target, options) with the
- * Binary renderer (full_chip_lsm.cpp:39-255): phi per Micro = reinit_sdf(mask - 0.5),
+/* FullChipLSMOptimizer::optimize(initial_binary, global_target, options)
+ * (full_chip_lsm.cpp:39-255) with the renderer of lsm->mask_type: phi per Micro = reinit_sdf(mask - 0.5),
  * loss region forced to Core + Guard, CEPT guard-only, Core metrics collected;
  * each iteration one RK2 update (litho_lsm_step) of every Micro with re-init
  * every reinit_interval updates (0: never), then the exchange.

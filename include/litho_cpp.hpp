@@ -563,12 +563,34 @@ class FullChipCTMOptimizer {
   double thr_, alpha_;
 };
 
-struct FullChipLSMOptions {         // full_chip_lsm.h:10-25 (Binary render)
+enum class LevelSetMaskType { Binary, Gray };   // levelset_mask_renderer.h
+
+// render_levelset_mask(phi, type, scale) (levelset_mask_renderer.cpp:7-33) on
+// device 0.  Gray: bilinear upscale by `scale` (cv::resize INTER_LINEAR
+// semantics), zero threshold, mean over each pixel's scale x scale block.
+inline Matrix render_levelset_mask(const Matrix& phi, LevelSetMaskType type, int scale = 4) {
+  // one context for these calls, created on first use and kept for the process
+  static litho_ctx* ctx = [] {
+    litho_ctx* c = nullptr;
+    detail::check(litho_ctx_create(0, &c));
+    return c;
+  }();
+  const size_t n = (size_t)phi.rows() * phi.cols();
+  detail::Buffer p(n), m(n);
+  p.upload(phi);
+  detail::check(litho_render_levelset_mask(ctx, 1, phi.rows(), phi.cols(), p.get(),
+                                           type == LevelSetMaskType::Gray ? 1 : 0, scale, m.get()));
+  return m.download(phi.rows(), phi.cols());
+}
+
+struct FullChipLSMOptions {         // full_chip_lsm.h:10-25
   int iterations = 200;
   double dx = 1.0, dy = 1.0;
   double cfl = 0.7;
   double curvature_weight = 0.01;
   int reinit_interval = 20;
+  LevelSetMaskType mask_type = LevelSetMaskType::Binary;
+  int render_scale = 4;             // Gray: upscale per direction
   MicroGradientOptions gradient_options{};
 };
 struct FullChipLSMIteration {
@@ -597,7 +619,8 @@ class FullChipLSMOptimizer {
     b.upload(initial_binary);
     t.upload(global_target);
     const litho_exchange_options o = detail::to_abi(options.gradient_options, thr_, alpha_);
-    const litho_lsm_options lo{options.dx, options.dy, options.cfl, options.curvature_weight};
+    const litho_lsm_options lo{options.dx, options.dy, options.cfl, options.curvature_weight,
+                               options.mask_type == LevelSetMaskType::Gray ? 1 : 0, options.render_scale};
     std::vector<double> h(5 * (size_t)(options.iterations + 1));
     FullChipLSMResult r;
     detail::check(litho_fullchip_lsm(cache_.plan(), meta_.tiles.data(), (int)meta_.tiles.size(), meta_.height,

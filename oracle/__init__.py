@@ -30,7 +30,10 @@ Parity pins (what each part is checked against, in tests/):
 * ctm: the reference optimizer-control invariants (test_ctm_optimizer.cpp).
   L-BFGS line-search trajectory: PARITY UNPINNED (LBFGS++ is not vendored).
 * levelset: the reference Godunov tests (test_level_set_godunov.cpp).
-  Chamfer SDF re-init and Gray rendering: PARITY UNPINNED (OpenCV absent).
+  Chamfer SDF re-init: PARITY UNPINNED against OpenCV (absent); the
+  restatement is pinned by exact shortest paths on the 5x5 chamfer graph.
+  Gray rendering: PARITY UNPINNED against cv::resize (absent); the
+  restatement is pinned by closed forms (scale 1, constant and linear fields).
 * fullchip: the reference exchange / trajectory tests
   (test_micro_gradient_exchange.cpp, test_full_chip_ctm.cpp,
   test_full_chip_lsm.cpp, test_mask_continuity.cpp).

@@ -334,6 +334,8 @@ class LSMOptions:
     cfl: float = 0.7
     curvature_weight: float = 0.01
     reinit_interval: int = 20
+    mask_type: str = "Binary"   # or "Gray" (full_chip_lsm.h:14-15)
+    render_scale: int = 4
 
 
 def evolution_term(phi, grad, lo: LSMOptions):
@@ -345,8 +347,8 @@ def evolution_term(phi, grad, lo: LSMOptions):
 
 
 def lsm_update_tile(phi, mask_gradient, tile: Tile, size: int, lo: LSMOptions, iteration: int):
-    """One RK2 update of one Micro (full_chip_lsm.cpp:215-241), Binary render.
-    Returns (next_phi, next_mask, dt)."""
+    """One RK2 update of one Micro (full_chip_lsm.cpp:215-241), then the
+    render of lo.mask_type.  Returns (next_phi, next_mask, dt)."""
     w = tile.update_local
     grad = mask_gradient * update_window(tile, size)
     n1, dt = evolution_term(phi, grad, lo)
@@ -359,12 +361,12 @@ def lsm_update_tile(phi, mask_gradient, tile: Tile, size: int, lo: LSMOptions, i
         signs = (nxt >= 0.0).astype(np.float64) - 0.5
         sdf = ls.reinit_sdf(signs, lo.dx, lo.dy)
         nxt[w.sl()] = sdf[w.sl()]
-    return nxt, ls.render_binary(nxt), dt
+    return nxt, ls.render_levelset_mask(nxt, lo.mask_type, lo.render_scale), dt
 
 
 def full_chip_lsm(meta: Meta, cache: Cache, initial_binary, global_target, o: Options, lo: LSMOptions,
                   threshold: float, alpha: float):
-    """FullChipLSMOptimizer::optimize, Binary render (full_chip_lsm.cpp:39-255)."""
+    """FullChipLSMOptimizer::optimize (full_chip_lsm.cpp:39-255), render of lo.mask_type."""
     o = Options(**{**o.__dict__, "loss_region": "CoreGuard", "cept_guard_only": True,
                    "collect_core_metrics": True})
     phi, masks = [], []
@@ -372,7 +374,7 @@ def full_chip_lsm(meta: Meta, cache: Cache, initial_binary, global_target, o: Op
         m = extract_local_tile(initial_binary, t)
         p = ls.reinit_sdf(m - 0.5, lo.dx, lo.dy)
         phi.append(p)
-        masks.append(ls.render_binary(p))
+        masks.append(ls.render_levelset_mask(p, lo.mask_type, lo.render_scale))
     ev = exchange_compute(meta, cache, masks, global_target, o, threshold, alpha)
     best = {"best_loss": np.inf}
     history = []

@@ -200,3 +200,60 @@ def reinit_sdf(phi: np.ndarray, dx: float, dy: float) -> np.ndarray:
 def render_binary(phi: np.ndarray) -> np.ndarray:
     """render_levelset_mask, Binary mode (levelset_mask_renderer.cpp:7-8)."""
     return (phi >= 0.0).astype(np.float64)
+
+
+def _linear_taps(n_src: int, scale: int):
+    """Source indices and float32 weights of a bilinear upscale by `scale`
+    with pixel-centre alignment and replicated borders, as cv::resize with
+    INTER_LINEAR defines them: f = (d + 0.5) / scale - 0.5 in FP64, rounded to
+    float32; s = floor(f); f -= s; s < 0 -> (0, f = 0); s >= n - 1 -> (n - 1,
+    f = 0)."""
+    d = np.arange(n_src * scale, dtype=np.float64)
+    f = ((d + 0.5) * (1.0 / scale) - 0.5).astype(np.float32)
+    s = np.floor(f).astype(np.int64)
+    f = (f - s.astype(np.float32)).astype(np.float32)
+    low = s < 0
+    f[low] = 0.0
+    s[low] = 0
+    high = s >= n_src - 1
+    f[high] = 0.0
+    s[high] = n_src - 1
+    return s, np.minimum(s + 1, n_src - 1), f
+
+
+def render_gray(phi: np.ndarray, scale: int) -> np.ndarray:
+    """render_levelset_mask, Gray mode (levelset_mask_renderer.cpp:10-33):
+    float32 phi upscaled by `scale` (bilinear, each output row interpolated
+    along x first, then the two rows along y, float32 products and sums),
+    subpixels >= 0 counted over each pixel's scale x scale block, divided by
+    scale^2.  OpenCV is not present here: parity with cv::resize is unpinned
+    (its float rounding depends on the build's vector code); this restatement
+    is pinned by closed forms (constant and linear fields, scale 1)."""
+    h, w = np.shape(phi)
+    v = upscale_linear(phi, scale)
+    inside = (v >= np.float32(0.0)).reshape(h, scale, w, scale).sum(axis=(1, 3))
+    return inside.astype(np.float64) / float(scale * scale)
+
+
+def upscale_linear(phi: np.ndarray, scale: int) -> np.ndarray:
+    """The float32 bilinear upscale of render_gray, (rows * scale, cols * scale)."""
+    if scale <= 0:
+        raise ValueError("render_levelset_mask: scale must be positive")
+    p = np.asarray(phi, dtype=np.float32)
+    h, w = p.shape
+    ry0, ry1, fy = _linear_taps(h, scale)
+    cx0, cx1, fx = _linear_taps(w, scale)
+    one = np.float32(1.0)
+    a0, a1 = (one - fx).astype(np.float32), fx
+    b0, b1 = (one - fy).astype(np.float32)[:, None], fy[:, None]
+    r0, r1 = p[ry0], p[ry1]
+    h0 = r0[:, cx0] * a0 + r0[:, cx1] * a1
+    h1 = r1[:, cx0] * a0 + r1[:, cx1] * a1
+    return h0 * b0 + h1 * b1
+
+
+def render_levelset_mask(phi: np.ndarray, mask_type: str = "Binary", scale: int = 4) -> np.ndarray:
+    """render_levelset_mask(phi, type, scale) (levelset_mask_renderer.cpp:7-33)."""
+    if mask_type == "Binary":
+        return render_binary(phi)
+    return render_gray(phi, scale)

@@ -168,6 +168,50 @@ void test_full_chip() {
   require(binary, "FullChipLSM Binary render returns a 0/1 mask");
   std::printf("FullChipLSM: loss %.4f -> best %.4f (update %d), dt %.4f..%.4f\n", rl.history[0].loss, rl.best_loss,
               rl.best_iteration, rl.history[1].min_dt, rl.history[1].max_dt);
+
+  lo.mask_type = litho::LevelSetMaskType::Gray;
+  const auto rg = lsm.optimize(target, target, lo);
+  bool quarters = true, fractional = false;
+  for (size_t i = 0; i < rg.global_mask.size(); ++i) {
+    const double v = rg.global_mask.data()[i] * 16.0;
+    quarters = quarters && v == (double)(int)v && v >= 0.0 && v <= 16.0;
+    fractional = fractional || (v > 0.0 && v < 16.0);
+  }
+  require(quarters && fractional, "FullChipLSM Gray render returns coverages in sixteenths, some partial");
+  std::printf("FullChipLSM Gray: loss %.4f -> best %.4f\n", rg.history[0].loss, rg.best_loss);
+}
+
+// render_levelset_mask closed forms: scale 1 is the Binary render; a field
+// linear in x with its zero at x = c covers, per pixel x, the fraction of its
+// subpixel centres (x * s + k + 0.5) / s - 0.5 >= c (clamped to the tile).
+void test_render() {
+  litho::Matrix phi(3, 16);
+  const double c = 7.0;
+  for (int y = 0; y < 3; ++y)
+    for (int x = 0; x < 16; ++x) phi(y, x) = x - c;
+  const auto b = litho::render_levelset_mask(phi, litho::LevelSetMaskType::Binary);
+  const auto g1 = litho::render_levelset_mask(phi, litho::LevelSetMaskType::Gray, 1);
+  const auto g4 = litho::render_levelset_mask(phi, litho::LevelSetMaskType::Gray, 4);
+  bool same = true, ramp = true;
+  for (int x = 0; x < 16; ++x) {
+    same = same && b(1, x) == g1(1, x) && b(1, x) == (x >= c ? 1.0 : 0.0);
+    int inside = 0;
+    for (int k = 0; k < 4; ++k) {
+      double u = (x * 4 + k + 0.5) / 4.0 - 0.5;
+      u = u < 0.0 ? 0.0 : (u > 15.0 ? 15.0 : u);
+      inside += u - c >= 0.0 ? 1 : 0;
+    }
+    ramp = ramp && g4(1, x) == inside / 4.0;
+  }
+  require(same, "render_levelset_mask: Gray at scale 1 equals Binary");
+  require(ramp, "render_levelset_mask: Gray coverage of a linear field");
+  bool threw = false;
+  try {
+    (void)litho::render_levelset_mask(phi, litho::LevelSetMaskType::Gray, 0);
+  } catch (const std::invalid_argument&) {
+    threw = true;
+  }
+  require(threw, "render_levelset_mask: scale 0 is rejected");
 }
 
 }  // namespace
@@ -176,6 +220,7 @@ int main() {
   try {
     test_synthetic();
     test_full_chip();
+    test_render();
   } catch (const std::exception& e) {
     std::printf("FAIL: exception %s\n", e.what());
     return 1;

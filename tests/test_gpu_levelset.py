@@ -174,3 +174,49 @@ def test_lsm_step_rejects_bad_window(gpu_ctx):
     phi = torch.zeros((1, n, n), dtype=torch.float32, device="cuda")
     with pytest.raises(la.LithoError):
         gpu_ctx.lsm_step(phi, phi, [(0, n + 1, 0, n)], phi)
+
+
+@pytest.mark.parametrize("rows,cols,scale,seed", [(37, 53, 4, 1), (64, 64, 3, 2), (640, 640, 4, 3), (20, 9, 1, 4),
+                                                  (16, 16, 5, 5)])
+def test_render_levelset_mask_bit_exact(gpu_ctx, rows, cols, scale, seed):
+    """Gray render: the same float32 operations in the same order on both
+    sides, so the coverages match bit for bit (Binary likewise)."""
+    import torch
+    phi = np.stack([smooth_phi(rows, seed, cols), smooth_phi(rows, seed + 10, cols) - 0.5]).astype(np.float32)
+    d_phi = dev(phi)
+    for mask_type in ("Binary", "Gray"):
+        out = torch.zeros_like(d_phi)
+        print(f"case render {mask_type} {rows}x{cols} scale {scale}", flush=True)
+        gpu_ctx.render_levelset_mask(d_phi, out, mask_type=mask_type, scale=scale)
+        got = out.cpu().numpy()
+        for b in range(2):
+            ref = ls.render_levelset_mask(phi[b], mask_type, scale).astype(np.float32)
+            assert np.array_equal(got[b], ref)
+
+
+def test_render_levelset_mask_rejects_bad_scale(gpu_ctx):
+    import torch
+    la = litho_amd()
+    phi = torch.zeros((1, 8, 8), dtype=torch.float32, device="cuda")
+    with pytest.raises(la.LithoError):
+        gpu_ctx.render_levelset_mask(phi, torch.zeros_like(phi), mask_type="Gray", scale=0)
+
+
+@pytest.mark.parametrize("reinit", [False, True])
+def test_lsm_step_gray_render(gpu_ctx, reinit):
+    """lsm_step with the Gray renderer: phi as with Binary; the mask is the
+    Gray render of the returned phi (bit for bit against the oracle's render
+    of that same float32 phi)."""
+    import torch
+    n = 96
+    phi = f32(ls.reinit_sdf(manhattan(n, 7) - 0.5, 1.0, 1.0))
+    grad = f32(0.05 * np.random.default_rng(8).standard_normal((n, n)))
+    d_phi, d_grad = dev(phi[None]), dev(grad[None])
+    out, mask = torch.zeros_like(d_phi), torch.zeros_like(d_phi)
+    print(f"case lsm_step gray reinit={reinit}", flush=True)
+    gpu_ctx.lsm_step(d_phi, d_grad, [(4, n - 4, 4, n - 4)], out, mask, cfl=0.5, reinit=reinit, mask_type="Gray",
+                     render_scale=4)
+    got_phi = out.cpu().numpy()[0]
+    ref = ls.render_levelset_mask(got_phi, "Gray", 4).astype(np.float32)
+    assert np.array_equal(mask.cpu().numpy()[0], ref)
+    assert np.any((ref > 0) & (ref < 1))

@@ -134,3 +134,42 @@ def test_reinit_sdf_signs_and_empty_tile():
     assert sdf[7, 7] == pytest.approx(3.0, abs=1e-6)   # 3 pixels from the nearest outside pixel
     empty = ls.reinit_sdf(-np.ones((8, 8)), 1.0, 1.0)
     assert np.all(empty == -8192.0)                     # no inside pixel: OpenCV's DIST_MAX clamp
+
+
+# ---------------------------------------------------------------- renderer
+# render_levelset_mask (levelset_mask_renderer.cpp:7-33).  OpenCV is absent,
+# so the Gray restatement of cv::resize INTER_LINEAR is pinned by closed
+# forms: scale 1 is the Binary render, a constant field is all in or all
+# out, and a field linear in x is interpolated exactly, so each pixel covers
+# the fraction of its subpixel centres u = (x s + k + 0.5) / s - 0.5 (clamped
+# to the tile) with u >= c for the zero at x = c.
+
+def test_render_gray_scale_one_is_binary():
+    rng = np.random.default_rng(3)
+    phi = rng.standard_normal((9, 13)).astype(np.float32)
+    assert np.array_equal(ls.render_levelset_mask(phi, "Gray", 1), ls.render_binary(phi))
+
+
+def test_render_gray_constant_fields():
+    assert np.all(ls.render_gray(np.full((5, 7), 0.25), 4) == 1.0)
+    assert np.all(ls.render_gray(np.full((5, 7), -0.25), 3) == 0.0)
+    assert np.all(ls.render_gray(np.zeros((4, 4)), 2) == 1.0)      # phi >= 0 counts
+
+
+@pytest.mark.parametrize("scale", [2, 3, 4, 5])
+@pytest.mark.parametrize("c", [3.3, 7.0, 7.5, 10.125, -2.0, 20.0])
+def test_render_gray_linear_field(scale, c):
+    n = 16
+    phi = np.tile(np.arange(n, dtype=np.float64) - c, (4, 1))
+    got = ls.render_gray(phi, scale)
+    for x in range(n):
+        u = np.clip((x * scale + np.arange(scale) + 0.5) / scale - 0.5, 0, n - 1)
+        assert np.array_equal(got[:, x], np.full(4, np.mean(u - c >= 0)))
+
+
+def test_render_gray_transpose_and_scale_validation():
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal((11, 6))
+    assert np.array_equal(ls.render_gray(phi, 3).T, ls.render_gray(phi.T, 3))
+    with pytest.raises(ValueError):
+        ls.render_gray(phi, 0)
