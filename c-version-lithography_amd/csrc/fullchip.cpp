@@ -277,6 +277,51 @@ void check_chip(const Plan& plan, const litho_tile* tiles, int T, int height, in
 
 }  // namespace
 
+void micro_imaging(Plan& plan, const litho_tile* tiles, int T, int height, int width, const float* global_mask,
+                   double threshold, double alpha, float* global_aerial, float* global_wafer, float* local_aerial,
+                   float* local_wafer) {
+  check_chip(plan, tiles, T, height, width, global_mask, global_mask);
+  if (!std::isfinite(threshold) || !std::isfinite(alpha)) throw Error(1, "Imaging: threshold and alpha must be finite");
+  const int n = plan.n;
+  const size_t plane = (size_t)n * n;
+  hipStream_t s = plan.ctx->stream;
+  const std::vector<int> tab = tile_table(tiles, T);
+  DevMem table(tab.size() * sizeof(int)), masks, la, lw;
+  LITHO_HIP(hipMemcpyAsync(table.ptr, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  alloc_tiles(masks, T, n);
+  float* aerial = local_aerial;
+  float* wafer = local_wafer;
+  if (aerial == nullptr) {
+    alloc_tiles(la, T, n);
+    aerial = la.as<float>();
+  }
+  if (wafer == nullptr) {
+    alloc_tiles(lw, T, n);
+    wafer = lw.as<float>();
+  }
+  int max_core = 0;
+  for (int i = 0; i < T; ++i) max_core = std::max(max_core, area(tiles[i].core_local));
+  // every Micro reads the global mask with periodic Halo indices (mirco_imaging.cpp:15-17)
+  launch_extract(global_mask, height, width, table.as<int>(), T, n, masks.as<float>(), s);
+  for (int c = 0; c < T; c += plan.capacity) {
+    const int b = std::min(plan.capacity, T - c);
+    EvalRequest rq;
+    rq.mode = 0;
+    rq.base = masks.as<float>() + (size_t)c * plane;
+    rq.adjoint = false;
+    rq.im.thr = (float)threshold;
+    rq.im.alpha = (float)alpha;
+    rq.im.aerial = aerial + (size_t)c * plane;
+    rq.im.wafer = wafer + (size_t)c * plane;
+    run_eval(plan, b, rq);
+  }
+  // each Core writes the part of the chip it owns (mirco_imaging.cpp:23-25)
+  if (global_aerial != nullptr) launch_assemble(aerial, table.as<int>(), T, n, width, global_aerial, max_core, s);
+  if (global_wafer != nullptr) launch_assemble(wafer, table.as<int>(), T, n, width, global_wafer, max_core, s);
+  LITHO_HIP(hipGetLastError());
+  LITHO_HIP(hipStreamSynchronize(s));
+}
+
 FullChipResult fullchip_ctm(Plan& plan, const litho_tile* tiles, int T, int height, int width,
                             const float* initial_pattern, const float* global_target,
                             const litho_exchange_options& options, int iterations, double learning_rate,

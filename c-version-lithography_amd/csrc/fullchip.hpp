@@ -42,6 +42,14 @@ class Exchanger {
   Exchanger& operator=(const Exchanger&) = delete;
 };
 
+// MicroImaging::compute(global_mask, threshold, alpha) (mirco_imaging.cpp:9-27):
+// every Micro extracted from the global mask (periodic Halo), imaged in
+// batches of the plan capacity, the Cores assembled into the global aerial /
+// wafer images.  Outputs nullable; local_* are [T][n][n].
+void micro_imaging(Plan& plan, const litho_tile* tiles, int num_tiles, int height, int width, const float* global_mask,
+                   double threshold, double alpha, float* global_aerial, float* global_wafer, float* local_aerial,
+                   float* local_wafer);
+
 struct FullChipResult {
   int best_iteration = 0;
   double best_loss = 0.0;

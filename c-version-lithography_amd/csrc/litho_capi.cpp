@@ -567,6 +567,17 @@ int litho_fullchip_ctm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
   });
 }
 
+int litho_micro_imaging(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                        const float* global_mask, double threshold, double alpha, float* global_aerial,
+                        float* global_wafer, float* local_aerial, float* local_wafer) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    set_device(p);
+    litho::micro_imaging(p, tiles, num_tiles, height, width, global_mask, threshold, alpha, global_aerial,
+                         global_wafer, local_aerial, local_wafer);
+  });
+}
+
 int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
                        const float* initial_binary, const float* global_target, const litho_exchange_options* options,
                        const litho_lsm_options* lsm, int iterations, int reinit_interval, float* global_mask_out,

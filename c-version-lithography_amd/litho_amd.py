@@ -142,6 +142,7 @@ SIGNATURES = {
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
     "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
     "litho_render_levelset_mask": (_I, [_P, _I, _I, _I, _P, _I, _I, _P]),
+    "litho_micro_imaging": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _D, _D, _P, _P, _P, _P]),
     "litho_lsm_step": (_I, [_P, _I, _I, _P, _P, _IP, C.POINTER(LsmOptions), _I, _P, _P, _DP]),
     "litho_partition_build": (_I, [_I, _I, _I, _I, _I, _I, _IP, C.POINTER(Tile)]),
     "litho_partition_extract": (_I, [_P, _P, _I, _I, C.POINTER(Tile), _I, _I, _P]),
@@ -432,6 +433,14 @@ class Plan:
                                           C.byref(options), iterations, learning_rate, ptr(global_mask_out),
                                           ptr(local_masks_out), _dp(hist), C.byref(bi), C.byref(bl)))
         return dict(best_iteration=bi.value, best_loss=bl.value, history=hist)
+
+    def micro_imaging(self, tiles, global_mask, global_aerial=None, global_wafer=None, local_aerial=None,
+                      local_wafer=None, threshold=0.25, alpha=50.0):
+        """MicroImaging::compute over the Micros `tiles` of a (height, width) device mask."""
+        h, w = global_mask.shape
+        check(self.lib.litho_micro_imaging(self.h, tiles, len(tiles), h, w, ptr(global_mask), threshold, alpha,
+                                           ptr(global_aerial), ptr(global_wafer), ptr(local_aerial),
+                                           ptr(local_wafer)))
 
     def fullchip_lsm(self, tiles, initial_binary, target, options, iterations, global_mask_out, global_phi_out=None,
                      dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01, reinit_interval=20, mask_type="Binary",

@@ -349,6 +349,15 @@ int litho_tile_exchange(litho_plan* plan, const litho_tile* tiles, int num_tiles
                         const float* local_targets, const litho_exchange_options* options, float* mask_gradient,
                         float* raw_gradient, float* wafer, litho_tile_loss* per_tile, double* totals);
 
+/* MicroImaging::compute(global_mask, threshold, alpha) (mirco_imaging.cpp:9-27):
+ * each Micro's tile is gathered from the global mask with periodic indices,
+ * imaged (Imaging::compute, batches of the plan capacity), and every Core is
+ * written into the global aerial / wafer images.  global_mask, global_*:
+ * device height x width; local_* (nullable): device [num_tiles][n][n]. */
+int litho_micro_imaging(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                        const float* global_mask, double threshold, double alpha, float* global_aerial,
+                        float* global_wafer, float* local_aerial, float* local_wafer);
+
 /* FullChipCTMOptimizer::optimize(initial_pattern, global_target, options)
  * (full_chip_ctm.cpp:25-152): theta per Micro from initialize_theta of the
  * pattern, fixed-step gradient descent on theta inside Core + Guard (Halo frozen),

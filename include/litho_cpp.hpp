@@ -438,6 +438,43 @@ inline void check_chip_cache(const PartitionMeta& meta, const ImagingCache& cach
 }
 }  // namespace detail
 
+// MicroImaging (mirco_imaging.h): every Micro imaged from its periodic extract
+// of the global mask, the Cores assembled (mirco_imaging.cpp:9-27).  Built
+// from the cache rather than an Imaging object: the Micros run as batches of
+// the cache's plan capacity.
+struct MicroImagingResult {
+  std::vector<Matrix> local_aerial_images, local_wafer_images;
+  Matrix global_aerial_image, global_wafer_image;
+};
+class MicroImaging {
+ public:
+  MicroImaging(const ImagingCache& cache, const PartitionMeta& meta) : cache_(cache), meta_(meta) {
+    detail::check_chip_cache(meta, cache);
+  }
+  MicroImagingResult compute(const Matrix& global_mask, double threshold = 0.25, double alpha = 50.0) {
+    if (global_mask.rows() != meta_.height || global_mask.cols() != meta_.width)
+      throw std::invalid_argument("MicroImaging: mask size must match the partition");
+    const size_t g = (size_t)meta_.height * meta_.width, t = (size_t)cache_.N * cache_.N;
+    const size_t T = meta_.tiles.size();
+    detail::Buffer m(g), ga(g), gw(g), la(T * t), lw(T * t);
+    m.upload(global_mask);
+    detail::check(litho_micro_imaging(cache_.plan(), meta_.tiles.data(), (int)T, meta_.height, meta_.width, m.get(),
+                                      threshold, alpha, ga.get(), gw.get(), la.get(), lw.get()));
+    MicroImagingResult r;
+    for (size_t i = 0; i < T; ++i) {
+      r.local_aerial_images.push_back(la.download(cache_.N, cache_.N, i * t));
+      r.local_wafer_images.push_back(lw.download(cache_.N, cache_.N, i * t));
+    }
+    r.global_aerial_image = ga.download(meta_.height, meta_.width);
+    r.global_wafer_image = gw.download(meta_.height, meta_.width);
+    return r;
+  }
+
+ private:
+  const ImagingCache& cache_;
+  const PartitionMeta& meta_;
+};
+
 struct CoreLossMetrics {
   double pe_loss = 0.0, bg_penalty_raw = 0.0;
 };

@@ -346,6 +346,18 @@ def evolution_term(phi, grad, lo: LSMOptions):
     return kappa - delta, dt
 
 
+def micro_imaging(meta: Meta, cache: Cache, global_mask, threshold: float, alpha: float):
+    """MicroImaging::compute (mirco_imaging.cpp:9-27): every Micro imaged from
+    its periodic extract of the global mask, the Cores assembled.
+    Returns (global_aerial, global_wafer, local_aerial, local_wafer)."""
+    la, lw = [], []
+    for t in meta.tiles:
+        a, w, _ = forward(cache, extract_local_tile(global_mask, t), threshold, alpha)
+        la.append(a)
+        lw.append(w)
+    return assemble_global(la, meta), assemble_global(lw, meta), la, lw
+
+
 def lsm_update_tile(phi, mask_gradient, tile: Tile, size: int, lo: LSMOptions, iteration: int):
     """One RK2 update of one Micro (full_chip_lsm.cpp:215-241), then the
     render of lo.mask_type.  Returns (next_phi, next_mask, dt)."""

@@ -4,6 +4,8 @@
 // 1.35, 4 nm pixels, threshold 0.25, alpha 85) and the consistency checks of
 // tests/test_micro_gradient_exchange.cpp / test_full_chip_ctm.cpp.
 // Exit status 0 when every check holds; each failure prints its message.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -153,6 +155,24 @@ void test_full_chip() {
   require(rc.global_mask.allFinite() && rc.best_loss <= rc.history[0].loss, "FullChipCTM best state");
   std::printf("FullChipCTM: loss %.4f -> best %.4f (iteration %d)\n", rc.history[0].loss, rc.best_loss,
               rc.best_iteration);
+
+  // MicroImaging: each local image equals Imaging::compute of the Micro's
+  // extract; the global images hold the Cores
+  litho::MicroImaging micro(cache, meta);
+  const auto mi = micro.compute(target, 0.225, 50.0);
+  litho::Imaging single(cache);
+  const litho::Matrix first = litho::Partition::extract_local_tile(target, meta.tiles[0]);
+  const auto ref0 = single.compute(first, 0.225, 50.0);
+  double dmax = 0.0, amax = 0.0;
+  for (size_t i = 0; i < ref0.aerial_image.size(); ++i) {
+    dmax = std::max(dmax, std::abs(mi.local_aerial_images[0].data()[i] - ref0.aerial_image.data()[i]));
+    amax = std::max(amax, std::abs(ref0.aerial_image.data()[i]));
+  }
+  const litho_rect& core = meta.tiles[0].core_local;
+  const litho_rect& cg = meta.tiles[0].core_global;
+  require(mi.local_aerial_images.size() == meta.tiles.size() && dmax <= 1e-6 * amax &&
+              mi.global_aerial_image(cg.y0, cg.x0) == mi.local_aerial_images[0](core.y0, core.x0),
+          "MicroImaging local images and Core assembly");
 
   litho::FullChipLSMOptions lo;
   lo.iterations = 3;

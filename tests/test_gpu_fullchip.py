@@ -90,6 +90,29 @@ def test_extract_assemble_exact(gpu_ctx):
     assert np.array_equal(host(out), fc.assemble_global(ref, meta))
 
 
+@pytest.mark.parametrize("capacity", [4, 3])
+def test_micro_imaging_matches_oracle(capacity):
+    """MicroImaging::compute: the chunked batch of Micros (capacity 3 splits the
+    four) against the oracle's per-Micro imaging; aerial <= 1e-5 relative L-inf
+    (the forward tolerance), wafer <= 1e-4 absolute."""
+    import torch
+    la, ctx, plan, ts, tiles, meta, cache = setup(capacity)
+    mask = f32(np.clip(chip_target(96, 96, 7) + 0.2 * np.random.default_rng(2).standard_normal((96, 96)), 0, 1))
+    print(f"case micro_imaging capacity {capacity}", flush=True)
+    ga, gw = torch.zeros((96, 96), dtype=torch.float32, device="cuda"), torch.zeros((96, 96), dtype=torch.float32,
+                                                                                       device="cuda")
+    loc = torch.zeros((len(tiles), ts, ts), dtype=torch.float32, device="cuda")
+    plan.micro_imaging(tiles, dev(mask), ga, gw, local_aerial=loc, threshold=THR, alpha=ALPHA)
+    ra, rw, rla, _ = fc.micro_imaging(meta, cache, mask, THR, ALPHA)
+    ea = np.max(np.abs(host(ga) - ra)) / np.max(np.abs(ra))
+    print(f"micro_imaging: aerial rel Linf {ea:.2e}, wafer abs {np.max(np.abs(host(gw) - rw)):.2e}")
+    assert ea <= 1e-5
+    assert np.max(np.abs(host(gw) - rw)) <= 1e-4
+    assert np.max(np.abs(host(loc) - np.stack(rla))) <= 1e-5 * np.max(np.abs(ra))
+    plan.close()
+    ctx.close()
+
+
 CASES = {
     "pe_core": dict(),
     "penalty_coreguard_cept2_avg": dict(loss_type="PEPenalty", loss_region="CoreGuard", enable_cept=True,
