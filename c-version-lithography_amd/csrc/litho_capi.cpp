@@ -154,7 +154,7 @@ int litho_socs_window(int n, double pixel_nm, double wavelength_nm, double na, d
                       int* win_rows, int* win_cols, int* win_y0, int* win_x0, int* num_source_points,
                       double* weight_sum) {
   return guarded([&] {
-    litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out};
+    litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, {}};
     const litho::SourceSet src = litho::make_source_set(op);
     const litho::Window w = litho::socs_window(op, src);
     if (win_rows) *win_rows = w.rows;
@@ -173,7 +173,23 @@ int litho_socs_precompute(litho_ctx* ctx, int n, double pixel_nm, double wavelen
     require(ctx != nullptr, "null context");
     require(kernels_out != nullptr && weights_out != nullptr, "null output arrays");
     LITHO_HIP(hipSetDevice(ctx->c.device));
-    litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out};
+    litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, {}};
+    litho::socs_precompute_device(op, k, ctx->c.stream, kernels_out, weights_out, singular_values_out);
+  });
+}
+
+int litho_socs_precompute_zernike(litho_ctx* ctx, int n, double pixel_nm, double wavelength_nm, double na,
+                                  double sigma_in, double sigma_out, const int* zernike_index,
+                                  const double* zernike_coeff, int num_terms, int k, double* kernels_out,
+                                  double* weights_out, double* singular_values_out) {
+  return guarded([&] {
+    require(ctx != nullptr, "null context");
+    require(kernels_out != nullptr && weights_out != nullptr, "null output arrays");
+    require(num_terms >= 0 && (num_terms == 0 || (zernike_index != nullptr && zernike_coeff != nullptr)),
+            "Zernike terms: null arrays");
+    LITHO_HIP(hipSetDevice(ctx->c.device));
+    litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, {}};
+    for (int i = 0; i < num_terms; ++i) op.zernike.emplace_back(zernike_index[i], zernike_coeff[i]);
     litho::socs_precompute_device(op, k, ctx->c.stream, kernels_out, weights_out, singular_values_out);
   });
 }

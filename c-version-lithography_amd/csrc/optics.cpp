@@ -28,6 +28,25 @@ std::vector<double> linspaced(int num, double low, double high) {
 
 }  // namespace
 
+double zernike_fringe(int index, double f, double g) {
+  const double s3 = std::sqrt(3.0), s5 = std::sqrt(5.0), s6 = std::sqrt(6.0), s8 = std::sqrt(8.0);
+  const double r2 = f * f + g * g;
+  switch (index) {
+    case 1: return 1.0;                                        // piston
+    case 2: return 2.0 * f;                                    // tilt x
+    case 3: return 2.0 * g;                                    // tilt y
+    case 4: return s3 * (2.0 * r2 - 1.0);                      // defocus
+    case 5: return s6 * (2.0 * f * g);                         // astigmatism 45
+    case 6: return s6 * (f * f - g * g);                       // astigmatism 0
+    case 7: return s8 * (3.0 * r2 - 2.0) * g;                  // coma y
+    case 8: return s8 * (3.0 * r2 - 2.0) * f;                  // coma x
+    case 9: return s8 * (3.0 * f * f * g - g * g * g);         // trefoil y
+    case 10: return s8 * (f * f * f - 3.0 * f * g * g);        // trefoil x
+    case 11: return s5 * (6.0 * r2 * r2 - 6.0 * r2 + 1.0);     // spherical
+    default: return 0.0;
+  }
+}
+
 SourceSet make_source_set(const OpticsParams& op) {
   if (op.n < 2) throw Error(1, "Grid: grid size must be at least 2");
   if (!(op.pixel_nm > 0.0) || !(op.wavelength_nm > 0.0) || !(op.na > 0.0))

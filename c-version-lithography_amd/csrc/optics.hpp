@@ -1,6 +1,7 @@
 // Optical model (FP64 host code) and the one-time SOCS kernel precompute.
 //
-// Restates Grid (grid.cpp:4-48), Pupil without aberrations (pupil.cpp:14-46),
+// Restates Grid (grid.cpp:4-48), Pupil with fringe Zernike aberrations
+// (pupil.cpp:5-46, zernike.cpp:11-53),
 // Source::compute_source_map (source.cpp:114-158),
 // LithoPrepare::_extract_source_points (litho_prepare.cpp:65-102) and the
 // compact SOCS window (litho_prepare.cpp:238-256) with the same IEEE operation
@@ -8,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <utility>
 #include <vector>
 
 namespace litho {
@@ -15,7 +17,14 @@ namespace litho {
 struct OpticsParams {
   int n;
   double pixel_nm, wavelength_nm, na, sigma_in, sigma_out;
+  // fringe Zernike terms {index 1..11, coefficient in waves} of the pupil
+  // wavefront, summed in the given order; zero coefficients are skipped
+  std::vector<std::pair<int, double>> zernike;
 };
+
+// Fringe Zernike Z_index(f, g) on normalised pupil coordinates
+// (zernike.cpp:11-43); 0 for an index outside 1..11.
+double zernike_fringe(int index, double f, double g);
 
 struct SourceSet {
   std::vector<double> f_norm;          // centred frequency axis / (NA / lambda), length n

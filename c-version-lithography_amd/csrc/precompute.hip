@@ -81,12 +81,25 @@ void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, d
   const double dfx = src.f_norm[1] - src.f_norm[0];
   const double dfy = dfx;
 
-  // pupil on the centred grid: [rho^2 <= 1] (no aberrations)
+  // pupil on the centred grid: [rho^2 <= 1] exp(i 2 pi W), W = sum_j c_j Z_j
+  // (pupil.cpp:14-23,41-46; zernike.cpp:46-53)
+  for (const auto& z : op.zernike)
+    if (z.second != 0.0 && (z.first < 1 || z.first > 11 || !std::isfinite(z.second)))
+      throw Error(1, "Pupil: Zernike terms must have fringe index 1..11 and a finite coefficient");
+  constexpr double kTwoPi = 6.283185307179586476925286766559;
   std::vector<double2> pupil((size_t)n * n);
   for (int y = 0; y < n; ++y)
     for (int x = 0; x < n; ++x) {
       const double fx = src.f_norm[x], fy = src.f_norm[y];
-      pupil[(size_t)y * n + x] = make_double2((fx * fx + fy * fy) <= 1.0 ? 1.0 : 0.0, 0.0);
+      if (!((fx * fx + fy * fy) <= 1.0)) {
+        pupil[(size_t)y * n + x] = make_double2(0.0, 0.0);
+        continue;
+      }
+      double w = 0.0;
+      for (const auto& z : op.zernike)
+        if (z.second != 0.0) w += z.second * zernike_fringe(z.first, fx, fy);
+      const double ph = kTwoPi * w;
+      pupil[(size_t)y * n + x] = make_double2(std::cos(ph), std::sin(ph));
     }
   // per-point integer shift and bilinear weights, same operation order as the reference
   std::vector<PointShift> pts(ns);

@@ -122,6 +122,7 @@ SIGNATURES = {
     "litho_ctx_set_graphs": (_I, [_P, _I]),
     "litho_socs_window": (_I, [_I, _D, _D, _D, _D, _D, _IP, _IP, _IP, _IP, _IP, _DP]),
     "litho_socs_precompute": (_I, [_P, _I, _D, _D, _D, _D, _D, _I, _DP, _DP, _DP]),
+    "litho_socs_precompute_zernike": (_I, [_P, _I, _D, _D, _D, _D, _D, _IP, _DP, _I, _I, _DP, _DP, _DP]),
     "litho_plan_create": (_I, [_P, _I, _I, _I, _I, _I, _I, _DP, _DP, _D, _I, C.POINTER(_P)]),
     "litho_plan_destroy": (_I, [_P]),
     "litho_plan_device_bytes": (C.c_size_t, [_P]),
@@ -220,14 +221,23 @@ class Context:
         except Exception:
             pass
 
-    def socs_precompute(self, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, k, singular_values=False):
-        """Device SOCS precompute; returns dict like oracle SocsKernels fields."""
+    def socs_precompute(self, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, k, singular_values=False,
+                        aberrations=None):
+        """Device SOCS precompute; returns dict like oracle SocsKernels fields.  aberrations:
+        {fringe Zernike index: coefficient in waves} (pupil wavefront), or None."""
         win = socs_window(n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out)
         kern = np.zeros((k, win["rows"], win["cols"], 2), dtype=np.float64)
         w = np.zeros(k, dtype=np.float64)
         sv = np.zeros(win["num_points"], dtype=np.float64) if singular_values else None
-        check(self.lib.litho_socs_precompute(self.h, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, k,
-                                             _dp(kern), _dp(w), _dp(sv)))
+        if aberrations:
+            idx = np.array(list(aberrations.keys()), dtype=np.int32)
+            cf = np.array(list(aberrations.values()), dtype=np.float64)
+            check(self.lib.litho_socs_precompute_zernike(self.h, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out,
+                                                         idx.ctypes.data_as(_IP), _dp(cf), len(idx), k, _dp(kern),
+                                                         _dp(w), _dp(sv)))
+        else:
+            check(self.lib.litho_socs_precompute(self.h, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, k,
+                                                 _dp(kern), _dp(w), _dp(sv)))
         out = dict(win)
         out.update(kernels=kern[..., 0] + 1j * kern[..., 1], weights=w, singular_values=sv)
         return out

@@ -80,6 +80,16 @@ int litho_socs_precompute(litho_ctx* ctx, int n, double pixel_nm, double wavelen
                           double sigma_in, double sigma_out, int k, double* kernels_out,
                           double* weights_out, double* singular_values_out);
 
+/* The same with pupil aberrations, Pupil (pupil.cpp:5-46) with
+ * ZernikeGenerator::wavefront (zernike.cpp:11-53): P = [rho^2 <= 1]
+ * exp(i 2 pi sum_j c_j Z_j) with fringe Zernike Z_1..Z_11 on the normalised
+ * pupil coordinates; num_terms (index, coefficient-in-waves) pairs, summed in
+ * the given order, zero coefficients skipped. */
+int litho_socs_precompute_zernike(litho_ctx* ctx, int n, double pixel_nm, double wavelength_nm, double na,
+                                  double sigma_in, double sigma_out, const int* zernike_index,
+                                  const double* zernike_coeff, int num_terms, int k, double* kernels_out,
+                                  double* weights_out, double* singular_values_out);
+
 /* ---------------------------------------------------------------- plan
  * Replaces ImagingCache (litho_prepare.h:18-47) plus the FFTW plans of Imaging
  * (imaging.cpp:20-51) and Gradient (gradient.cpp:14-25).  kernels/weights are

@@ -97,3 +97,29 @@ def test_eigengap_at_k24():
     s = op.socs_kernels(512, **OPT, num_kernels=25)
     gap = (s.weights[23] - s.weights[24]) / s.weights[23]
     assert abs(gap - 0.164) < 0.01
+
+
+def test_fringe_zernike_orthonormal_on_unit_disk():
+    """Z1..Z11 of zernike.cpp:11-43 are orthonormal over the unit disk under the
+    disk average (a wrong factor or sign in the table breaks this); pinned by a
+    midpoint rule on a fine polar grid."""
+    nr, nt = 400, 720
+    r = (np.arange(nr) + 0.5) / nr
+    t = (np.arange(nt) + 0.5) * 2.0 * np.pi / nt
+    rr, tt = np.meshgrid(r, t, indexing="ij")
+    f, g = rr * np.cos(tt), rr * np.sin(tt)
+    w = rr / (nr * nt) * 2.0          # dA / (pi) with dA = r dr dtheta
+    z = [op.zernike(j, f, g) for j in range(1, 12)]
+    gram = np.array([[np.sum(a * b * w) for b in z] for a in z])
+    assert np.allclose(gram, np.eye(11), atol=2e-4)
+    assert np.all(op.zernike(12, f, g) == 0.0)
+
+
+def test_pupil_with_aberrations_is_unit_modulus_inside():
+    n = 64
+    _, f_norm = op.grid_axes(n, 4.0, 1.35, 193.0)
+    p = op.pupil(f_norm, {4: 0.05, 9: -0.02})
+    p0 = op.pupil(f_norm)
+    inside = p0 != 0
+    assert np.allclose(np.abs(p[inside]), 1.0) and np.all(p[~inside] == 0)
+    assert np.max(np.abs(np.angle(p[inside]))) > 0.1
