@@ -383,6 +383,30 @@ __device__ __forceinline__ void run_passes(float2* buf, const float2* __restrict
   }
 }
 
+// The same passes with the twiddles of stage s >= 1 read from the stage
+// tables of stage_tables(N) (the inverse-order half: index s - 1), which
+// serve both directions (the forward transform applies the conjugates).
+template <int N, int IDX, int P, int LINES, int LS, int NT, int DIR, int SH>
+__device__ __forceinline__ void run_passes_staged(float2* buf, const float2* tabs) {
+  constexpr Radices f = factorize(N);
+  constexpr StageTables st = stage_tables(N);
+  if constexpr (IDX < f.count) {
+    constexpr int R = f.r[IDX];
+    if constexpr (IDX == 0)
+      stockham_pass<N, R, P, LINES, LS, NT, DIR, SH>(buf, TableTwiddles{tabs});   // P = 1: no twiddles read
+    else
+      stockham_pass<N, R, P, LINES, LS, NT, DIR, SH>(buf, PassTwiddles<P>{tabs + st.off[IDX - 1]});
+    run_passes_staged<N, IDX + 1, P * R, LINES, LS, NT, DIR, SH>(buf, tabs);
+  }
+}
+template <int N, int LINES, int LS, int NT, int DIR, int SH>
+__device__ __forceinline__ void fft_lines_staged(float2* buf, const float2* tabs) {
+  static_assert(factorize(N).count > 0, "unsupported FFT length");
+  run_passes_staged<N, 0, 1, LINES, LS, NT, DIR, SH>(buf, tabs);
+}
+// Entries of the inverse-order half of stage_tables(n).
+constexpr int staged_entries(int n) { return stage_tables(n).off[factorize(n).count - 1]; }
+
 // Transform LINES lines in LDS in place.  The caller must __syncthreads()
 // after writing the input; the routine ends with a barrier.
 template <int N, int LINES, int LS, int NT, int DIR, int SH>

@@ -1,6 +1,7 @@
 // Plan construction and the batched evaluation sequence.
 #include "plan.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdlib>
@@ -98,6 +99,20 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     LITHO_HIP(hipMemcpy(p->row_freq.ptr, rf.data(), p->row_freq.bytes, hipMemcpyHostToDevice));
     LITHO_HIP(hipMemcpy(p->col_freq.ptr, cf.data(), p->col_freq.bytes, hipMemcpyHostToDevice));
 
+    // band-limited column path: L = the smallest supported power of two >= 2 W_r - 1
+    // (LITHO_COLUMN_PATH = fused / generic selects the transforms of length N instead)
+    int L = 64;
+    while (L < 2 * wr - 1) L *= 2;
+    const bool corr_ok = corr_len_supported(L) && 2 * wr - 1 <= n;
+    const bool corr_off = path != nullptr && (std::string(path) == "fused" || std::string(path) == "generic");
+    p->corr_len = corr_ok && !corr_off ? L : 0;
+    if (p->corr_len > 0) {
+      // stage twiddle tables of the length-L transforms (both directions read the inverse-order half)
+      const std::vector<float2> twl = row_stage_twiddles(L);
+      p->twl.alloc(twl.size() * sizeof(float2));
+      LITHO_HIP(hipMemcpy(p->twl.ptr, twl.data(), p->twl.bytes, hipMemcpyHostToDevice));
+    }
+
     const size_t c = (size_t)capacity, N = (size_t)n;
     p->rt.alloc(c * wc * N * sizeof(float2));
     p->mhat.alloc(c * wr * wc * sizeof(float2));
@@ -105,6 +120,13 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     p->v.alloc(c * k * wr * N * sizeof(float2));
     p->y.alloc(c * k * wr * wc * sizeof(float2));
     p->p.alloc(c * wr * N * sizeof(float2));
+    if (p->corr_len > 0) {
+      p->acorr.alloc(c * wr * N * sizeof(float2));
+      p->ghat.alloc(c * wr * N * sizeof(float2));
+      size_t sp = 0;   // the largest over the batch sizes a call may use
+      for (int bb = 1; bb <= capacity; ++bb) sp = std::max(sp, corr_power_bytes(n, p->corr_len, k, bb));
+      p->spart.alloc(sp);
+    }
     p->img_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
     p->grad_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
     p->red.alloc(c * 4 * sizeof(double));
@@ -113,7 +135,8 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     p->host_coef = new HostPinned(c * sizeof(float));
     p->device_bytes = p->h.bytes + p->tw.bytes + p->rtw.bytes + p->row_of_u.bytes + p->col_of_u.bytes +
                       p->row_freq.bytes + p->col_freq.bytes + p->rt.bytes +
-                      p->mhat.bytes + p->t.bytes + p->v.bytes + p->y.bytes + p->p.bytes + p->img_partials.bytes +
+                      p->mhat.bytes + p->t.bytes + p->v.bytes + p->y.bytes + p->p.bytes + p->twl.bytes +
+                      p->acorr.bytes + p->ghat.bytes + p->spart.bytes + p->img_partials.bytes +
                       p->grad_partials.bytes + p->red.bytes + p->coef.bytes;
   } catch (...) {
     delete p;

@@ -99,6 +99,8 @@ struct Plan {
   DevMem h, tw, rtw, row_freq, col_freq, row_of_u, col_of_u;
   int pruned = 0;                            // window-pruned column kernels in use
   int row_pruned = 0;                        // window-pruned row kernels in use
+  int corr_len = 0;                          // L of the band-limited column path, 0 = fused column kernel
+  DevMem twl, acorr, ghat, spart;
   int tuned_chunk = 0;                       // batch size chosen by autotune_chunk (0 = not tuned)
   int tuned_limit = 0;                       // the limit it was tuned for
   DevMem rt, mhat, t, v, y, p;
@@ -126,6 +128,12 @@ struct Plan {
     a.col_of_u = col_of_u.as<int>();
     a.rtw = rtw.as<float2>();
     a.row_pruned = row_pruned;
+    a.n = n;
+    a.corr_len = corr_len;
+    a.twl = twl.as<float2>();
+    a.acorr = acorr.as<float2>();
+    a.ghat = ghat.as<float2>();
+    a.spart = spart.as<float>();
     return a;
   }
 };

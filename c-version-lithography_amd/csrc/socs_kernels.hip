@@ -826,6 +826,338 @@ k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
   }
 }
 
+// ---------------------------------------------------------------- band-limited columns
+// The column stage computed from band-limited spectra.  Along a column x the
+// field spectrum T_k[:, x] is non-zero only on the W_r contiguous window
+// rows (frequencies nu_0 .. nu_0 + W_r - 1), so
+//   I[y] = sum_k |E_k[y]|^2 = sum_{|d| < W_r} c[d] e^{2 pi i d y / N},
+//   c[d] = sum_k sum_j T_k[j + d] conj(T_k[j])       (c[-d] = conj c[d]),
+// and the adjoint column spectra at the window rows are
+//   V_k[i] = sum_y g[y] E_k[y] e^{-2 pi i nu_i y / N} = sum_j ghat[i - j] T_k[j],
+//   ghat[m] = sum_y g[y] e^{-2 pi i m y / N}          (g real: ghat[-m] = conj ghat[m]).
+// Both are length-W_r correlations, evaluated exactly with FFTs of length
+// L >= 2 W_r - 1 (no circular wrap-around reaches the kept lags):
+//   K4a: c = IFFT_L(sum_k |FFT_L(t_k)|^2) / L, t_k = T_k[:, x] zero-padded;
+//   K4b: I = IDFT_N(c) (one column transform), resist, loss, dL/dI = g,
+//        ghat = DFT_N(g) at m in [0, W_r);
+//   K5:  V_k = IFFT_L(FFT_L(t_k) . FFT_L(h) / L)[W_r - 1 + i], h[p] = ghat[p - W_r + 1].
+// Per column that is 2K + 2 transforms of length L and two of length N,
+// against 3K transforms of length N in the fused kernel.
+template <int L>
+struct CorrCfg {
+  static constexpr int LINES = 4096 / L > 0 ? 4096 / L : 1;   // columns per block
+  static constexpr int NT = 512;
+  static constexpr int SH = 5;                                 // the passes run lanes along a line
+  static constexpr int TW = fft::staged_entries(L);            // stage twiddle tables (LDS)
+  static constexpr int LS = fft::padded_stride<SH>(L);
+  static constexpr int ELEMS = LINES * L;
+  static constexpr int PER = ELEMS / NT;
+  static_assert(ELEMS % NT == 0, "whole elements per thread");
+};
+
+bool corr_len_supported(int l) { return l == 64 || l == 128 || l == 256 || l == 512 || l == 1024; }
+
+// per-pixel resist, losses and dL/dI shared by the column image kernels
+__device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, float iv, double& pe, double& bg) {
+  const float w = 1.0f / (1.0f + expf(-im.alpha * (iv - im.thr)));
+  if (im.aerial != nullptr) im.aerial[gi] = iv;
+  if (im.wafer != nullptr) im.wafer[gi] = w;
+  float g = 0.f;
+  if (im.target != nullptr) {
+    const float tv = im.target[gi];
+    const float r = w - tv;
+    const float pw = im.pe_win != nullptr ? im.pe_win[gi] : 1.0f;
+    float drive = im.pe_weight * r * pw;
+    pe += (double)r * (double)r * (double)pw;
+    if (im.pen_win != nullptr) {
+      const float bw = im.pen_win[gi];
+      const float ex = fmaxf(w - im.pen_thr, 0.f);
+      const float bgm = tv < 0.5f ? 1.f : 0.f;
+      bg += (double)bw * (double)bgm * (double)ex * (double)ex;
+      drive += im.penalty * bgm * ex * bw;
+    }
+    g = drive * w * (1.0f - w);
+    if (im.dldi != nullptr) im.dldi[gi] = g;
+  }
+  return g;
+}
+
+// Element ownership of the length-L kernels: e = tid + s*NT, column e % LINES,
+// position e / LINES (coalesced LINES-wide segments of a window row).
+template <int L>
+__device__ __forceinline__ void corr_load_t(float2* buf, const float2* __restrict__ tk, int wr, int n, int x0) {
+  using C = CorrCfg<L>;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int line = e % C::LINES, p = e / C::LINES;
+    const int x = x0 + line;
+    buf[line * C::LS + fft::swz<C::SH>(p)] = (p < wr && x < n) ? tk[(size_t)p * n + x] : make_float2(0.f, 0.f);
+  }
+}
+
+// K4a (1/2): partial power spectra S_z[m] = sum_{k in group z} |FFT_L(t_k)[m]|^2
+// of each column, groups of k over blockIdx.z (blocks enough to fill the chip
+// at batch 1); S layout [b][z][m][x].
+template <int L>
+__global__ void __launch_bounds__(CorrCfg<L>::NT)
+k_corr_power(SocsArgs a, float* __restrict__ spart) {
+  using C = CorrCfg<L>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ float2 tw[C::TW > 0 ? C::TW : 1];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int n = a.n, x0 = blockIdx.x * C::LINES;
+  const int groups = gridDim.z, z = blockIdx.z;
+  const int k0 = (int)((long long)a.k * z / groups), k1 = (int)((long long)a.k * (z + 1) / groups);
+  for (int e = threadIdx.x; e < C::TW; e += C::NT) tw[e] = a.twl[e];
+  const float2* t = a.t + (size_t)b * a.k * a.wr * n;
+  float acc[C::PER];
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) acc[s] = 0.f;
+#pragma unroll 1
+  for (int k = k0; k < k1; ++k) {
+    corr_load_t<L>(buf, t + (size_t)k * a.wr * n, a.wr, n, x0);
+    __syncthreads();
+    fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, -1, C::SH>(buf, tw);
+#pragma unroll
+    for (int s = 0; s < C::PER; ++s) {
+      const int e = threadIdx.x + s * C::NT;
+      const float2 v = buf[(e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES)];
+      acc[s] = fmaf(v.x, v.x, fmaf(v.y, v.y, acc[s]));
+    }
+    __syncthreads();
+  }
+  float* out = spart + ((size_t)b * groups + z) * L * n;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int x = x0 + e % C::LINES;
+    if (x < n) out[(size_t)(e / C::LINES) * n + x] = acc[s];
+  }
+}
+
+// K4a (2/2): S = sum of the groups' partials (increasing z), c = IFFT_L(S) / L on d in [0, W_r)
+template <int L>
+__global__ void __launch_bounds__(CorrCfg<L>::NT)
+k_corr_finish(SocsArgs a, const float* __restrict__ spart, int groups) {
+  using C = CorrCfg<L>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ float2 tw[C::TW > 0 ? C::TW : 1];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int n = a.n, x0 = blockIdx.x * C::LINES;
+  for (int e = threadIdx.x; e < C::TW; e += C::NT) tw[e] = a.twl[e];
+  const float* sp = spart + (size_t)b * groups * L * n;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int line = e % C::LINES, m = e / C::LINES;
+    const int x = x0 + line;
+    float v = 0.f;
+    if (x < n)
+      for (int z = 0; z < groups; ++z) v += sp[((size_t)z * L + m) * n + x];
+    buf[line * C::LS + fft::swz<C::SH>(m)] = make_float2(v, 0.f);
+  }
+  __syncthreads();
+  fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, +1, C::SH>(buf, tw);
+  const float inv = 1.f / (float)L;
+  float2* out = a.acorr + (size_t)b * a.wr * n;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int line = e % C::LINES, p = e / C::LINES;
+    const int x = x0 + line;
+    if (p < a.wr && x < n) {
+      const float2 v = buf[line * C::LS + fft::swz<C::SH>(p)];
+      out[(size_t)p * n + x] = make_float2(v.x * inv, v.y * inv);
+    }
+  }
+}
+
+// K4b: I = IDFT_N(c) per column, resist / losses / dL/dI, ghat = DFT_N(g) on [0, W_r)
+template <int N>
+__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
+k_col_image_corr(SocsArgs a, ImageArgs im) {
+  using C = ColCfg<N>;
+  constexpr int TW = fft::staged_entries(N);
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ double red[2 * C::NT / 64];
+  __shared__ float2 tw[TW > 0 ? TW : 1];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int x0 = blockIdx.x * C::LINES;
+  // the inverse-order half of the plan's stage tables (row_stage_twiddles(N))
+  for (int e = threadIdx.x; e < TW; e += C::NT) tw[e] = a.rtw[e];
+  const size_t tile = (size_t)b * N * N;
+  const float2* ac = a.acorr + (size_t)b * a.wr * N;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    if (e < C::ELEMS) {
+      const int line = e % C::LINES, d = e / C::LINES;
+      const int x = x0 + line;
+      float2 v = make_float2(0.f, 0.f);
+      if (x < N) {
+        if (d < a.wr)
+          v = ac[(size_t)d * N + x];
+        else if (N - d < a.wr)
+          v = cconj(ac[(size_t)(N - d) * N + x]);
+      }
+      buf[line * C::LS + fft::swz<C::SH>(d)] = v;
+    }
+  }
+  __syncthreads();
+  fft::fft_lines_staged<N, C::LINES, C::LS, C::NT, +1, C::SH>(buf, tw);
+  // each thread replaces its own elements of I by (dL/dI, 0), the input of the forward transform
+  double pe = 0.0, bg = 0.0;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int line = e % C::LINES, y = e / C::LINES;
+    const int x = x0 + line;
+    if (e < C::ELEMS) {
+      float2* q = buf + line * C::LS + fft::swz<C::SH>(y);
+      const float gv = x < N ? image_pixel(im, tile + (size_t)y * N + x, q->x, pe, bg) : 0.f;
+      *q = make_float2(gv, 0.f);
+    }
+  }
+  {
+    for (int off = 32; off > 0; off >>= 1) {
+      pe += __shfl_down(pe, off, 64);
+      bg += __shfl_down(bg, off, 64);
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+      red[2 * wave] = pe;
+      red[2 * wave + 1] = bg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && im.partials != nullptr) {
+      double sp = 0.0, sb = 0.0;
+      for (int w = 0; w < C::NT / 64; ++w) {
+        sp += red[2 * w];
+        sb += red[2 * w + 1];
+      }
+      double* out = im.partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
+      out[0] = sp;
+      out[1] = sb;
+    }
+  }
+  if (im.target == nullptr || a.v == nullptr) return;
+  // the reduction's barrier orders the (dL/dI, 0) writes before the transform
+  fft::fft_lines_staged<N, C::LINES, C::LS, C::NT, -1, C::SH>(buf, tw);
+  float2* gh = a.ghat + (size_t)b * a.wr * N;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int line = e % C::LINES, m = e / C::LINES;
+    const int x = x0 + line;
+    if (e < C::ELEMS && m < a.wr && x < N) gh[(size_t)m * N + x] = buf[line * C::LS + fft::swz<C::SH>(m)];
+  }
+}
+
+// K5: V_k at the window rows from the band-limited ghat
+template <int L>
+__global__ void __launch_bounds__(CorrCfg<L>::NT)
+k_corr_adjoint(SocsArgs a) {
+  using C = CorrCfg<L>;
+  __shared__ float2 buf[C::LINES * C::LS];
+  __shared__ float2 tw[C::TW > 0 ? C::TW : 1];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int n = a.n, x0 = blockIdx.x * C::LINES, wr = a.wr;
+  for (int e = threadIdx.x; e < C::TW; e += C::NT) tw[e] = a.twl[e];
+  const float2* gh = a.ghat + (size_t)b * wr * n;
+  // h[p] = ghat[p - (W_r - 1)] on p in [0, 2 W_r - 2], ghat[-m] = conj ghat[m]
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int line = e % C::LINES, p = e / C::LINES;
+    const int x = x0 + line, m = p - (wr - 1);
+    float2 v = make_float2(0.f, 0.f);
+    if (x < n && p <= 2 * (wr - 1)) v = m >= 0 ? gh[(size_t)m * n + x] : cconj(gh[(size_t)(-m) * n + x]);
+    buf[line * C::LS + fft::swz<C::SH>(p)] = v;
+  }
+  __syncthreads();
+  fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, -1, C::SH>(buf, tw);
+  float2 hh[C::PER];
+  const float inv = 1.f / (float)L;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const float2 v = buf[(e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES)];
+    hh[s] = make_float2(v.x * inv, v.y * inv);
+  }
+  __syncthreads();
+  const float2* t = a.t + (size_t)b * a.k * wr * n;
+  float2* vv = a.v + (size_t)b * a.k * wr * n;
+  const int groups = gridDim.z, z = blockIdx.z;
+  const int k0 = (int)((long long)a.k * z / groups), k1 = (int)((long long)a.k * (z + 1) / groups);
+#pragma unroll 1
+  for (int k = k0; k < k1; ++k) {
+    corr_load_t<L>(buf, t + (size_t)k * wr * n, wr, n, x0);
+    __syncthreads();
+    fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, -1, C::SH>(buf, tw);
+#pragma unroll
+    for (int s = 0; s < C::PER; ++s) {
+      const int e = threadIdx.x + s * C::NT;
+      float2* q = buf + (e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES);
+      *q = cmul(*q, hh[s]);
+    }
+    __syncthreads();
+    fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, +1, C::SH>(buf, tw);
+    float2* vk = vv + (size_t)k * wr * n;
+#pragma unroll
+    for (int s = 0; s < C::PER; ++s) {
+      const int e = threadIdx.x + s * C::NT;
+      const int line = e % C::LINES, p = e / C::LINES;
+      const int x = x0 + line, i = p - (wr - 1);
+      if (i >= 0 && i < wr && x < n) vk[(size_t)i * n + x] = buf[line * C::LS + fft::swz<C::SH>(p)];
+    }
+    __syncthreads();
+  }
+}
+
+// groups of k per column block: enough blocks to fill 256 CUs a few times over
+static int corr_groups(int n, int lines, int k, int batch) {
+  const int blocks = (n + lines - 1) / lines * batch;
+  int g = 1;
+  while (g < k && g < 8 && blocks * g < 1024) g *= 2;
+  return g < k ? g : k;
+}
+
+template <int L>
+static void launch_corr(int stage, const SocsArgs& a, int batch, float* spart, hipStream_t s) {
+  using C = CorrCfg<L>;
+  const int groups = corr_groups(a.n, C::LINES, a.k, batch);
+  dim3 g((a.n + C::LINES - 1) / C::LINES, batch, groups);
+  dim3 g1((a.n + C::LINES - 1) / C::LINES, batch);
+  if (stage == 0) {
+    hipLaunchKernelGGL(k_corr_power<L>, g, dim3(C::NT), 0, s, a, spart);
+    hipLaunchKernelGGL(k_corr_finish<L>, g1, dim3(C::NT), 0, s, a, (const float*)spart, groups);
+  } else {
+    hipLaunchKernelGGL(k_corr_adjoint<L>, g, dim3(C::NT), 0, s, a);
+  }
+}
+
+// stage 0: K4a (power spectra and autocorrelation); stage 1: K5
+static void launch_corr_any(int stage, const SocsArgs& a, int batch, float* spart, hipStream_t s) {
+  switch (a.corr_len) {
+    case 64: launch_corr<64>(stage, a, batch, spart, s); break;
+    case 128: launch_corr<128>(stage, a, batch, spart, s); break;
+    case 256: launch_corr<256>(stage, a, batch, spart, s); break;
+    case 512: launch_corr<512>(stage, a, batch, spart, s); break;
+    case 1024: launch_corr<1024>(stage, a, batch, spart, s); break;
+    default: break;
+  }
+}
+
+size_t corr_power_bytes(int n, int l, int k, int batch) {
+  const int lines = 4096 / l > 0 ? 4096 / l : 1;
+  return (size_t)batch * corr_groups(n, lines, k, batch) * l * n * sizeof(float);
+}
+
 // ---------------------------------------------------------------- K8 pruned
 template <int N>
 __global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
@@ -942,6 +1274,12 @@ struct SocsImpl {
   static void image_adjoint(const SocsArgs& a, const ImageArgs& im, int batch, hipStream_t s) {
     using C = ColCfg<N>;
     dim3 g(col_blocks(), batch);
+    if (a.corr_len > 0) {
+      launch_corr_any(0, a, batch, a.spart, s);
+      hipLaunchKernelGGL(k_col_image_corr<N>, g, dim3(C::NT), 0, s, a, im);
+      if (im.target != nullptr && a.v != nullptr) launch_corr_any(1, a, batch, nullptr, s);
+      return;
+    }
     if constexpr (fft::factorize(N).count >= 2) {
       if (a.pruned) {
         hipLaunchKernelGGL(k_col_image_adj_pruned<N>, g, dim3(C::NT), 0, s, a, im);

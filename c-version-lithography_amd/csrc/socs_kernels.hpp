@@ -35,7 +35,18 @@ struct SocsArgs {
   const int* col_of_u;            // [N] window column of row frequency u, or -1
   const float2* rtw;              // stage twiddle tables of the row transforms (row_stage_twiddles)
   int row_pruned;                 // window columns fit the pruned row transforms
+  int n;                          // tile size
+  int corr_len;                   // L of the band-limited column path (0: fused column kernel)
+  const float2* twl;              // stage twiddle tables of the length-L transforms (row_stage_twiddles(L))
+  float2* acorr;                  // [b][wr][n] column autocorrelation c[d], d in [0, wr)
+  float2* ghat;                   // [b][wr][n] column spectrum of dL/dI at m in [0, wr)
+  float* spart;                   // partial power spectra of the k groups (corr_power_bytes)
 };
+
+// Lengths of the band-limited column path (powers of two).
+bool corr_len_supported(int l);
+// Bytes of the partial power spectra of a batch (the k groups grow as the batch shrinks).
+size_t corr_power_bytes(int n, int l, int k, int batch);
 
 struct ImageArgs {
   float thr, alpha;

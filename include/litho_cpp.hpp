@@ -23,6 +23,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "litho_amd.h"
@@ -133,6 +134,9 @@ struct OpticsSpec {                 // SimulationParameters system / optics / so
   double na = 1.35;
   double sigma_in = 0.6;            // annular source
   double sigma_out = 0.9;
+  // Pupil::Params::zernike_coeffs: {fringe index 1..11, coefficient in waves},
+  // summed in this order (pupil.cpp:5-46, zernike.cpp:46-53)
+  std::vector<std::pair<int, double>> zernike_coeffs;
 };
 
 // The part of ImagingCache (litho_prepare.h:18-47) the hot path reads, resident
@@ -170,9 +174,16 @@ class LithoPrepare {
     detail::check(litho_ctx_create(device, &c_.device->ctx));
     std::vector<double> kern((size_t)num_kernels * c_.win_rows * c_.win_cols * 2);
     c_.socs_eigenvalues.assign(num_kernels, 0.0);
-    detail::check(litho_socs_precompute(c_.device->ctx, n, optics.pixel_size_nm, optics.wavelength_nm, optics.na,
-                                        optics.sigma_in, optics.sigma_out, num_kernels, kern.data(),
-                                        c_.socs_eigenvalues.data(), nullptr));
+    std::vector<int> zi;
+    std::vector<double> zc;
+    for (const auto& z : optics.zernike_coeffs) {
+      zi.push_back(z.first);
+      zc.push_back(z.second);
+    }
+    detail::check(litho_socs_precompute_zernike(c_.device->ctx, n, optics.pixel_size_nm, optics.wavelength_nm,
+                                                optics.na, optics.sigma_in, optics.sigma_out, zi.data(), zc.data(),
+                                                (int)zi.size(), num_kernels, kern.data(),
+                                                c_.socs_eigenvalues.data(), nullptr));
     detail::check(litho_plan_create(c_.device->ctx, n, num_kernels, c_.win_rows, c_.win_cols, c_.win_y0, c_.win_x0,
                                     kern.data(), c_.socs_eigenvalues.data(), c_.source_weight_sum, batch_capacity,
                                     &c_.device->plan));

@@ -3,10 +3,12 @@ the same tile size, on the GPU.
 
 Setting LITHO_ROW_PATH / LITHO_COLUMN_PATH to "generic" when a plan is
 created selects the unpruned kernels (full transforms with zero-filled
-spectra).  Each path is checked against the FP64 oracle in test_gpu_socs.py;
-here the four combinations must agree with each other to FP32 rounding:
-aerial image relative L-inf <= 1e-5, gradient relative L2 <= 1e-5, PE
-relative <= 1e-6.
+spectra); LITHO_COLUMN_PATH = "fused" selects the fused column kernel with
+pruned length-N transforms instead of the default band-limited column path
+(length-L correlations).  Each path is checked against the FP64 oracle in
+test_gpu_socs.py; here the combinations must agree with each other to FP32
+rounding: aerial image relative L-inf <= 1e-5, gradient relative L2 <= 1e-5,
+PE relative <= 1e-6.
 """
 import numpy as np
 import pytest
@@ -18,15 +20,16 @@ from test_gpu_socs import dev, empty, host, rel_l2, rel_linf, window_kernels
 
 pytestmark = pytest.mark.gpu
 
-PATHS = [("pruned", "pruned"), ("generic", "pruned"), ("pruned", "generic"), ("generic", "generic")]
+PATHS = [("pruned", "default"), ("generic", "default"), ("pruned", "fused"), ("pruned", "generic"),
+         ("generic", "generic")]
 
 
 def run(gpu_ctx, monkeypatch, s, mask, tgt, rows, cols):
     la = litho_amd()
     n = s.n
     for var, val in (("LITHO_ROW_PATH", rows), ("LITHO_COLUMN_PATH", cols)):
-        if val == "generic":
-            monkeypatch.setenv(var, "generic")
+        if val in ("generic", "fused"):
+            monkeypatch.setenv(var, val)
         else:
             monkeypatch.delenv(var, raising=False)
     plan = la.Plan.from_socs(gpu_ctx, s)
@@ -49,7 +52,7 @@ def test_pruned_transforms_equal_unpruned(gpu_ctx, monkeypatch, n, k):
     for rows, cols in PATHS:
         print(f"case paths n={n} k={k} rows={rows} columns={cols}", flush=True)
         out[(rows, cols)] = run(gpu_ctx, monkeypatch, s, mask, tgt, rows, cols)
-    a0, g0, pe0 = out[("pruned", "pruned")]
+    a0, g0, pe0 = out[("pruned", "default")]
     assert np.max(a0) > 0.05 and np.linalg.norm(g0) > 0.0
     for key, (a, g, pe) in out.items():
         ea, eg = rel_linf(a, a0), rel_l2(g, g0)
