@@ -112,10 +112,11 @@ int main(int argc, char** argv) {
               "hipMemcpy");
   check(litho_initialize_theta(ctx, target, theta, (size_t)batch * n * n), "litho_initialize_theta");
   if (mode == "eval" || mode == "evalb") {
-    double st[4];
+    double st[5];
     check(litho_profile_ctm_eval(plan, batch, theta, target, 0.225, 50.0, reps, st), "litho_profile_ctm_eval");
-    std::printf("n=%d k=%d batch=%d window=%dx%d stages_ms forward=%.4f column=%.4f adjoint_tail=%.4f eval=%.4f\n", n,
-                k, batch, wr, wc, st[0], st[1], st[2], st[3]);
+    std::printf("n=%d k=%d batch=%d window=%dx%d stages_ms forward=%.4f column=%.4f (image %.4f adjoint %.4f) "
+                "adjoint_tail=%.4f eval=%.4f\n",
+                n, k, batch, wr, wc, st[0], st[1] + st[2], st[1], st[2], st[3], st[4]);
   } else if (mode == "ctm") {
     const int solves = argc > 6 ? std::atoi(argv[6]) : 3;
     litho_ctm_options o{1, reps, 0.9, 10, 1e-2, 0.225, 50.0};

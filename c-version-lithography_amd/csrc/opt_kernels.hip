@@ -97,8 +97,10 @@ __device__ __forceinline__ float4 sub4(float4 a, float4 b) { return make_float4(
 // position 4 MB.  With shift set, xp <- x and gp <- g afterwards (the accepted
 // point becomes the base of the next line search).  Vectors are read as
 // float4: n is a multiple of 4 for every supported tile (N^2 with N even).
+// At most 128 VGPRs (four waves per SIMD): the kOptBlocks x 4 waves of a tile
+// then run in one round; the 4 (MB + 1) FP64 accumulators alone take 88 at MB = 10.
 template <int MB>
-__global__ void __launch_bounds__(kOptThreads)
+__global__ void __launch_bounds__(kOptThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* __restrict__ g,
              float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy, const int* __restrict__ need,
              const int* __restrict__ meta, size_t n, int shift, double* __restrict__ partials) {

@@ -107,8 +107,8 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     const bool corr_off = path != nullptr && (std::string(path) == "fused" || std::string(path) == "generic");
     p->corr_len = corr_ok && !corr_off ? L : 0;
     if (p->corr_len > 0) {
-      // stage twiddle tables of the length-L transforms (both directions read the inverse-order half)
-      const std::vector<float2> twl = row_stage_twiddles(L);
+      // pass twiddle tables of the register-resident length-L transforms (the inverse applies conjugates)
+      const std::vector<float2> twl = corr_twiddles(L);
       p->twl.alloc(twl.size() * sizeof(float2));
       LITHO_HIP(hipMemcpy(p->twl.ptr, twl.data(), p->twl.bytes, hipMemcpyHostToDevice));
     }
@@ -123,9 +123,7 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     if (p->corr_len > 0) {
       p->acorr.alloc(c * wr * N * sizeof(float2));
       p->ghat.alloc(c * wr * N * sizeof(float2));
-      size_t sp = 0;   // the largest over the batch sizes a call may use
-      for (int bb = 1; bb <= capacity; ++bb) sp = std::max(sp, corr_power_bytes(n, p->corr_len, k, bb));
-      p->spart.alloc(sp);
+      p->spart.alloc(corr_power_bytes(n, p->corr_len, k, capacity));
     }
     p->img_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
     p->grad_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
@@ -155,7 +153,7 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   im.partials = plan.img_partials.as<double>();
   SocsArgs a2 = a;
   if (!rq.adjoint) a2.v = nullptr;
-  plan.ops->image_adjoint(a2, im, batch, s);
+  plan.ops->image_adjoint(a2, im, batch, s, kColumnAll);
   reduce_pairs(plan.img_partials.as<double>(), plan.col_blocks, plan.red.as<double>(), 4, batch, s);
   if (rq.adjoint && rq.im.target != nullptr) {
     GradArgs ga = rq.ga;
@@ -211,9 +209,9 @@ void EvalGraph::launch() {
 void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, double* stage_ms) {
   if (repeats <= 0) throw Error(1, "profile: repeats must be positive");
   hipStream_t s = plan.ctx->stream;
-  hipEvent_t ev[4];
+  hipEvent_t ev[5];
   for (auto& e : ev) LITHO_HIP(hipEventCreate(&e));
-  double acc[4] = {0, 0, 0, 0};
+  double acc[5] = {0, 0, 0, 0, 0};
   SocsArgs a = plan.args();
   a.active = rq.active;
   ImageArgs im = rq.im;
@@ -222,28 +220,30 @@ void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, dou
   ga.partials = plan.grad_partials.as<double>();
   // untimed warm-up: the first launch of each kernel loads its code object
   plan.ops->forward_half(a, batch, rq.mode, rq.base, rq.dir, rq.coef, rq.theta_out, rq.mask_out, s);
-  plan.ops->image_adjoint(a, im, batch, s);
+  plan.ops->image_adjoint(a, im, batch, s, kColumnAll);
   plan.ops->adjoint_tail(a, ga, batch, s);
   LITHO_HIP(hipStreamSynchronize(s));
   for (int r = 0; r < repeats; ++r) {
     LITHO_HIP(hipEventRecord(ev[0], s));
     plan.ops->forward_half(a, batch, rq.mode, rq.base, rq.dir, rq.coef, rq.theta_out, rq.mask_out, s);
     LITHO_HIP(hipEventRecord(ev[1], s));
-    plan.ops->image_adjoint(a, im, batch, s);
+    plan.ops->image_adjoint(a, im, batch, s, kColumnImage);
     LITHO_HIP(hipEventRecord(ev[2], s));
-    plan.ops->adjoint_tail(a, ga, batch, s);
+    plan.ops->image_adjoint(a, im, batch, s, kColumnAdjoint);
     LITHO_HIP(hipEventRecord(ev[3], s));
-    LITHO_HIP(hipEventSynchronize(ev[3]));
+    plan.ops->adjoint_tail(a, ga, batch, s);
+    LITHO_HIP(hipEventRecord(ev[4], s));
+    LITHO_HIP(hipEventSynchronize(ev[4]));
     float ms = 0.f;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
       LITHO_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
       acc[i] += ms;
     }
-    LITHO_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
-    acc[3] += ms;
+    LITHO_HIP(hipEventElapsedTime(&ms, ev[0], ev[4]));
+    acc[4] += ms;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  for (int i = 0; i < 4; ++i) stage_ms[i] = acc[i] / repeats;
+  for (int i = 0; i < 5; ++i) stage_ms[i] = acc[i] / repeats;
 }
 
 const double* fetch_reductions(Plan& plan, int batch) {

@@ -33,6 +33,7 @@
 
 #include "fft_columns.hpp"
 #include "fft_engine.hpp"
+#include "fft_regs.hpp"
 #include "fft_rows.hpp"
 #include "opt_kernels.hpp"
 #include "socs_kernels.hpp"
@@ -843,17 +844,30 @@ k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
 //   K5:  V_k = IFFT_L(FFT_L(t_k) . FFT_L(h) / L)[W_r - 1 + i], h[p] = ghat[p - W_r + 1].
 // Per column that is 2K + 2 transforms of length L and two of length N,
 // against 3K transforms of length N in the fused kernel.
+// The length-L transforms run register-resident (fft_regs.hpp): thread
+// (column x0 + line, j) owns positions j + r L/8 of its column, loads them
+// from the window rows of T_k and stores them to the window rows of V_k
+// directly, and exchanges through LDS only between passes.
 template <int L>
 struct CorrCfg {
-  static constexpr int LINES = 4096 / L > 0 ? 4096 / L : 1;   // columns per block
-  static constexpr int NT = 512;
-  static constexpr int SH = 5;                                 // the passes run lanes along a line
-  static constexpr int TW = fft::staged_entries(L);            // stage twiddle tables (LDS)
-  static constexpr int LS = fft::padded_stride<SH>(L);
-  static constexpr int ELEMS = LINES * L;
-  static constexpr int PER = ELEMS / NT;
-  static_assert(ELEMS % NT == 0, "whole elements per thread");
+  static constexpr int LINES = 4096 / L;                       // columns per block
+  using F = fft::RegFFT<L, LINES>;
+  static constexpr int NT = F::NT;                             // 512
+  static constexpr int Q = F::Q;                               // threads per column
+  // waves per SIMD the register budget is sized for: two blocks per CU where
+  // the two exchange buffers fit twice in LDS (L >= 256), else one
+  static constexpr int WAVES = 2 * F::BUF * 8 * 2 <= 160 * 1024 ? 4 : 2;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id mod 8).
+// When the grid's x extent is a multiple of 8, block x runs on XCD x mod 8;
+// giving XCD c the contiguous column blocks [c nb/8, (c+1) nb/8) keeps the
+// 128-byte lines shared by neighbouring column blocks in one L2.
+__device__ __forceinline__ int xcd_column_block(int bx, int nb) {
+  return nb % 8 == 0 ? (bx % 8) * (nb / 8) + bx / 8 : bx;
+}
+
+constexpr int kCorrMaxGroups = 8;   // k groups of the power spectra (partials summed by k_corr_finish)
 
 bool corr_len_supported(int l) { return l == 64 || l == 128 || l == 256 || l == 512 || l == 1024; }
 
@@ -882,17 +896,18 @@ __device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, flo
   return g;
 }
 
-// Element ownership of the length-L kernels: e = tid + s*NT, column e % LINES,
-// position e / LINES (coalesced LINES-wide segments of a window row).
+// Loads positions j + r Q, r < 4, of a column's window rows (zero beyond W_r
+// or n).  L >= 2 W_r - 1 gives W_r <= L/2 = 4 Q: positions with r >= 4 are
+// always zero, and the transforms take them as such (RegFFT HALF).
 template <int L>
-__device__ __forceinline__ void corr_load_t(float2* buf, const float2* __restrict__ tk, int wr, int n, int x0) {
+__device__ __forceinline__ void corr_load_col(float2 (&v)[8], const float2* __restrict__ src, int wr, int n,
+                                              bool xin) {
   using C = CorrCfg<L>;
+  const int j = threadIdx.x / C::LINES;
 #pragma unroll
-  for (int s = 0; s < C::PER; ++s) {
-    const int e = threadIdx.x + s * C::NT;
-    const int line = e % C::LINES, p = e / C::LINES;
-    const int x = x0 + line;
-    buf[line * C::LS + fft::swz<C::SH>(p)] = (p < wr && x < n) ? tk[(size_t)p * n + x] : make_float2(0.f, 0.f);
+  for (int r = 0; r < 4; ++r) {
+    const int p = j + r * C::Q;
+    v[r] = (p < wr && xin) ? src[(size_t)p * n] : make_float2(0.f, 0.f);
   }
 }
 
@@ -900,41 +915,41 @@ __device__ __forceinline__ void corr_load_t(float2* buf, const float2* __restric
 // of each column, groups of k over blockIdx.z (blocks enough to fill the chip
 // at batch 1); S layout [b][z][m][x].
 template <int L>
-__global__ void __launch_bounds__(CorrCfg<L>::NT)
+__global__ void __launch_bounds__(CorrCfg<L>::NT) __attribute__((amdgpu_waves_per_eu(CorrCfg<L>::WAVES)))
 k_corr_power(SocsArgs a, float* __restrict__ spart) {
   using C = CorrCfg<L>;
-  __shared__ float2 buf[C::LINES * C::LS];
-  __shared__ float2 tw[C::TW > 0 ? C::TW : 1];
+  using F = typename C::F;
+  __shared__ float2 lds[2 * F::BUF];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
-  const int n = a.n, x0 = blockIdx.x * C::LINES;
+  const int n = a.n, wr = a.wr;
+  const int line = threadIdx.x % C::LINES, j = threadIdx.x / C::LINES;
+  const int x = xcd_column_block(blockIdx.x, gridDim.x) * C::LINES + line;
+  const bool xin = x < n;
   const int groups = gridDim.z, z = blockIdx.z;
   const int k0 = (int)((long long)a.k * z / groups), k1 = (int)((long long)a.k * (z + 1) / groups);
-  for (int e = threadIdx.x; e < C::TW; e += C::NT) tw[e] = a.twl[e];
-  const float2* t = a.t + (size_t)b * a.k * a.wr * n;
-  float acc[C::PER];
+  float2 tw[F::TWN];
+  F::load_twiddles(a.twl, tw, j);
+  const float2* t = a.t + (size_t)b * a.k * wr * n + (xin ? x : 0);
+  float acc[8];
 #pragma unroll
-  for (int s = 0; s < C::PER; ++s) acc[s] = 0.f;
+  for (int r = 0; r < 8; ++r) acc[r] = 0.f;
+  float2 nx[8], v[8];
+  if (k0 < k1) corr_load_col<L>(nx, t + (size_t)k0 * wr * n, wr, n, xin);
+  int ex = 0;
 #pragma unroll 1
   for (int k = k0; k < k1; ++k) {
-    corr_load_t<L>(buf, t + (size_t)k * a.wr * n, a.wr, n, x0);
-    __syncthreads();
-    fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, -1, C::SH>(buf, tw);
 #pragma unroll
-    for (int s = 0; s < C::PER; ++s) {
-      const int e = threadIdx.x + s * C::NT;
-      const float2 v = buf[(e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES)];
-      acc[s] = fmaf(v.x, v.x, fmaf(v.y, v.y, acc[s]));
-    }
-    __syncthreads();
-  }
-  float* out = spart + ((size_t)b * groups + z) * L * n;
+    for (int r = 0; r < 4; ++r) v[r] = nx[r];
+    if (k + 1 < k1) corr_load_col<L>(nx, t + (size_t)(k + 1) * wr * n, wr, n, xin);
+    F::template transform<-1, true>(v, lds, tw, line, j, ex);
 #pragma unroll
-  for (int s = 0; s < C::PER; ++s) {
-    const int e = threadIdx.x + s * C::NT;
-    const int x = x0 + e % C::LINES;
-    if (x < n) out[(size_t)(e / C::LINES) * n + x] = acc[s];
+    for (int r = 0; r < 8; ++r) acc[r] = fmaf(v[r].x, v[r].x, fmaf(v[r].y, v[r].y, acc[r]));
   }
+  if (!xin) return;
+  float* out = spart + ((size_t)b * groups + z) * L * n + x;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) out[(size_t)(j + r * C::Q) * n] = acc[r];
 }
 
 // K4a (2/2): S = sum of the groups' partials (increasing z), c = IFFT_L(S) / L on d in [0, W_r)
@@ -942,36 +957,37 @@ template <int L>
 __global__ void __launch_bounds__(CorrCfg<L>::NT)
 k_corr_finish(SocsArgs a, const float* __restrict__ spart, int groups) {
   using C = CorrCfg<L>;
-  __shared__ float2 buf[C::LINES * C::LS];
-  __shared__ float2 tw[C::TW > 0 ? C::TW : 1];
+  using F = typename C::F;
+  __shared__ float2 lds[2 * F::BUF];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
-  const int n = a.n, x0 = blockIdx.x * C::LINES;
-  for (int e = threadIdx.x; e < C::TW; e += C::NT) tw[e] = a.twl[e];
-  const float* sp = spart + (size_t)b * groups * L * n;
+  const int n = a.n, wr = a.wr;
+  const int line = threadIdx.x % C::LINES, j = threadIdx.x / C::LINES;
+  const int x = xcd_column_block(blockIdx.x, gridDim.x) * C::LINES + line;
+  const bool xin = x < n;
+  float2 tw[F::TWN];
+  F::load_twiddles(a.twl, tw, j);
+  const float* sp = spart + (size_t)b * groups * L * n + (xin ? x : 0);
+  float2 v[8];
 #pragma unroll
-  for (int s = 0; s < C::PER; ++s) {
-    const int e = threadIdx.x + s * C::NT;
-    const int line = e % C::LINES, m = e / C::LINES;
-    const int x = x0 + line;
-    float v = 0.f;
-    if (x < n)
-      for (int z = 0; z < groups; ++z) v += sp[((size_t)z * L + m) * n + x];
-    buf[line * C::LS + fft::swz<C::SH>(m)] = make_float2(v, 0.f);
-  }
-  __syncthreads();
-  fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, +1, C::SH>(buf, tw);
-  const float inv = 1.f / (float)L;
-  float2* out = a.acorr + (size_t)b * a.wr * n;
+  for (int r = 0; r < 8; ++r) v[r] = make_float2(0.f, 0.f);
+  // groups <= kCorrMaxGroups: the unrolled loop keeps every partial's load in flight
 #pragma unroll
-  for (int s = 0; s < C::PER; ++s) {
-    const int e = threadIdx.x + s * C::NT;
-    const int line = e % C::LINES, p = e / C::LINES;
-    const int x = x0 + line;
-    if (p < a.wr && x < n) {
-      const float2 v = buf[line * C::LS + fft::swz<C::SH>(p)];
-      out[(size_t)p * n + x] = make_float2(v.x * inv, v.y * inv);
+  for (int z = 0; z < kCorrMaxGroups; ++z) {
+    if (z < groups && xin) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r].x += sp[((size_t)z * L + j + r * C::Q) * n];
     }
+  }
+  int ex = 0;
+  F::template transform<+1>(v, lds, tw, line, j, ex);
+  if (!xin) return;
+  const float inv = 1.f / (float)L;
+  float2* out = a.acorr + (size_t)b * wr * n + x;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int p = j + r * C::Q;
+    if (p < wr) out[(size_t)p * n] = make_float2(v[r].x * inv, v[r].y * inv);
   }
 }
 
@@ -1059,85 +1075,108 @@ k_col_image_corr(SocsArgs a, ImageArgs im) {
 
 // K5: V_k at the window rows from the band-limited ghat
 template <int L>
-__global__ void __launch_bounds__(CorrCfg<L>::NT)
+__global__ void __launch_bounds__(CorrCfg<L>::NT) __attribute__((amdgpu_waves_per_eu(CorrCfg<L>::WAVES)))
 k_corr_adjoint(SocsArgs a) {
   using C = CorrCfg<L>;
-  __shared__ float2 buf[C::LINES * C::LS];
-  __shared__ float2 tw[C::TW > 0 ? C::TW : 1];
+  using F = typename C::F;
+  __shared__ float2 lds[2 * F::BUF];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
-  const int n = a.n, x0 = blockIdx.x * C::LINES, wr = a.wr;
-  for (int e = threadIdx.x; e < C::TW; e += C::NT) tw[e] = a.twl[e];
-  const float2* gh = a.ghat + (size_t)b * wr * n;
-  // h[p] = ghat[p - (W_r - 1)] on p in [0, 2 W_r - 2], ghat[-m] = conj ghat[m]
-#pragma unroll
-  for (int s = 0; s < C::PER; ++s) {
-    const int e = threadIdx.x + s * C::NT;
-    const int line = e % C::LINES, p = e / C::LINES;
-    const int x = x0 + line, m = p - (wr - 1);
-    float2 v = make_float2(0.f, 0.f);
-    if (x < n && p <= 2 * (wr - 1)) v = m >= 0 ? gh[(size_t)m * n + x] : cconj(gh[(size_t)(-m) * n + x]);
-    buf[line * C::LS + fft::swz<C::SH>(p)] = v;
-  }
-  __syncthreads();
-  fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, -1, C::SH>(buf, tw);
-  float2 hh[C::PER];
-  const float inv = 1.f / (float)L;
-#pragma unroll
-  for (int s = 0; s < C::PER; ++s) {
-    const int e = threadIdx.x + s * C::NT;
-    const float2 v = buf[(e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES)];
-    hh[s] = make_float2(v.x * inv, v.y * inv);
-  }
-  __syncthreads();
-  const float2* t = a.t + (size_t)b * a.k * wr * n;
-  float2* vv = a.v + (size_t)b * a.k * wr * n;
+  const int n = a.n, wr = a.wr;
+  const int line = threadIdx.x % C::LINES, j = threadIdx.x / C::LINES;
+  const int x = xcd_column_block(blockIdx.x, gridDim.x) * C::LINES + line;
+  const bool xin = x < n;
+  const int xs = xin ? x : 0;
   const int groups = gridDim.z, z = blockIdx.z;
   const int k0 = (int)((long long)a.k * z / groups), k1 = (int)((long long)a.k * (z + 1) / groups);
+  float2 tw[F::TWN];
+  F::load_twiddles(a.twl, tw, j);
+  // h[p] = ghat[p - (W_r - 1)] on p in [0, 2 W_r - 2], ghat[-m] = conj ghat[m]
+  const float2* gh = a.ghat + (size_t)b * wr * n + xs;
+  float2 hh[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int p = j + r * C::Q, m = p - (wr - 1);
+    float2 v = make_float2(0.f, 0.f);
+    if (xin && p <= 2 * (wr - 1)) v = m >= 0 ? gh[(size_t)m * n] : cconj(gh[(size_t)(-m) * n]);
+    hh[r] = v;
+  }
+  int ex = 0;
+  F::template transform<-1>(hh, lds, tw, line, j, ex);
+  const float inv = 1.f / (float)L;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) hh[r] = make_float2(hh[r].x * inv, hh[r].y * inv);
+  const float2* t = a.t + (size_t)b * a.k * wr * n + xs;
+  float2* vv = a.v + (size_t)b * a.k * wr * n + xs;
+  float2 nx[8], v[8];
+  if (k0 < k1) corr_load_col<L>(nx, t + (size_t)k0 * wr * n, wr, n, xin);
 #pragma unroll 1
   for (int k = k0; k < k1; ++k) {
-    corr_load_t<L>(buf, t + (size_t)k * wr * n, wr, n, x0);
-    __syncthreads();
-    fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, -1, C::SH>(buf, tw);
 #pragma unroll
-    for (int s = 0; s < C::PER; ++s) {
-      const int e = threadIdx.x + s * C::NT;
-      float2* q = buf + (e % C::LINES) * C::LS + fft::swz<C::SH>(e / C::LINES);
-      *q = cmul(*q, hh[s]);
-    }
-    __syncthreads();
-    fft::fft_lines_staged<L, C::LINES, C::LS, C::NT, +1, C::SH>(buf, tw);
-    float2* vk = vv + (size_t)k * wr * n;
+    for (int r = 0; r < 4; ++r) v[r] = nx[r];
+    if (k + 1 < k1) corr_load_col<L>(nx, t + (size_t)(k + 1) * wr * n, wr, n, xin);
+    F::template transform<-1, true>(v, lds, tw, line, j, ex);
 #pragma unroll
-    for (int s = 0; s < C::PER; ++s) {
-      const int e = threadIdx.x + s * C::NT;
-      const int line = e % C::LINES, p = e / C::LINES;
-      const int x = x0 + line, i = p - (wr - 1);
-      if (i >= 0 && i < wr && x < n) vk[(size_t)i * n + x] = buf[line * C::LS + fft::swz<C::SH>(p)];
+    for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], hh[r]);
+    F::template transform<+1>(v, lds, tw, line, j, ex);
+    if (xin) {
+      float2* vk = vv + (size_t)k * wr * n;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int i = j + r * C::Q - (wr - 1);
+        if (i >= 0 && i < wr) vk[(size_t)i * n] = v[r];
+      }
     }
-    __syncthreads();
   }
 }
 
-// groups of k per column block: enough blocks to fill 256 CUs a few times over
-static int corr_groups(int n, int lines, int k, int batch) {
-  const int blocks = (n + lines - 1) / lines * batch;
-  int g = 1;
-  while (g < k && g < 8 && blocks * g < 1024) g *= 2;
-  return g < k ? g : k;
+// Groups of k per column block (k in [k z / g, k (z+1) / g) for group z).
+// More groups give more blocks but repeat the transform of h once per block
+// and add partial spectra; g in {1, 2, 4, 8} minimises the modelled time
+//   rounds(g) * (ceil(k / g) + 1) + g / 4,
+// rounds = blocks * g / resident blocks, rounded up below two rounds (a
+// partly filled last round costs a whole one) and fractional above.  Resident
+// blocks: two per CU when both exchange buffers fit twice in LDS (L >= 256).
+static int corr_groups(int n, int l, int k, int batch) {
+  static const int cus = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      v = 256;
+    return v;
+  }();
+  const int lines = 4096 / l;
+  const double slots = (double)cus * (l >= 256 ? 2 : 1);
+  const double blocks = (double)((n + lines - 1) / lines) * batch;
+  int best = 1;
+  double best_cost = 0.0;
+  for (int g = 1; g <= kCorrMaxGroups && g <= k; g *= 2) {
+    double rounds = blocks * g / slots;
+    if (rounds < 2.0) rounds = std::ceil(rounds);
+    const double cost = rounds * ((k + g - 1) / g + 1) + g / 4.0;
+    if (g == 1 || cost < best_cost) {
+      best = g;
+      best_cost = cost;
+    }
+  }
+  return best;
 }
+
+// The power spectra are summed over the k groups of a single tile (batch 1)
+// whatever the batch, so a tile's image does not depend on the tiles it is
+// batched with: the FP32 partial sums depend on the grouping.  The adjoint's
+// groups only split the independent V_k and follow the batch.
+static int corr_power_groups(int n, int l, int k) { return corr_groups(n, l, k, 1); }
 
 template <int L>
 static void launch_corr(int stage, const SocsArgs& a, int batch, float* spart, hipStream_t s) {
   using C = CorrCfg<L>;
-  const int groups = corr_groups(a.n, C::LINES, a.k, batch);
-  dim3 g((a.n + C::LINES - 1) / C::LINES, batch, groups);
-  dim3 g1((a.n + C::LINES - 1) / C::LINES, batch);
+  const int cols = (a.n + C::LINES - 1) / C::LINES;
   if (stage == 0) {
-    hipLaunchKernelGGL(k_corr_power<L>, g, dim3(C::NT), 0, s, a, spart);
-    hipLaunchKernelGGL(k_corr_finish<L>, g1, dim3(C::NT), 0, s, a, (const float*)spart, groups);
+    const int groups = corr_power_groups(a.n, L, a.k);
+    hipLaunchKernelGGL(k_corr_power<L>, dim3(cols, batch, groups), dim3(C::NT), 0, s, a, spart);
+    hipLaunchKernelGGL(k_corr_finish<L>, dim3(cols, batch), dim3(C::NT), 0, s, a, (const float*)spart, groups);
   } else {
-    hipLaunchKernelGGL(k_corr_adjoint<L>, g, dim3(C::NT), 0, s, a);
+    hipLaunchKernelGGL(k_corr_adjoint<L>, dim3(cols, batch, corr_groups(a.n, L, a.k, batch)), dim3(C::NT), 0, s, a);
   }
 }
 
@@ -1154,8 +1193,7 @@ static void launch_corr_any(int stage, const SocsArgs& a, int batch, float* spar
 }
 
 size_t corr_power_bytes(int n, int l, int k, int batch) {
-  const int lines = 4096 / l > 0 ? 4096 / l : 1;
-  return (size_t)batch * corr_groups(n, lines, k, batch) * l * n * sizeof(float);
+  return (size_t)batch * corr_power_groups(n, l, k) * l * n * sizeof(float);
 }
 
 // ---------------------------------------------------------------- K8 pruned
@@ -1271,15 +1309,18 @@ struct SocsImpl {
     }
   }
 
-  static void image_adjoint(const SocsArgs& a, const ImageArgs& im, int batch, hipStream_t s) {
+  static void image_adjoint(const SocsArgs& a, const ImageArgs& im, int batch, hipStream_t s, int part) {
     using C = ColCfg<N>;
     dim3 g(col_blocks(), batch);
     if (a.corr_len > 0) {
-      launch_corr_any(0, a, batch, a.spart, s);
-      hipLaunchKernelGGL(k_col_image_corr<N>, g, dim3(C::NT), 0, s, a, im);
-      if (im.target != nullptr && a.v != nullptr) launch_corr_any(1, a, batch, nullptr, s);
+      if (part != kColumnAdjoint) {
+        launch_corr_any(0, a, batch, a.spart, s);
+        hipLaunchKernelGGL(k_col_image_corr<N>, g, dim3(C::NT), 0, s, a, im);
+      }
+      if (part != kColumnImage && im.target != nullptr && a.v != nullptr) launch_corr_any(1, a, batch, nullptr, s);
       return;
     }
+    if (part == kColumnAdjoint) return;
     if constexpr (fft::factorize(N).count >= 2) {
       if (a.pruned) {
         hipLaunchKernelGGL(k_col_image_adj_pruned<N>, g, dim3(C::NT), 0, s, a, im);
@@ -1334,6 +1375,19 @@ static const SocsOps* make_ops() {
 int first_radix(int n) {
   const fft::Radices f = fft::factorize(n);
   return f.count >= 2 ? f.r[0] : 0;
+}
+
+std::vector<float2> corr_twiddles(int l) {
+  const fft::RegPlan pl = fft::reg_plan(l);
+  if (pl.count < 2) return {};
+  std::vector<float2> out((size_t)pl.tab);
+  for (int q = 1; q < pl.count; ++q)
+    for (int r = 1; r < pl.R[q]; ++r)
+      for (int k = 0; k < pl.P[q]; ++k) {
+        const double ang = -2.0 * fft::kPi * (double)r * (double)k / ((double)pl.P[q] * (double)pl.R[q]);
+        out[(size_t)pl.taboff[q] + (size_t)(r - 1) * pl.P[q] + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+      }
+  return out;
 }
 
 std::vector<float2> row_stage_twiddles(int n) {

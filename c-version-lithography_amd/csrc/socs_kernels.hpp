@@ -37,7 +37,7 @@ struct SocsArgs {
   int row_pruned;                 // window columns fit the pruned row transforms
   int n;                          // tile size
   int corr_len;                   // L of the band-limited column path (0: fused column kernel)
-  const float2* twl;              // stage twiddle tables of the length-L transforms (row_stage_twiddles(L))
+  const float2* twl;              // pass twiddle tables of the length-L transforms (corr_twiddles(L))
   float2* acorr;                  // [b][wr][n] column autocorrelation c[d], d in [0, wr)
   float2* ghat;                   // [b][wr][n] column spectrum of dL/dI at m in [0, wr)
   float* spart;                   // partial power spectra of the k groups (corr_power_bytes)
@@ -70,12 +70,16 @@ struct GradArgs {
   double* partials;               // [b][col_blocks][2]: |G_theta|^2, G_theta . dir
 };
 
+enum ColumnPart { kColumnAll = 0, kColumnImage = 1, kColumnAdjoint = 2 };
+
 struct SocsOps {
   int n;
   int (*col_blocks)();
   void (*forward_half)(const SocsArgs&, int batch, int mode, const float* base, const float* dir,
                        const float* coef, float* theta_out, float* mask_out, hipStream_t);
-  void (*image_adjoint)(const SocsArgs&, const ImageArgs&, int batch, hipStream_t);
+  // part: kColumnAll, or kColumnImage / kColumnAdjoint for the two halves of
+  // the band-limited path (the fused kernels run whole under kColumnImage)
+  void (*image_adjoint)(const SocsArgs&, const ImageArgs&, int batch, hipStream_t, int part);
   void (*adjoint_tail)(const SocsArgs&, const GradArgs&, int batch, hipStream_t);
 };
 
@@ -86,6 +90,8 @@ int first_radix(int n);
 // fft::stage_tables), computed in FP64 and rounded to FP32; empty if n has a
 // single stage.
 std::vector<float2> row_stage_twiddles(int n);
+// Pass twiddle tables of the register-resident length-L transforms (fft_regs.hpp).
+std::vector<float2> corr_twiddles(int l);
 void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s);
 
 }  // namespace litho

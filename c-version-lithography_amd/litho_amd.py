@@ -484,8 +484,9 @@ class Plan:
         return pe, gn
 
     def profile_eval(self, theta, target, threshold=0.25, alpha=50.0, repeats=10, batch=1):
-        """HIP-event stage times (ms): forward half, fused column kernel, adjoint tail, total."""
-        out = np.zeros(4)
+        """HIP-event stage times (ms): forward half, column image, column adjoint
+        (the correlation adjoint kernel alone), adjoint tail, total."""
+        out = np.zeros(5)
         check(self.lib.litho_profile_ctm_eval(self.h, batch, ptr(theta), ptr(target), threshold, alpha, repeats,
                                               _dp(out)))
         return out

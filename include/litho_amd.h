@@ -218,8 +218,11 @@ size_t litho_server_request_bytes(const litho_server* server, int plan_index);
 
 /* Timing of one CTM grad-eval by HIP events on the context stream (no reference
  * counterpart; the reference times with std::chrono, demo_CTM.cpp:72-98).
- * stage_ms[4] = mean over `repeats` of: forward half (3 kernels), fused column
- * image/adjoint kernel, adjoint tail (3 kernels), whole evaluation. */
+ * stage_ms[5] = mean over `repeats` of: forward half (row / column transforms
+ * of the mask), column image (power spectra, autocorrelation and image kernel
+ * of the band-limited path, or the whole fused column kernel), column adjoint
+ * (the correlation adjoint k_corr_adjoint alone; 0 on the fused path),
+ * adjoint tail (row / column transforms back to the mask), whole evaluation. */
 int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, const float* target, double threshold,
                            double alpha, int repeats, double* stage_ms);
 

@@ -111,3 +111,31 @@ def test_batch_across_contexts_equals_single():
         p.close()
     for c in ctxs:
         c.close()
+
+
+def test_band_limited_batch_equals_single_tile(gpu_ctx):
+    """At n = 640, K = 24 a single tile splits its power spectra and its
+    correlation adjoint over 8 k groups to fill the chip, where a batch of 6
+    tiles would split them differently; the aerial images, gradients and PE of
+    the batched evaluation must still equal the single-tile ones bit for bit."""
+    import torch
+    from test_gpu_socs import window_kernels
+    la = litho_amd()
+    n, k, b = 640, 24, 6
+    plan = la.Plan.from_socs(gpu_ctx, window_kernels(n, k), capacity=b)
+    tg = targets()
+    tgt = np.stack([tg.manhattan_target(n, 40 + i) for i in range(b)]).astype(np.float32)
+    rng = np.random.default_rng(3)
+    mask = np.clip(tgt + 0.3 * rng.standard_normal(tgt.shape), 0, 1).astype(np.float32)
+    print(f"case band-limited batch n={n} k={k} batch={b}", flush=True)
+    a_b = torch.zeros((b, n, n), dtype=torch.float32, device="cuda")
+    g_b = torch.zeros_like(a_b)
+    pe_b, _ = plan.pe_gradient(dev(mask), dev(tgt), grad_mask=g_b, aerial=a_b, threshold=0.225, alpha=50.0, batch=b)
+    for i in range(b):
+        a1 = torch.zeros((1, n, n), dtype=torch.float32, device="cuda")
+        g1 = torch.zeros_like(a1)
+        pe1, _ = plan.pe_gradient(dev(mask[i]), dev(tgt[i]), grad_mask=g1, aerial=a1, threshold=0.225, alpha=50.0)
+        assert torch.equal(a1[0], a_b[i])
+        assert torch.equal(g1[0], g_b[i])
+        assert pe1[0] == pe_b[i]
+    plan.close()

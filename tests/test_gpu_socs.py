@@ -75,8 +75,25 @@ def window_kernels(n, k):
     return op.SocsKernels(n=n, win_y0=y0, win_x0=x0, kernels=kern, weights=w, weight_sum=wsum)
 
 
+def wide_window_kernels(n, k, w):
+    """Random complex kernels on a centred w x w window: w = 300 at n = 640
+    needs correlations of length L = 1024 >= 2 w - 1 in the band-limited
+    column path (the true optics windows of the supported tiles stay <= 512)."""
+    rng = np.random.default_rng(n + k + w)
+    y0 = x0 = n // 2 - w // 2
+    kern = rng.standard_normal((k, w, w)) + 1j * rng.standard_normal((k, w, w))
+    wts = np.sort(rng.random(k) + 0.05)[::-1]
+    wsum = float(np.sum(wts))
+    dc = kern[:, n // 2 - y0, n // 2 - x0]
+    kern = kern / np.sqrt(np.sum(wts * np.abs(dc) ** 2) / wsum)
+    return op.SocsKernels(n=n, win_y0=y0, win_x0=x0, kernels=kern, weights=wts, weight_sum=wsum)
+
+
 def make_case(kind, n, k=24, seed=0):
-    if kind == "synthetic":
+    if kind == "wide":
+        s = wide_window_kernels(n, k, 300)
+        cache = im.Cache.from_socs(s)
+    elif kind == "synthetic":
         s = Synth(n)
         cache = s.cache
     elif kind == "window":
@@ -91,7 +108,8 @@ def make_case(kind, n, k=24, seed=0):
 
 
 CASES = [("synthetic", 20, 2), ("synthetic", 24, 2), ("socs", 64, 8), ("socs", 128, 24), ("socs", 512, 24),
-         ("window", 384, 6), ("window", 640, 6), ("window", 896, 6), ("window", 1152, 6), ("window", 2176, 4)]
+         ("window", 384, 6), ("window", 640, 6), ("window", 896, 6), ("window", 1152, 6), ("window", 2176, 4),
+         ("wide", 640, 4)]
 
 
 @pytest.mark.parametrize("kind,n,k", CASES)
