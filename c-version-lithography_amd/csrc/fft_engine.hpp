@@ -276,14 +276,17 @@ struct Dft {
 };
 
 // ------------------------------------------------------------------ LDS layout
-// gfx950 serves a 64-bit LDS access 32 lanes at a time across 32 eight-byte
-// bank slots; lanes that hit the same slot modulo 32 at different addresses
-// serialise.  Element p of a line lives at swz<SH>(p) = p + (p >> SH): one pad
-// slot per 2^SH complex values, so strided Stockham writes land on distinct
-// slots.  Line strides are padded_stride<SH>(N) = N + N/2^SH + 4.  The shift is
-// chosen per kernel family from a replay of its accesses under this bank
-// model: 4 for the column kernels (lanes spread across 8 lines), 5 for the row
-// kernels (lanes along one line).
+// gfx950 serves a 64-bit LDS access (each address of a read2 / write2 on its
+// own) 16 lanes at a time over 16 eight-byte bank slots; lanes of a group that
+// hit the same slot modulo 16 at different addresses serialise (measured with
+// SQ_LDS_BANK_CONFLICT on probe patterns).  Element p of a line lives at
+// swz<SH>(p) = p + (p >> SH): one pad slot per 2^SH complex values, so strided
+// Stockham writes spread over the slots.  Line strides are
+// padded_stride<SH>(N) = N + N/2^SH + 4.  The shift is 4 for the column kernels
+// (lanes spread across lines) and 5 for the row kernels (lanes along one
+// line); row shifts 3, 4 or 6, column shift 5 and pads of 1 or 2 instead of 4
+// time the same within 1 % on the 640 ... 2048 tiles, whose transforms are not
+// bound by these conflicts.
 template <int SH>
 __host__ __device__ constexpr int swz(int p) { return p + (p >> SH); }
 template <int SH>

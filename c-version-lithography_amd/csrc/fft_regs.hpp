@@ -11,10 +11,13 @@
 // after its reads of that buffer.
 //
 // Lanes run across lines (lane = line + LINES * jj), so the global loads and
-// stores of a wave cover LINES consecutive columns of a row.  LDS element p of
-// a line lives at p + (p >> 3); the line stride LS is congruent to J modulo 32,
-// J = 32 / min(LINES, 32) being the number of j values in a 32-lane half-wave,
-// so the half-wave's lanes fall on distinct 8-byte bank slots in every pass.
+// stores of a wave cover LINES consecutive columns of a row.  gfx950 serves a
+// 64-bit LDS access (each address of a read2 / write2 separately) 16 lanes at
+// a time over 16 eight-byte bank slots (measured with SQ_LDS_BANK_CONFLICT on
+// probe patterns).  LDS element p of a line lives at p + (p >> 3); the line
+// stride LS is congruent to J modulo 16, J = 16 / min(LINES, 16) being the
+// number of j values among 16 consecutive lanes, so those lanes fall on
+// distinct slots in every pass.
 #pragma once
 #include "fft_engine.hpp"
 
@@ -58,10 +61,10 @@ constexpr RegPlan reg_plan(int l) {
 }
 
 constexpr int reg_line_stride(int l, int lines) {
-  const int g = lines < 32 ? lines : 32;
-  const int j = 32 / g;
+  const int g = lines < 16 ? lines : 16;
+  const int j = 16 / g;
   int ls = l + l / 8;
-  while (ls % 32 != j % 32) ++ls;
+  while (ls % 16 != j % 16) ++ls;
   return ls;
 }
 

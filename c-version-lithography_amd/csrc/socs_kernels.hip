@@ -38,6 +38,7 @@
 #include "opt_kernels.hpp"
 #include "socs_kernels.hpp"
 
+
 namespace litho {
 
 using fft::cconj;
