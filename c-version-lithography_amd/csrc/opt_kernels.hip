@@ -89,6 +89,10 @@ __device__ __forceinline__ double dot4(float4 a, float4 b) {
   return fma((double)a.x, (double)b.x, fma((double)a.y, (double)b.y, fma((double)a.z, (double)b.z, (double)a.w * (double)b.w)));
 }
 __device__ __forceinline__ float4 sub4(float4 a, float4 b) { return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+// per-component select (a float4 ternary can become a private array indexed by the condition)
+__device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b) {
+  return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
 
 // L-BFGS pair update.  For tile b: s_new = x - xp, y_new = g - gp written to
 // slot new_slot of the history; FP64 dot products per block, NV = 4 (MB + 1):
@@ -97,14 +101,20 @@ __device__ __forceinline__ float4 sub4(float4 a, float4 b) { return make_float4(
 // position 4 MB.  With shift set, xp <- x and gp <- g afterwards (the accepted
 // point becomes the base of the next line search).  Vectors are read as
 // float4: n is a multiple of 4 for every supported tile (N^2 with N even).
-// At most 128 VGPRs (four waves per SIMD): the kOptBlocks x 4 waves of a tile
-// then run in one round; the 4 (MB + 1) FP64 accumulators alone take 88 at MB = 10.
+//
+// The two halves of a wave take the same 32 elements: lanes 0-31 (h = 0) the
+// s side (s_j and s_new dots, s written, xp shifted), lanes 32-63 (h = 1) the
+// y side.  Both halves load x, xp, g, gp at the same addresses in the same
+// instruction, so HBM reads each element once, while each lane keeps only
+// 2 (MB + 1) FP64 accumulators and has registers for more loads in flight.
 template <int MB>
 __global__ void __launch_bounds__(kOptThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* __restrict__ g,
              float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy, const int* __restrict__ need,
              const int* __restrict__ meta, size_t n, int shift, double* __restrict__ partials) {
   constexpr int NV = 4 * (MB + 1);
+  constexpr int NA = 2 * (MB + 1);
+  constexpr int WAVES = kOptThreads / 64;
   const int b = blockIdx.y;
   if (need[b] == 0) return;
   const int m = meta[b * kMetaStride + 0];
@@ -112,52 +122,58 @@ k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* _
   const int* slots = meta + b * kMetaStride + 2;
   const size_t tb = (size_t)b * n;
   const size_t hist = hist_tile_stride(n);
-  float* hs_b = hs + (size_t)b * hist;
-  float* hy_b = hy + (size_t)b * hist;
-  double acc[NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5;
+  // this half's history (s or y) of tile b, as float4
+  const float4* hv = reinterpret_cast<const float4*>((h ? hy : hs) + (size_t)b * hist);
+  float4* wn = reinterpret_cast<float4*>((h ? hy : hs) + (size_t)b * hist + (size_t)new_slot * hist_slot_stride(n));
+  size_t off[MB > 0 ? MB : 1];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  const float4* sp[MB > 0 ? MB : 1];
-  const float4* yp[MB > 0 ? MB : 1];
-#pragma unroll
-  for (int j = 0; j < MB; ++j) {
-    const int sl = j < m ? slots[j] : 0;
-    sp[j] = reinterpret_cast<const float4*>(hs_b + (size_t)sl * hist_slot_stride(n));
-    yp[j] = reinterpret_cast<const float4*>(hy_b + (size_t)sl * hist_slot_stride(n));
-  }
-  float4* sn = reinterpret_cast<float4*>(hs_b + (size_t)new_slot * hist_slot_stride(n));
-  float4* yn = reinterpret_cast<float4*>(hy_b + (size_t)new_slot * hist_slot_stride(n));
+  for (int j = 0; j < MB; ++j) off[j] = (size_t)(j < m ? slots[j] : 0) * (hist_slot_stride(n) / 4);
   const float4* x4 = reinterpret_cast<const float4*>(x + tb);
   float4* xp4 = reinterpret_cast<float4*>(xp + tb);
   const float4* g4 = reinterpret_cast<const float4*>(g + tb);
   float4* gp4 = reinterpret_cast<float4*>(gp + tb);
+  float4* shift4 = h ? gp4 : xp4;
+  double acc[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) acc[v] = 0.0;
   const size_t n4 = n / 4;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+  const size_t stride = (size_t)gridDim.x * WAVES * 32;
+  for (size_t i = ((size_t)blockIdx.x * WAVES + wave) * 32 + (lane & 31); i < n4; i += stride) {
     const float4 xv = x4[i], xpv = xp4[i], gv = g4[i], gpv = gp4[i];
-    const float4 s = sub4(xv, xpv);
     const float4 y = sub4(gv, gpv);
-    sn[i] = s;
-    yn[i] = y;
-    if (shift) {
-      xp4[i] = xv;
-      gp4[i] = gv;
-    }
+    const float4 w = sel4(h != 0, y, sub4(xv, xpv));
+    wn[i] = w;
+    if (shift) shift4[i] = sel4(h != 0, gv, xv);
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
       if (j < m) {
-        const float4 sj = sp[j][i], yj = yp[j][i];
-        acc[4 * j + 0] += dot4(sj, y);
-        acc[4 * j + 1] += dot4(yj, y);
-        acc[4 * j + 2] += dot4(sj, gv);
-        acc[4 * j + 3] += dot4(yj, gv);
+        const float4 vj = hv[off[j] + i];
+        acc[2 * j + 0] += dot4(vj, y);
+        acc[2 * j + 1] += dot4(vj, gv);
       }
     }
-    acc[4 * MB + 0] += dot4(s, y);
-    acc[4 * MB + 1] += dot4(y, y);
-    acc[4 * MB + 2] += dot4(s, gv);
-    acc[4 * MB + 3] += dot4(y, gv);
+    acc[2 * MB + 0] += dot4(w, y);
+    acc[2 * MB + 1] += dot4(w, gv);
   }
-  block_reduce_store<NV>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x) * NV);
+  // fixed-order reduction: each half over its 32 lanes, then the waves in order
+  __shared__ double red[WAVES][2][NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) {
+    double t = acc[v];
+    for (int o = 16; o > 0; o >>= 1) t += __shfl_down(t, o, 32);
+    if ((lane & 31) == 0) red[wave][h][v] = t;
+  }
+  __syncthreads();
+  double* out = partials + ((size_t)b * gridDim.x + blockIdx.x) * NV;
+  for (int v = threadIdx.x; v < NV; v += blockDim.x) {
+    // v = 4 j + 2 t + hv: half hv, accumulator 2 j + t (j = MB: the new pair)
+    const int jj = v >> 2, t = (v >> 1) & 1, hh = v & 1;
+    double sum = 0.0;
+    for (int w2 = 0; w2 < WAVES; ++w2) sum += red[w2][hh][2 * jj + t];
+    out[v] = sum;
+  }
 }
 
 // d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB active slots,

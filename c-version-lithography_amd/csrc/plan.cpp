@@ -125,7 +125,8 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
       p->ghat.alloc(c * wr * N * sizeof(float2));
       p->spart.alloc(corr_power_bytes(n, p->corr_len, k, capacity));
     }
-    p->img_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
+    p->img_blocks = p->corr_len > 0 ? ops->img_blocks() : p->col_blocks;
+    p->img_partials.alloc(c * p->img_blocks * 2 * sizeof(double));
     p->grad_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
     p->red.alloc(c * 4 * sizeof(double));
     p->coef.alloc(c * sizeof(float));
@@ -154,7 +155,7 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   SocsArgs a2 = a;
   if (!rq.adjoint) a2.v = nullptr;
   plan.ops->image_adjoint(a2, im, batch, s, kColumnAll);
-  reduce_pairs(plan.img_partials.as<double>(), plan.col_blocks, plan.red.as<double>(), 4, batch, s);
+  reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, plan.red.as<double>(), 4, batch, s);
   if (rq.adjoint && rq.im.target != nullptr) {
     GradArgs ga = rq.ga;
     ga.partials = plan.grad_partials.as<double>();

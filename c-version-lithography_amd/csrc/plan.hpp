@@ -96,6 +96,7 @@ struct Plan {
   double weight_sum = 0.0;
   const SocsOps* ops = nullptr;
   int col_blocks = 0;
+  int img_blocks = 0;   // image-kernel blocks (FP64 partials of PE and penalty)
   DevMem h, tw, rtw, row_freq, col_freq, row_of_u, col_of_u;
   int pruned = 0;                            // window-pruned column kernels in use
   int row_pruned = 0;                        // window-pruned row kernels in use

@@ -147,6 +147,29 @@ struct ColCfg {
   static constexpr int MAX_WAVES_PER_EU = LITHO_COL_OCC_RULE == 1 ? 8 : (NT >= 512 ? NT / 256 : 1);
 };
 
+// Block shape of the band-limited image kernel (one length-N inverse and one
+// forward per column, stage tables in LDS), tuned apart from the column
+// kernels (litho_bench evalb, image stage): 8 columns x 1024 threads for the
+// powers of two from 1024 (2048, b = 1: 0.065 ms against 0.081 with 4 x 512),
+// 4 x 512 for the other sizes from 1024 (1152, b = 8: 0.159 against 0.172),
+// 8 x 512 from 640 (640, b = 8: 0.073 against 0.079 with 4 x 512).
+template <int N>
+struct ImgCfg {
+  using CC = ColCfg<N>;
+  static constexpr bool OWN = !CC::BIG_ODD && N >= 640;
+  static constexpr bool POW2 = (N & (N - 1)) == 0;
+  static constexpr int LINES = !OWN ? CC::LINES : (N >= 1024 ? (POW2 ? 8 : 4) : 8);
+  static constexpr int NT = !OWN ? CC::NT : (N >= 1024 ? (POW2 ? 1024 : 512) : 512);
+  static constexpr int SH = CC::SH;
+  static constexpr int LS = fft::padded_stride<SH>(N);
+  static constexpr int ELEMS = LINES * N;
+  static constexpr int PER = (ELEMS + NT - 1) / NT;
+  static constexpr int TW = fft::staged_entries(N);
+  static constexpr size_t LDS_BYTES = (size_t)LINES * LS * 8 + (size_t)(TW > 0 ? TW : 1) * 8 + (size_t)NT / 64 * 16;
+  static constexpr int BLOCKS_PER_CU = 160 * 1024 / LDS_BYTES > 0 ? (int)(160 * 1024 / LDS_BYTES) : 1;
+  static constexpr int WAVES = BLOCKS_PER_CU * NT / 256 > 8 ? 8 : (BLOCKS_PER_CU * NT / 256 > 0 ? BLOCKS_PER_CU * NT / 256 : 1);
+};
+
 // Twiddle sources of the pruned column kernels: the N-entry table (LDS copy or
 // global) and the middle-stage tables in LDS.  The opaque zero offset taken
 // per call stops the compiler from hoisting the (loop-invariant) twiddle loads
@@ -994,10 +1017,10 @@ k_corr_finish(SocsArgs a, const float* __restrict__ spart, int groups) {
 
 // K4b: I = IDFT_N(c) per column, resist / losses / dL/dI, ghat = DFT_N(g) on [0, W_r)
 template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::MIN_WAVES_PER_EU, ColCfg<N>::MAX_WAVES_PER_EU)))
+__global__ void __launch_bounds__(ImgCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ImgCfg<N>::WAVES, 8)))
 k_col_image_corr(SocsArgs a, ImageArgs im) {
-  using C = ColCfg<N>;
-  constexpr int TW = fft::staged_entries(N);
+  using C = ImgCfg<N>;
+  constexpr int TW = C::TW;
   __shared__ float2 buf[C::LINES * C::LS];
   __shared__ double red[2 * C::NT / 64];
   __shared__ float2 tw[TW > 0 ? TW : 1];
@@ -1263,6 +1286,7 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
 template <int N>
 struct SocsImpl {
   static int col_blocks() { return (N + ColCfg<N>::LINES - 1) / ColCfg<N>::LINES; }
+  static int img_blocks() { return (N + ImgCfg<N>::LINES - 1) / ImgCfg<N>::LINES; }
 
   static void forward_half(const SocsArgs& a, int batch, int mode, const float* base, const float* dir,
                            const float* coef, float* theta_out, float* mask_out, hipStream_t s) {
@@ -1316,7 +1340,7 @@ struct SocsImpl {
     if (a.corr_len > 0) {
       if (part != kColumnAdjoint) {
         launch_corr_any(0, a, batch, a.spart, s);
-        hipLaunchKernelGGL(k_col_image_corr<N>, g, dim3(C::NT), 0, s, a, im);
+        hipLaunchKernelGGL(k_col_image_corr<N>, dim3(img_blocks(), batch), dim3(ImgCfg<N>::NT), 0, s, a, im);
       }
       if (part != kColumnImage && im.target != nullptr && a.v != nullptr) launch_corr_any(1, a, batch, nullptr, s);
       return;
@@ -1368,7 +1392,7 @@ void reduce_pairs(const double* partials, int nblk, double* out, int out_stride,
 
 template <int N>
 static const SocsOps* make_ops() {
-  static const SocsOps ops = {N, &SocsImpl<N>::col_blocks, &SocsImpl<N>::forward_half,
+  static const SocsOps ops = {N, &SocsImpl<N>::col_blocks, &SocsImpl<N>::img_blocks, &SocsImpl<N>::forward_half,
                               &SocsImpl<N>::image_adjoint, &SocsImpl<N>::adjoint_tail};
   return &ops;
 }

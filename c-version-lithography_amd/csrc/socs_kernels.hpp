@@ -57,7 +57,7 @@ struct ImageArgs {
   const float* pen_win;           // null = no background penalty
   float pe_weight, penalty, pen_thr;
   float* dldi;                    // optional debug output
-  double* partials;               // [b][col_blocks][2]: PE, background penalty
+  double* partials;               // [b][image blocks][2]: PE, background penalty
 };
 
 struct GradArgs {
@@ -75,6 +75,7 @@ enum ColumnPart { kColumnAll = 0, kColumnImage = 1, kColumnAdjoint = 2 };
 struct SocsOps {
   int n;
   int (*col_blocks)();
+  int (*img_blocks)();   // blocks of the band-limited image kernel (its FP64 partials)
   void (*forward_half)(const SocsArgs&, int batch, int mode, const float* base, const float* dir,
                        const float* coef, float* theta_out, float* mask_out, hipStream_t);
   // part: kColumnAll, or kColumnImage / kColumnAdjoint for the two halves of
