@@ -248,7 +248,7 @@ struct GdPool final : CtmPool::Impl {
       if (job[b].out_theta != nullptr) copy_tile(job[b].out_theta, x.as<float>() + b * n2, n2, s);
       finish(b);
     }
-    LITHO_HIP(hipStreamSynchronize(s));   // the staged flags are rewritten next step
+    spin_wait(s);   // the staged flags are rewritten next step
   }
 };
 
@@ -505,7 +505,7 @@ struct LbfgsPool final : CtmPool::Impl {
       copy_tile(gp.as<float>() + b * n2, g.as<float>() + b * n2, n2, s);
       begin_line_search(b, -st[b].gnorm * st[b].gnorm);
     }
-    LITHO_HIP(hipStreamSynchronize(s));   // the staged flags / steps are rewritten next round
+    spin_wait(s);   // the staged flags / steps are rewritten next round
   }
 
   void step() override {

@@ -190,5 +190,7 @@ void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, dou
 
 // Copy plan.red of `batch` tiles to the pinned host mirror and wait.
 const double* fetch_reductions(Plan& plan, int batch);
+// Waits for every command on the stream by polling it.
+void spin_wait(hipStream_t s);
 
 }  // namespace litho
