@@ -1189,7 +1189,13 @@ static int corr_groups(int n, int l, int k, int batch) {
 // whatever the batch, so a tile's image does not depend on the tiles it is
 // batched with: the FP32 partial sums depend on the grouping.  The adjoint's
 // groups only split the independent V_k and follow the batch.
-static int corr_power_groups(int n, int l, int k) { return corr_groups(n, l, k, 1); }
+static int corr_power_groups(int n, int l, int k) {
+  // at most 4: each group adds a partial spectrum per column to write and sum
+  // (640^2 alone: 0.113 ms per eval with 4 groups against 0.114 with the
+  // model's 8; 896^2, batch 32: 1.277 against 1.333 ms)
+  const int g = corr_groups(n, l, k, 1);
+  return g < 4 ? g : 4;
+}
 
 template <int L>
 static void launch_corr(int stage, const SocsArgs& a, int batch, float* spart, hipStream_t s) {
