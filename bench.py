@@ -73,19 +73,29 @@ def corr_adjoint_bytes(n, k, w):
     return 16 * k * w * n + 8 * w * n
 
 
-ROOF_KERNEL = f"k_corr_adjoint<{corr_len(W)}>"
-TRAFFIC_FILE = ROOT / "profiles" / f"k_corr_adjoint_{corr_len(W)}_2048_hbm_traffic.md"
-TRAFFIC_KEY = f"traffic_bytes_per_launch {ROOF_KERNEL}:"
+def pair_kernel_bytes(n, m):
+    """Compulsory HBM bytes of one L-BFGS pair-kernel launch: x, xp, g, gp and
+    the 2m history vectors read, s, y and the shifted xp, gp written (N^2
+    floats each)."""
+    return (2 * m + 8) * 4 * n * n
+
+
+HISTORY = 10
+PAIR_KERNEL = f"k_lbfgs_pair<{HISTORY}>"
+EVAL_KERNEL = f"k_corr_adjoint<{corr_len(W)}>"
+PAIR_TRAFFIC_FILE = ROOT / "profiles" / f"k_lbfgs_pair_{HISTORY}_2048_hbm_traffic.md"
+EVAL_TRAFFIC_FILE = ROOT / "profiles" / f"k_corr_adjoint_{corr_len(W)}_2048_hbm_traffic.md"
 FP32_PEAK = 157.3e12
 
 
-def measured_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+def measured_traffic(path, kernel):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None."""
+    key = f"traffic_bytes_per_launch {kernel}:"
     try:
-        for line in TRAFFIC_FILE.read_text().splitlines():
-            if line.startswith(TRAFFIC_KEY):
-                return int(line[len(TRAFFIC_KEY):].strip())
+        for line in path.read_text().splitlines():
+            if line.startswith(key):
+                return int(line[len(key):].strip())
     except OSError:
         pass
     return None
@@ -176,8 +186,11 @@ def main():
     adj_flops = corr_adjoint_flops(N, K, W)
     adj_bytes = corr_adjoint_bytes(N, K, W)
     eval_bytes = algorithmic_bytes_per_eval(N, K, W)
+    # the pair kernel alone (HIP events on the library stream), one launch per L-BFGS iteration
+    pair_ms = ctx.profile_lbfgs_pair(N * N, HISTORY, 20)
+    pair_bytes = pair_kernel_bytes(N, HISTORY)
     print(f"stages (ms): forward {st[0]:.3f} column image {st[1]:.3f} column adjoint {st[2]:.3f} "
-          f"adjoint-tail {st[3]:.3f} eval {st[4]:.3f}", file=sys.stderr, flush=True)
+          f"adjoint-tail {st[3]:.3f} eval {st[4]:.3f}; {PAIR_KERNEL} {pair_ms:.4f}", file=sys.stderr, flush=True)
     line = {
         "metric": "CTM grad-evals/s (SOCS fwd+adjoint) 2048^2, K=24",
         "value": round(value, 3), "unit": "grad-evals/s", "n_gpus": 1, "steps": args.steps,
@@ -186,15 +199,25 @@ def main():
         "config": {"workload": "single-tile CTM L-BFGS, 2048x2048 @4nm, K=24 (219x219 window), m=10, "
                                "50 iterations, tol 1e-2, seeded Manhattan target",
                    "n": N, "k": K, "window": W, "evals_per_step": round(evals / args.steps, 2)},
-        # The dominant kernel is FP32 vector (VALU) FFT arithmetic, priced against
-        # MI355X's published FP32 peak (packed FP32 is not used, so the issue
-        # ceiling is half of it); its compulsory HBM bytes are reported beside.
-        "roofline": {"bound": "mfma", "kernel": ROOF_KERNEL, "compute": "fp32 vector FFT butterflies (no MFMA)",
-                     "achieved": round(adj_flops / adj_s / 1e12, 2), "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": round(adj_flops / adj_s / FP32_PEAK, 4), "traffic": measured_traffic(),
-                     "algorithmic_flops": adj_flops, "launch_ms": round(st[2], 4),
-                     "hbm": {"algorithmic_bytes": adj_bytes, "achieved_GBps": round(adj_bytes / adj_s / 1e9, 1),
-                             "frac": round(adj_bytes / adj_s / HBM_PEAK, 4)}},
+        # The kernel with the most time in the timed solves: one pair-kernel
+        # launch per L-BFGS iteration, HBM-bound.
+        "roofline": {"bound": "hbm", "kernel": PAIR_KERNEL,
+                     "achieved": round(pair_bytes / (pair_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(pair_bytes / (pair_ms * 1e-3) / HBM_PEAK, 4),
+                     "traffic": measured_traffic(PAIR_TRAFFIC_FILE, PAIR_KERNEL), "algorithmic_bytes": pair_bytes,
+                     "launch_ms": round(pair_ms, 4)},
+        # The dominant kernel of a grad-eval, bound by FP32 vector (VALU) FFT
+        # arithmetic, priced against MI355X's published FP32 peak (packed FP32
+        # is not used, so the issue ceiling is half of it); its compulsory HBM
+        # bytes are reported beside.
+        "eval_kernel_roofline": {"bound": "fp32 valu", "kernel": EVAL_KERNEL,
+                                 "achieved": round(adj_flops / adj_s / 1e12, 2), "peak": FP32_PEAK / 1e12,
+                                 "unit": "TFLOP/s", "frac": round(adj_flops / adj_s / FP32_PEAK, 4),
+                                 "traffic": measured_traffic(EVAL_TRAFFIC_FILE, EVAL_KERNEL),
+                                 "algorithmic_flops": adj_flops, "launch_ms": round(st[2], 4),
+                                 "hbm": {"algorithmic_bytes": adj_bytes,
+                                         "achieved_GBps": round(adj_bytes / adj_s / 1e9, 1),
+                                         "frac": round(adj_bytes / adj_s / HBM_PEAK, 4)}},
         "eval_roofline": {"algorithmic_bytes": eval_bytes, "eval_ms": round(st[4], 4),
                           "achieved_GBps": round(eval_bytes / (st[4] * 1e-3) / 1e9, 1),
                           "frac": round(eval_bytes / (st[4] * 1e-3) / HBM_PEAK, 4)},

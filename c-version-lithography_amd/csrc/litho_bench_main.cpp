@@ -49,7 +49,7 @@ std::vector<float> lattice_target(int n) {
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s eval N K REPEATS | ctm N K ITERS | copy MIB\n", argv[0]);
+    std::fprintf(stderr, "usage: %s eval N K REPEATS | ctm N K ITERS | copy MIB | pair N M REPEATS\n", argv[0]);
     return 2;
   }
   const std::string mode = argv[1];
@@ -62,6 +62,18 @@ int main(int argc, char** argv) {
     for (int i = 0; i < 3; ++i) hip_check(hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
     hip_check(hipDeviceSynchronize(), "sync");
     std::printf("copied %zu bytes x3\n", bytes);
+    return 0;
+  }
+  if (mode == "pair") {
+    // litho_bench pair N M REPEATS: the L-BFGS pair kernel over one N x N tile
+    if (argc < 5) return 2;
+    const size_t nn = (size_t)std::atol(argv[2]) * (size_t)std::atol(argv[2]);
+    litho_ctx* pctx = nullptr;
+    check(litho_ctx_create(0, &pctx), "litho_ctx_create");
+    double ms = 0.0;
+    check(litho_profile_lbfgs_pair(pctx, nn, std::atoi(argv[3]), std::atoi(argv[4]), &ms), "litho_profile_lbfgs_pair");
+    std::printf("pair n=%s m=%s ms=%.4f\n", argv[2], argv[3], ms);
+    check(litho_ctx_destroy(pctx), "litho_ctx_destroy");
     return 0;
   }
   if (argc < 5) return 2;

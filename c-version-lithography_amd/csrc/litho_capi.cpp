@@ -12,6 +12,7 @@
 #include "fullchip.hpp"
 #include "lsm.hpp"
 #include "optics.hpp"
+#include "opt_kernels.hpp"
 #include "plan.hpp"
 #include "server.hpp"
 #include "tiling.hpp"
@@ -454,6 +455,53 @@ int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, cons
     rq.ga.scale = (float)(4.0 * alpha);
     rq.ga.theta = theta;
     litho::profile_eval(p, batch, rq, repeats, stage_ms);
+  });
+}
+
+int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, double* ms) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    require(ms != nullptr && repeats > 0, "null argument or non-positive repeats");
+    require(n > 0 && n % 4 == 0, "n must be a positive multiple of 4");
+    require(m >= 1 && m <= litho::kMaxHist, "history m out of range");
+    hipStream_t s = c.stream;
+    litho::DevMem x, xp, g, gp, hs, hy, need, meta, part;
+    for (litho::DevMem* v : {&x, &xp, &g, &gp}) {
+      v->alloc(n * sizeof(float));
+      LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
+    }
+    for (litho::DevMem* v : {&hs, &hy}) {
+      v->alloc(litho::hist_tile_stride(n) * sizeof(float));
+      LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
+    }
+    // one tile with a full history of m pairs in slots 0 .. m-1, the spare slot m
+    std::vector<int> hm(litho::kMetaStride, 0);
+    hm[0] = m;
+    hm[1] = m;
+    for (int j = 0; j < m; ++j) hm[2 + j] = j;
+    const int one = 1;
+    need.alloc(sizeof(int));
+    meta.alloc(hm.size() * sizeof(int));
+    part.alloc((size_t)litho::kOptBlocks * litho::kPairDots * sizeof(double));
+    LITHO_HIP(hipMemcpyAsync(need.ptr, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    LITHO_HIP(hipMemcpyAsync(meta.ptr, hm.data(), meta.bytes, hipMemcpyHostToDevice, s));
+    auto launch = [&] {
+      litho::launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(),
+                               hy.as<float>(), need.as<int>(), meta.as<int>(), n, 1, m, true, part.as<double>(), s);
+    };
+    launch();   // untimed: loads the code object
+    hipEvent_t e0, e1;
+    LITHO_HIP(hipEventCreate(&e0));
+    LITHO_HIP(hipEventCreate(&e1));
+    LITHO_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < repeats; ++r) launch();
+    LITHO_HIP(hipEventRecord(e1, s));
+    LITHO_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    LITHO_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / repeats;
   });
 }
 

@@ -139,6 +139,7 @@ SIGNATURES = {
     "litho_server_request_bytes": (C.c_size_t, [_P, _I]),
     "litho_initialize_theta": (_I, [_P, _P, _P, C.c_size_t]),
     "litho_profile_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _I, _DP]),
+    "litho_profile_lbfgs_pair": (_I, [_P, C.c_size_t, _I, _I, _DP]),
     "litho_ls_terms": (_I, [_P, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P]),
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
     "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
@@ -214,6 +215,12 @@ class Context:
 
     def set_graphs(self, enable: bool):
         check(self.lib.litho_ctx_set_graphs(self.h, 1 if enable else 0))
+
+    def profile_lbfgs_pair(self, n, m=10, repeats=20):
+        """HIP-event time (ms) of one L-BFGS pair-kernel launch over n floats with m pairs."""
+        ms = C.c_double(0.0)
+        check(self.lib.litho_profile_lbfgs_pair(self.h, n, m, repeats, C.byref(ms)))
+        return ms.value
 
     def __del__(self):
         try:

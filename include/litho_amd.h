@@ -226,6 +226,14 @@ size_t litho_server_request_bytes(const litho_server* server, int plan_index);
 int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, const float* target, double threshold,
                            double alpha, int repeats, double* stage_ms);
 
+/* Timing of the L-BFGS pair kernel alone (no reference counterpart; the
+ * reference's LBFGS++ forms s, y and its dot products on the host): one tile
+ * of n floats (a multiple of 4) with a full history of m pairs, the
+ * accepted-step variant that also shifts xp <- x and gp <- g (24 vectors read,
+ * 4 written).  *ms = mean over `repeats` back-to-back launches, by HIP events
+ * on the context stream. */
+int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, double* ms);
+
 /* CTM_Optimizer::initialize_theta (CTM_Optimizer.cpp:66-73): 0.3 pi where
  * pattern > 0, else 0.7 pi.  Device arrays of n*n floats. */
 int litho_initialize_theta(litho_ctx* ctx, const float* pattern, float* theta, size_t count);

@@ -187,3 +187,16 @@ def test_lbfgs_2048_final_pe(gpu_ctx, kernels2048):
     assert abs(r["final_pe"] - info["final_pe"]) <= 5e-3 * info["final_pe"]
     plan.close()
     im.set_threads(1)
+
+
+def test_profile_lbfgs_pair(gpu_ctx):
+    """The pair-kernel timing entry point runs a launch over one tile with a
+    full history and rejects sizes that are not float4-aligned."""
+    print("case profile_lbfgs_pair 64x64 m=3", flush=True)
+    ms = gpu_ctx.profile_lbfgs_pair(64 * 64, 3, 2)
+    assert ms > 0.0
+    la = litho_amd()
+    with pytest.raises(la.LithoError):
+        gpu_ctx.profile_lbfgs_pair(64 * 64 + 2, 3, 2)
+    with pytest.raises(la.LithoError):
+        gpu_ctx.profile_lbfgs_pair(64 * 64, 0, 2)
