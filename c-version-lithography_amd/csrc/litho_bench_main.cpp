@@ -59,9 +59,19 @@ int main(int argc, char** argv) {
     hip_check(hipMalloc(&a, bytes), "hipMalloc");
     hip_check(hipMalloc(&b, bytes), "hipMalloc");
     hip_check(hipMemset(a, 1, bytes), "hipMemset");
-    for (int i = 0; i < 3; ++i) hip_check(hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
-    hip_check(hipDeviceSynchronize(), "sync");
-    std::printf("copied %zu bytes x3\n", bytes);
+    hip_check(hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");   // untimed
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
+    for (int i = 0; i < 3; ++i) hip_check(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nullptr), "hipMemcpyAsync");
+    hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    // read + write bytes per copy
+    std::printf("copied %zu bytes x3, %.4f ms per copy, %.1f GB/s (read + write)\n", bytes, ms / 3,
+                2.0 * (double)bytes / (ms / 3 * 1e-3) / 1e9);
     return 0;
   }
   if (mode == "pair") {
