@@ -491,9 +491,23 @@ k_final_row_inv(SocsArgs a) {
     const int l = e / a.wc, j = e - l * a.wc;
     const int i = i0 + l;
     if (i < a.wr) {
+      // eight k per batch of loads, summed in increasing k
+      const float2* src = yk + (size_t)i * a.wc + j;
+      const size_t ks = (size_t)a.wr * a.wc;
       float2 s = make_float2(0.f, 0.f);
-      for (int k = 0; k < a.k; ++k) {
-        const float2 u = yk[((size_t)k * a.wr + i) * a.wc + j];
+      int k = 0;
+      for (; k + 8 <= a.k; k += 8) {
+        float2 w[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) w[t] = src[(size_t)(k + t) * ks];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          s.x += w[t].x;
+          s.y += w[t].y;
+        }
+      }
+      for (; k < a.k; ++k) {
+        const float2 u = src[(size_t)k * ks];
         s.x += u.x;
         s.y += u.y;
       }
@@ -661,8 +675,23 @@ k_final_row_inv_pruned(SocsArgs a) {
         const int i = i0 + l, j = a.col_of_u[u];
         float2 s = make_float2(0.f, 0.f);
         if (i >= a.wr || j < 0) return s;
-        for (int k = 0; k < a.k; ++k) {
-          const float2 w = yk[((size_t)k * a.wr + i) * a.wc + j];
+        // loads eight k at a time (one round trip per batch instead of per k),
+        // sums in increasing k as before
+        const float2* src = yk + (size_t)i * a.wc + j;
+        const size_t ks = (size_t)a.wr * a.wc;
+        int k = 0;
+        for (; k + 8 <= a.k; k += 8) {
+          float2 w[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) w[t] = src[(size_t)(k + t) * ks];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            s.x += w[t].x;
+            s.y += w[t].y;
+          }
+        }
+        for (; k < a.k; ++k) {
+          const float2 w = src[(size_t)k * ks];
           s.x += w.x;
           s.y += w.y;
         }
