@@ -150,6 +150,7 @@ struct CtmPool::Impl {
     rq.ga.scale = (float)(4.0 * opt.alpha);
     rq.ga.grad_theta = grad_theta;
     rq.active = pm.d_active();
+    rq.red_out = plan.host_red->dev_as<double>();   // read by wait_reductions, no copy
     return rq;
   }
   virtual bool admit(const CtmJob& j) = 0;
@@ -201,7 +202,7 @@ struct GdPool final : CtmPool::Impl {
     }
     pm.upload_active_step(s);
     graph->launch();
-    const double* red = fetch_reductions(plan, S);
+    const double* red = wait_reductions(plan);
     std::vector<int> done;
     bool update = false;
     for (int b = 0; b < S; ++b) {
@@ -254,7 +255,7 @@ struct GdPool final : CtmPool::Impl {
 
 // ------------------------------------------------------------------ L-BFGS
 struct LbfgsPool final : CtmPool::Impl {
-  DevMem x, xp, g, gp, d, hs, hy, pair_part, dir_part, vec_red, dg_red;
+  DevMem x, xp, g, gp, d, hs, hy, pair_part, dir_part, vec_red;
   // device-resident history of every slot: [m, spare, slots] ints, S'Y and
   // Y'Y (kMaxHist^2 doubles each) and the direction coefficients
   DevMem d_meta, d_sy, d_yy, d_coef;
@@ -269,7 +270,7 @@ struct LbfgsPool final : CtmPool::Impl {
   const int m_max;
 
   LbfgsPool(Plan& p, int slots, const CtmOptions& o)
-      : Impl(p, slots, o), hdg((size_t)slots * sizeof(double)), st(slots), pending_dg0(slots, 0),
+      : Impl(p, slots, o), hdg((size_t)slots * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent), st(slots), pending_dg0(slots, 0),
         m_max(o.history) {
     alloc_slots(x, S, n2);
     alloc_slots(xp, S, n2);
@@ -281,7 +282,6 @@ struct LbfgsPool final : CtmPool::Impl {
     pair_part.alloc((size_t)S * kOptBlocks * kPairDots * sizeof(double));
     dir_part.alloc((size_t)S * kOptBlocks * sizeof(double));
     vec_red.alloc((size_t)S * kPairDots * sizeof(double));
-    dg_red.alloc((size_t)S * sizeof(double));
     d_meta.alloc((size_t)S * kMetaStride * sizeof(int));
     d_sy.alloc((size_t)S * kMaxHist * kMaxHist * sizeof(double));
     d_yy.alloc((size_t)S * kMaxHist * kMaxHist * sizeof(double));
@@ -393,8 +393,8 @@ struct LbfgsPool final : CtmPool::Impl {
                         d_yy.as<double>(), d_coef.as<double>(), s);
     launch_lbfgs_direction(g.as<float>(), hs.as<float>(), hy.as<float>(), pm.d_need(), d_meta.as<int>(),
                            d_coef.as<double>(), n2, d.as<float>(), S, mb_dir, dir_part.as<double>(), s);
-    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, dg_red.as<double>(), 1, S, s);
-    LITHO_HIP(hipMemcpyAsync(hdg.ptr, dg_red.ptr, (size_t)S * sizeof(double), hipMemcpyDeviceToHost, s));
+    // g . d straight into mapped host memory (read after the evaluation's wait)
+    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, hdg.dev_as<double>(), 1, S, s);
   }
 
   // (B) one batched evaluation of every slot with a pending trial
@@ -414,7 +414,7 @@ struct LbfgsPool final : CtmPool::Impl {
     pm.upload_active_step(s);
     if (directions) launch_directions();
     graph->launch();
-    const double* red = fetch_reductions(plan, S);
+    const double* red = wait_reductions(plan);
     std::vector<int> started;
     for (int b = 0; b < S; ++b) {
       if (!flags[b]) continue;

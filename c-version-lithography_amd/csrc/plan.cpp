@@ -130,7 +130,7 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     p->grad_partials.alloc(c * p->col_blocks * 2 * sizeof(double));
     p->red.alloc(c * 4 * sizeof(double));
     p->coef.alloc(c * sizeof(float));
-    p->host_red = new HostPinned(c * 4 * sizeof(double));
+    p->host_red = new HostPinned(c * 4 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent);
     p->host_coef = new HostPinned(c * sizeof(float));
     p->device_bytes = p->h.bytes + p->tw.bytes + p->rtw.bytes + p->row_of_u.bytes + p->col_of_u.bytes +
                       p->row_freq.bytes + p->col_freq.bytes + p->rt.bytes +
@@ -155,12 +155,13 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   SocsArgs a2 = a;
   if (!rq.adjoint) a2.v = nullptr;
   plan.ops->image_adjoint(a2, im, batch, s, kColumnAll);
-  reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, plan.red.as<double>(), 4, batch, s);
+  double* red = rq.red_out != nullptr ? rq.red_out : plan.red.as<double>();
+  reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, red, 4, batch, s);
   if (rq.adjoint && rq.im.target != nullptr) {
     GradArgs ga = rq.ga;
     ga.partials = plan.grad_partials.as<double>();
     plan.ops->adjoint_tail(a, ga, batch, s);
-    reduce_pairs(plan.grad_partials.as<double>(), plan.col_blocks, plan.red.as<double>() + 2, 4, batch, s);
+    reduce_pairs(plan.grad_partials.as<double>(), plan.col_blocks, red + 2, 4, batch, s);
   }
   LITHO_HIP(hipGetLastError());
 }
@@ -255,6 +256,11 @@ void spin_wait(hipStream_t s) {
   while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
   }
   LITHO_HIP(e);
+}
+
+const double* wait_reductions(Plan& plan) {
+  spin_wait(plan.ctx->stream);
+  return plan.host_red->as<double>();
 }
 
 const double* fetch_reductions(Plan& plan, int batch) {

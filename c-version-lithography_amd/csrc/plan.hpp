@@ -59,17 +59,27 @@ struct DevMem {
   T* as() const { return static_cast<T*>(ptr); }
 };
 
-// Pinned host staging.
+// Pinned host staging.  With mapped | coherent flags the kernels can write it
+// directly (dev is the device-side address): the writes are visible to the
+// host once the writing kernel has completed.
 struct HostPinned {
   void* ptr = nullptr;
-  explicit HostPinned(size_t b) {
-    if (hipHostMalloc(&ptr, b, hipHostMallocDefault) != hipSuccess) throw Error(2, "hipHostMalloc failed");
+  void* dev = nullptr;
+  explicit HostPinned(size_t b, unsigned flags = hipHostMallocDefault) {
+    if (hipHostMalloc(&ptr, b, flags) != hipSuccess) throw Error(2, "hipHostMalloc failed");
+    if ((flags & hipHostMallocMapped) && hipHostGetDevicePointer(&dev, ptr, 0) != hipSuccess) {
+      (void)hipHostFree(ptr);
+      ptr = nullptr;
+      throw Error(2, "hipHostGetDevicePointer failed");
+    }
   }
   HostPinned(const HostPinned&) = delete;
   HostPinned& operator=(const HostPinned&) = delete;
   ~HostPinned() { if (ptr) (void)hipHostFree(ptr); }
   template <class T>
   T* as() const { return static_cast<T*>(ptr); }
+  template <class T>
+  T* dev_as() const { return static_cast<T*>(dev); }
 };
 
 struct Context {
@@ -157,6 +167,7 @@ struct EvalRequest {
   GradArgs ga{};
   bool adjoint = true;
   const int* active = nullptr;       // device [batch] flags, null = all tiles
+  double* red_out = nullptr;         // device-visible [batch][4] reductions (PE, penalty, |g|^2, g.d); null = plan.red
 };
 
 void run_eval(Plan& plan, int batch, const EvalRequest& rq);
@@ -192,5 +203,8 @@ void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, dou
 const double* fetch_reductions(Plan& plan, int batch);
 // Waits for every command on the stream by polling it.
 void spin_wait(hipStream_t s);
+// The reductions of an evaluation whose request wrote them to plan.host_red
+// (red_out = its device address): waits for the stream, no copy.
+const double* wait_reductions(Plan& plan);
 
 }  // namespace litho
