@@ -133,7 +133,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(tiles would shard across GPUs; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "(tiles would shard across GPUs, one optimizer pool per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -93,7 +93,7 @@ def test_socs_converges_to_abbe():
 
 
 def test_eigengap_at_k24():
-    # K = 24 must not split a degenerate pair ((l24-l25)/l24 = 0.164 @ 512)
+    # K = 24 must not split a degenerate pair: at 512 the 24th and 25th eigenvalues differ by (l24-l25)/l24 = 0.164
     s = op.socs_kernels(512, **OPT, num_kernels=25)
     gap = (s.weights[23] - s.weights[24]) / s.weights[23]
     assert abs(gap - 0.164) < 0.01
