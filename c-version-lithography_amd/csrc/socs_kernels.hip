@@ -1058,7 +1058,8 @@ k_col_image_corr(SocsArgs a, ImageArgs im) {
   __shared__ float2 tw[TW > 0 ? TW : 1];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
-  const int x0 = blockIdx.x * C::LINES;
+  const int cb = xcd_column_block(blockIdx.x, gridDim.x);   // XCD-contiguous column blocks
+  const int x0 = cb * C::LINES;
   // the inverse-order half of the plan's stage tables (row_stage_twiddles(N))
   for (int e = threadIdx.x; e < TW; e += C::NT) tw[e] = a.rtw[e];
   const size_t tile = (size_t)b * N * N;
@@ -1111,7 +1112,7 @@ k_col_image_corr(SocsArgs a, ImageArgs im) {
         sp += red[2 * w];
         sb += red[2 * w + 1];
       }
-      double* out = im.partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
+      double* out = im.partials + ((size_t)b * gridDim.x + cb) * 2;
       out[0] = sp;
       out[1] = sb;
     }
@@ -1269,7 +1270,8 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
   __shared__ float2 mid[CF::MID_ENTRIES > 0 ? CF::MID_ENTRIES : 1];
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
-  const int x0 = blockIdx.x * C::LINES;
+  const int cb = xcd_column_block(blockIdx.x, gridDim.x);   // XCD-contiguous column blocks
+  const int x0 = cb * C::LINES;
   const size_t tile = (size_t)b * N * N;
   CF::fill_mid_tables(mid, a.tw);
   __syncthreads();
@@ -1314,7 +1316,7 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
       sn += red[2 * w];
       sd += red[2 * w + 1];
     }
-    double* out = ga.partials + ((size_t)b * gridDim.x + blockIdx.x) * 2;
+    double* out = ga.partials + ((size_t)b * gridDim.x + cb) * 2;
     out[0] = sn;
     out[1] = sd;
   }
