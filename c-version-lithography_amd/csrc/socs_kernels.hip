@@ -1273,6 +1273,20 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
   const int cb = xcd_column_block(blockIdx.x, gridDim.x);   // XCD-contiguous column blocks
   const int x0 = cb * C::LINES;
   const size_t tile = (size_t)b * N * N;
+  // theta at this thread's output elements, all loads in flight at once
+  // (inside the epilogue, sinf's branches left one round trip per element)
+  float th[CF::PERM][CF::RM];
+#pragma unroll
+  for (int s = 0; s < CF::PERM; ++s) {
+    const int q = threadIdx.x + s * C::NT;
+    const int x = x0 + q % C::LINES, i = q / C::LINES;
+    const bool ok = (CF::BM % C::NT == 0 || q < CF::BM) && x < N;
+#pragma unroll
+    for (int r = 0; r < CF::RM; ++r) {
+      const size_t gi = tile + (size_t)(i + r * CF::TM) * N + x;
+      th[s][r] = ok && ga.theta != nullptr ? ga.theta[gi] : 0.f;
+    }
+  }
   CF::fill_mid_tables(mid, a.tw);
   __syncthreads();
   float2 v[CF::PERM][CF::RM];
@@ -1292,12 +1306,25 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
       const float gm = ga.scale * v[s][r].x;
       if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gm;
       if (ga.theta != nullptr) {
-        float gt = -0.5f * gm * sinf(ga.theta[gi]);
+        float gt = -0.5f * gm * sinf(th[s][r]);
         if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
         if (ga.grad_theta != nullptr) ga.grad_theta[gi] = gt;
         nrm += (double)gt * (double)gt;
-        if (ga.dir != nullptr) dot += (double)gt * (double)ga.dir[gi];
+        th[s][r] = gt;
       }
+    }
+  }
+  // G_theta . dir in a second pass without sinf's branches, so that the
+  // direction's loads are in flight together (same summation order)
+  if (ga.theta != nullptr && ga.dir != nullptr) {
+#pragma unroll
+    for (int s = 0; s < CF::PERM; ++s) {
+      const int q = threadIdx.x + s * C::NT;
+      const int x = x0 + q % C::LINES, i = q / C::LINES;
+      if (!((CF::BM % C::NT == 0 || q < CF::BM) && x < N)) continue;
+#pragma unroll
+      for (int r = 0; r < CF::RM; ++r)
+        dot += (double)th[s][r] * (double)ga.dir[tile + (size_t)(i + r * CF::TM) * N + x];
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
