@@ -125,13 +125,46 @@ struct RowFFT {
         float2 v[RM];
 #pragma unroll
         for (int r = 0; r < RM; ++r) v[r] = in(line, i + r * TM);
-        Dft<RM, -1>::run(v);
-        float2* d = buf + line * LS;
-        const int so = swz<SH>(i * RM);
-#pragma unroll
-        for (int r = 0; r < RM; ++r) d[sw_at<SH, sw_uniform_write<SH, 1, RM>(), 1>(i * RM, so, r)] = v[r];
+        first_store(buf, line, i, v);
       }
     }
+    forward_tail(buf, tabs, out);
+  }
+
+  // forward() with the input split in two: load(line, x) returns the raw
+  // values of a position and make(line, x, raw) the transform's input.  All
+  // loads of a pass are issued before the first make, so their latency is
+  // paid once per pass even when make branches (cosf) or stores.
+  template <class Load, class Make, class Out>
+  __device__ __forceinline__ static void forward_split(float2* buf, const float2* __restrict__ tabs, Load load,
+                                                       Make make, Out out) {
+#pragma unroll
+    for (int s = 0; s < PERM; ++s) {
+      const int q = threadIdx.x + s * NT;
+      if (BM % NT == 0 || q < BM) {
+        const int line = q / TM, i = q - line * TM;
+        decltype(load(0, 0)) raw[RM];
+#pragma unroll
+        for (int r = 0; r < RM; ++r) raw[r] = load(line, i + r * TM);
+        float2 v[RM];
+#pragma unroll
+        for (int r = 0; r < RM; ++r) v[r] = make(line, i + r * TM, raw[r]);
+        first_store(buf, line, i, v);
+      }
+    }
+    forward_tail(buf, tabs, out);
+  }
+
+  __device__ __forceinline__ static void first_store(float2* buf, int line, int i, float2 (&v)[RM]) {
+    Dft<RM, -1>::run(v);
+    float2* d = buf + line * LS;
+    const int so = swz<SH>(i * RM);
+#pragma unroll
+    for (int r = 0; r < RM; ++r) d[sw_at<SH, sw_uniform_write<SH, 1, RM>(), 1>(i * RM, so, r)] = v[r];
+  }
+
+  template <class Out>
+  __device__ __forceinline__ static void forward_tail(float2* buf, const float2* __restrict__ tabs, Out out) {
     __syncthreads();
     fwd_middle<1, RM>(buf, tabs);
     const float2* lt = tabs + ST.off[FWD_LAST];

@@ -548,9 +548,24 @@ k_row_fwd_pruned(SocsArgs a, int mode, const float* base, const float* __restric
   const size_t tile = (size_t)b * N * N;
   const int row0 = blockIdx.x * 2 * C::LINES;
   const float cf = (dir != nullptr) ? coef[b] : 0.f;
-  RF::forward(
+  const bool use_dir = mode == 1 && dir != nullptr;
+  RF::forward_split(
       buf, a.rtw,
-      [&](int l, int x) {
+      [&](int l, int x) {   // base and direction of the two rows, loads only
+        float4 raw = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int y = row0 + 2 * l;
+        const size_t g = tile + (size_t)y * N + x;
+        if (y < N) {
+          raw.x = base[g];
+          if (use_dir) raw.z = dir[g];
+        }
+        if (y + 1 < N) {
+          raw.y = base[g + N];
+          if (use_dir) raw.w = dir[g + N];
+        }
+        return raw;
+      },
+      [&](int l, int x, float4 raw) {
         float m2[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -558,9 +573,9 @@ k_row_fwd_pruned(SocsArgs a, int mode, const float* base, const float* __restric
           float m = 0.f;
           if (y < N) {
             const size_t g = tile + (size_t)y * N + x;
-            float v = base[g];
+            float v = h == 0 ? raw.x : raw.y;
             if (mode == 1) {
-              if (dir != nullptr) v = v + cf * dir[g];
+              if (dir != nullptr) v = v + cf * (h == 0 ? raw.z : raw.w);
               if (theta_out != nullptr) theta_out[g] = v;
               m = 0.5f * (1.0f + cosf(v));
             } else {
