@@ -10,8 +10,10 @@ evaluation (line-search trials included).  Inputs are resident in HBM before
 the timed region.  FP32 fields, FP64 loss / norm / L-BFGS scalars.
 
 The CPU baseline is the FP64 numpy/scipy oracle (the reference's full-FFT
-dataflow, 3K+1 N x N FFTs per grad-eval) timed on a bounded sample of
-grad-evals of the same workload on this host's cores.
+dataflow, 3K+1 N x N FFTs per grad-eval) timed on bounded samples on this
+host: the 2048^2 workload on all cores (its rate is `value`), and the
+512^2 CPU configuration on 1 thread and on all cores; each figure is the
+median of 3 repeats.
 """
 from __future__ import annotations
 
@@ -85,6 +87,7 @@ PAIR_KERNEL = f"k_lbfgs_pair<{HISTORY}>"
 EVAL_KERNEL = f"k_corr_adjoint<{corr_len(W)}>"
 PAIR_TRAFFIC_FILE = ROOT / "profiles" / f"k_lbfgs_pair_{HISTORY}_2048_hbm_traffic.md"
 EVAL_TRAFFIC_FILE = ROOT / "profiles" / f"k_corr_adjoint_{corr_len(W)}_2048_hbm_traffic.md"
+EVAL_TRAFFIC_FILE_ALL = ROOT / "profiles" / "socs_eval_2048_k24_hbm_traffic.md"
 FP32_PEAK = 157.3e12
 
 
@@ -101,25 +104,57 @@ def measured_traffic(path, kernel):
     return None
 
 
-def cpu_baseline(kernels, target, seconds=20.0):
-    from oracle import imaging as im
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(cache, target, threads, evals, repeats=3):
+    """Median over `repeats` of evals / wall time of the oracle grad-eval, and its PE."""
     from oracle import ctm as oc
-    cores = len(os.sched_getaffinity(0))
-    im.set_threads(cores)
-    cache = im.Cache.from_socs(kernels)
+    from oracle import imaging as im
+    im.set_threads(threads)
     theta = oc.initialize_theta(target.astype(np.float64))
     t64 = target.astype(np.float64)
-    im.ctm_evaluate(cache, theta, t64, THR, ALPHA)            # warm-up (plans, page-in)
-    evals, t0 = 0, time.perf_counter()
-    while True:
-        im.ctm_evaluate(cache, theta, t64, THR, ALPHA)
-        evals += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or evals >= 50:
-            break
-    return {"value": evals / el, "unit": "grad-evals/s", "cores": cores, "kind": "port",
-            "sample": f"{evals} grad-evals of the 2048^2 K=24 workload (FP64 numpy/scipy oracle, "
-                      f"full N x N FFTs, {cores} threads) in {el:.1f} s"}
+    pe = im.ctm_evaluate(cache, theta, t64, THR, ALPHA)[0]          # warm-up (plans, page-in)
+    rates = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            im.ctm_evaluate(cache, theta, t64, THR, ALPHA)
+        rates.append(evals / (time.perf_counter() - t0))
+    return float(np.median(rates)), pe
+
+
+def cpu_baseline(dk, target, ctx, tg, evals_2048=2):
+    """The oracle timed on this host (bounded: about 20-40 s in all)."""
+    from oracle import imaging as im
+    from oracle import optics as op
+    cores = len(os.sched_getaffinity(0))
+    sk = op.SocsKernels(n=N, win_y0=dk["y0"], win_x0=dk["x0"], kernels=dk["kernels"], weights=dk["weights"],
+                        weight_sum=dk["weight_sum"])
+    rate, _ = _oracle_rate(im.Cache.from_socs(sk), target, cores, evals_2048)
+    # the CPU configuration (512^2, K = 24): 1 thread and all cores, same result
+    d5 = ctx.socs_precompute(512, OPT["pixel_nm"], OPT["wavelength_nm"], OPT["na"], OPT["sigma_in"],
+                             OPT["sigma_out"], K)
+    c5 = im.Cache.from_socs(op.SocsKernels(n=512, win_y0=d5["y0"], win_x0=d5["x0"], kernels=d5["kernels"],
+                                           weights=d5["weights"], weight_sum=d5["weight_sum"]))
+    t5 = tg.manhattan_target(512, 512)
+    r1, pe1 = _oracle_rate(c5, t5, 1, 3)
+    rn, pen = _oracle_rate(c5, t5, cores, 3)
+    im.set_threads(1)
+    return {"value": round(rate, 4), "unit": "grad-evals/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(),
+            "sample": f"FP64 numpy/scipy oracle (full N x N FFTs, 3K+1 per grad-eval): median of 3 repeats of "
+                      f"{evals_2048} grad-evals of the 2048^2 K=24 workload on {cores} threads",
+            "c1_512": {"threads_1": round(r1, 3), f"threads_{cores}": round(rn, 3), "unit": "grad-evals/s",
+                       "sample": "median of 3 repeats of 3 grad-evals, 512^2 K=24",
+                       "pe_rel_diff_1_vs_all": abs(pe1 - pen) / abs(pen)}}
 
 
 def main():
@@ -129,7 +164,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -139,21 +173,19 @@ def main():
 
     la = _load("litho_amd", ROOT / "c-version-lithography_amd" / "litho_amd.py")
     tg = _load("litho_targets", ROOT / "c-version-lithography_amd" / "targets.py")
-    from oracle import optics as op
 
     ctx = la.Context(0)
     t0 = time.perf_counter()
     dk = ctx.socs_precompute(N, OPT["pixel_nm"], OPT["wavelength_nm"], OPT["na"], OPT["sigma_in"],
                              OPT["sigma_out"], K)
     t_pre = time.perf_counter() - t0
-    sk = op.SocsKernels(n=N, win_y0=dk["y0"], win_x0=dk["x0"], kernels=dk["kernels"], weights=dk["weights"],
-                        weight_sum=dk["weight_sum"])
-    plan = la.Plan.from_socs(ctx, sk, capacity=1)
+    plan = la.Plan(ctx, N, dk["kernels"], dk["weights"], dk["weight_sum"], dk["y0"], dk["x0"], capacity=1)
     target = tg.manhattan_target(N, 2048)
     d_target = torch.from_numpy(target).cuda()
     out = torch.zeros_like(d_target)
     torch.cuda.synchronize()
-    print(f"setup: precompute {t_pre:.2f} s, window {dk['rows']}x{dk['cols']}, plan {plan.device_bytes()/2**20:.0f} MiB",
+    print(f"setup: precompute {t_pre:.2f} s, window {dk['rows']}x{dk['cols']}, eigen gap at K "
+          f"{dk['eigen_gap']:.4f}, plan {plan.device_bytes()/2**20:.0f} MiB",
           file=sys.stderr, flush=True)
 
     def solve():
@@ -163,8 +195,12 @@ def main():
     for _ in range(args.warmup):
         solve()
     torch.cuda.synchronize()
+    graphs0 = ctx.graph_counts()
     evals = 0
     infos = []
+    # HIP events around every pair-kernel launch of the timed solves, on the
+    # library stream the kernel runs on
+    ctx.time_pair_kernel(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = solve()
@@ -172,6 +208,9 @@ def main():
         infos.append(r)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    pair_launches, pair_total_ms = ctx.pair_kernel_time()
+    ctx.time_pair_kernel(False)
+    graphs1 = ctx.graph_counts()
     value = evals / elapsed
     print(f"timed: {args.steps} solves, {evals} grad-evals in {elapsed:.3f} s; final PE {infos[-1]['final_pe']:.3f} "
           f"status {infos[-1]['status']}", file=sys.stderr, flush=True)
@@ -186,11 +225,14 @@ def main():
     adj_flops = corr_adjoint_flops(N, K, W)
     adj_bytes = corr_adjoint_bytes(N, K, W)
     eval_bytes = algorithmic_bytes_per_eval(N, K, W)
-    # the pair kernel alone (HIP events on the library stream), one launch per L-BFGS iteration
-    pair_ms = ctx.profile_lbfgs_pair(N * N, HISTORY, 20)
+    # the pair kernel's mean launch time inside the timed solves (one launch per L-BFGS iteration)
+    if pair_launches <= 0:
+        raise RuntimeError("no pair-kernel launch was timed in the solves")
+    pair_ms = pair_total_ms / pair_launches
     pair_bytes = pair_kernel_bytes(N, HISTORY)
     print(f"stages (ms): forward {st[0]:.3f} column image {st[1]:.3f} column adjoint {st[2]:.3f} "
-          f"adjoint-tail {st[3]:.3f} eval {st[4]:.3f}; {PAIR_KERNEL} {pair_ms:.4f}", file=sys.stderr, flush=True)
+          f"adjoint-tail {st[3]:.3f} eval {st[4]:.3f}; {PAIR_KERNEL} {pair_ms:.4f} (mean of {pair_launches} "
+          f"launches in the timed solves)", file=sys.stderr, flush=True)
     line = {
         "metric": "CTM grad-evals/s (SOCS fwd+adjoint) 2048^2, K=24",
         "value": round(value, 3), "unit": "grad-evals/s", "n_gpus": 1, "steps": args.steps,
@@ -198,14 +240,15 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (fp64 reductions)", "data": "synthetic",
         "config": {"workload": "single-tile CTM L-BFGS, 2048x2048 @4nm, K=24 (219x219 window), m=10, "
                                "50 iterations, tol 1e-2, seeded Manhattan target",
-                   "n": N, "k": K, "window": W, "evals_per_step": round(evals / args.steps, 2)},
+                   "n": N, "k": K, "window": W, "evals_per_step": round(evals / args.steps, 2),
+                   "eval_graphs": {k: graphs1[k] - graphs0[k] for k in graphs1}},
         # The kernel with the most time in the timed solves: one pair-kernel
         # launch per L-BFGS iteration, HBM-bound.
         "roofline": {"bound": "hbm", "kernel": PAIR_KERNEL,
                      "achieved": round(pair_bytes / (pair_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(pair_bytes / (pair_ms * 1e-3) / HBM_PEAK, 4),
                      "traffic": measured_traffic(PAIR_TRAFFIC_FILE, PAIR_KERNEL), "algorithmic_bytes": pair_bytes,
-                     "launch_ms": round(pair_ms, 4)},
+                     "launch_ms": round(pair_ms, 4), "launches_timed": pair_launches},
         # The dominant kernel of a grad-eval, bound by FP32 vector (VALU) FFT
         # arithmetic, priced against MI355X's published FP32 peak (packed FP32
         # is not used, so the issue ceiling is half of it); its compulsory HBM
@@ -218,12 +261,19 @@ def main():
                                  "hbm": {"algorithmic_bytes": adj_bytes,
                                          "achieved_GBps": round(adj_bytes / adj_s / 1e9, 1),
                                          "frac": round(adj_bytes / adj_s / HBM_PEAK, 4)}},
+        # One whole grad-eval as the solves run it (theta written, gradient
+        # stored, g.d formed), priced with the reference's materialised-field
+        # bytes, and with the HBM bytes the kernels actually move (counters).
         "eval_roofline": {"algorithmic_bytes": eval_bytes, "eval_ms": round(st[4], 4),
                           "achieved_GBps": round(eval_bytes / (st[4] * 1e-3) / 1e9, 1),
-                          "frac": round(eval_bytes / (st[4] * 1e-3) / HBM_PEAK, 4)},
+                          "frac": round(eval_bytes / (st[4] * 1e-3) / HBM_PEAK, 4),
+                          "traffic": measured_traffic(EVAL_TRAFFIC_FILE_ALL, "grad-eval"),
+                          "traffic_frac": (None if measured_traffic(EVAL_TRAFFIC_FILE_ALL, "grad-eval") is None else
+                                           round(measured_traffic(EVAL_TRAFFIC_FILE_ALL, "grad-eval") /
+                                                 (st[4] * 1e-3) / HBM_PEAK, 4))},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(sk, target, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(dk, target, ctx, tg)
     plan.close()
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -3,14 +3,18 @@
 //
 // Gradient descent restates CTM_Optimizer::_optimize_gradient_descent
 // (CTM_Optimizer.cpp:109-173).  L-BFGS restates the published algorithm of
-// LBFGS++ (LBFGSSolver::minimize as called at CTM_Optimizer.cpp:194-212): the
+// LBFGS++ v0.4.0 (LBFGSSolver<double> with its default line search,
+// LineSearchNocedalWright, as constructed at CTM_Optimizer.cpp:194-200): the
 // inverse-Hessian product uses the compact representation of Byrd, Nocedal &
 // Schnabel (1994), which equals LBFGS++'s two-loop recursion with
 // H0 = (s'y / y'y) I; pairs are kept only when s'y > eps * y'y; the first step
-// is 1/||d||, later steps 1; the strong-Wolfe line search follows Nocedal &
-// Wright Alg. 3.5 / 3.6 with a safeguarded cubic zoom.  Every objective
-// evaluation (line-search trials included) is one grad-eval and appends one PE
-// to the history, like the reference's _error_history (CTM_Optimizer.cpp:204).
+// is 1/||d||, later steps 1; the strong-Wolfe line search brackets with
+// expansion 2 (Nocedal & Wright Alg. 3.5) and zooms with the
+// quadratic-interpolation trial of ls_quad_interp (Alg. 3.6); when
+// max_linesearch trials run out it takes the best point of sufficient decrease
+// and the solve goes on.  Every objective evaluation (line-search trials
+// included) is one grad-eval and appends one PE to the history, like the
+// reference's _error_history (CTM_Optimizer.cpp:204).
 #include "ctm_driver.hpp"
 
 #include <algorithm>
@@ -21,19 +25,17 @@
 
 namespace litho {
 
-double zoom_trial(double a_lo, double f_lo, double g_lo, double a_hi, double f_hi, double g_hi) {
-  const double lo = std::min(a_lo, a_hi), hi = std::max(a_lo, a_hi);
-  const double width = hi - lo;
-  double a = std::numeric_limits<double>::quiet_NaN();
-  const double d1 = g_lo + g_hi - 3.0 * (f_lo - f_hi) / (a_lo - a_hi);
-  const double rad = d1 * d1 - g_lo * g_hi;
-  if (rad >= 0.0) {
-    const double d2 = std::copysign(std::sqrt(rad), a_hi - a_lo);
-    const double den = g_hi - g_lo + 2.0 * d2;
-    if (den != 0.0) a = a_hi - (a_hi - a_lo) * (g_hi + d2 - d1) / den;
-  }
-  if (!std::isfinite(a) || a < lo + 0.1 * width || a > hi - 0.1 * width) a = 0.5 * (lo + hi);
-  return a;
+double ls_quad_interp(double step_lo, double step_hi, double fx_lo, double fx_hi, double dg_lo) {
+  const double fdiff = fx_hi - fx_lo;
+  const double sdiff = step_hi - step_lo;
+  const double smid = (step_hi + step_lo) / 2.0;
+  double cand = fdiff * step_lo - smid * sdiff * dg_lo;
+  cand = cand / (fdiff - sdiff * dg_lo);
+  if (!std::isfinite(cand)) return smid;
+  const double end_dist = std::min(std::abs(cand - step_lo), std::abs(cand - step_hi));
+  const bool near_end = end_dist < 0.01 * std::abs(sdiff);
+  if (cand <= std::min(step_lo, step_hi) || cand >= std::max(step_lo, step_hi) || near_end) return smid;
+  return cand;
 }
 
 void validate_ctm_options(const CtmOptions& opt) {
@@ -79,13 +81,16 @@ void copy_tile(float* dst, const float* src, size_t n, hipStream_t s) {
 void alloc_slots(DevMem& m, int slots, size_t n2, size_t per = 1) { m.alloc((size_t)slots * per * n2 * sizeof(float)); }
 
 struct LbfgsTile {
-  enum Phase { INIT, BRACKET, ZOOM, NEED_PAIR, DONE };
+  // INIT: first evaluation; BRACKET / ZOOM: line-search trials; REEVAL: the
+  // point of the best sufficient-decrease trial, evaluated again (uncounted)
+  // after the trials ran out, since only the last trial's gradient is kept
+  enum Phase { INIT, BRACKET, ZOOM, REEVAL, NEED_PAIR, DONE };
   Phase phase = INIT;
-  int k = 0, trials = 0;
-  int pairs = 0;   // pair updates formed so far: the history holds at most min(pairs, m) pairs
-  double f = 0, gnorm = 0, f0 = 0, dg0 = 0, step = 0, a = 0;
-  double a_prev = 0, f_prev = 0, dg_prev = 0;
-  double lo_a = 0, lo_f = 0, lo_g = 0, hi_a = 0, hi_f = 0, hi_g = 0;
+  int k = 0;         // LBFGS++ iteration counter
+  int ls_iter = 0;   // LineSearchNocedalWright's trial counter
+  int pairs = 0;     // pair updates formed so far: the history holds at most min(pairs, m) pairs
+  double f = 0, gnorm = 0, f0 = 0, dg0 = 0, test_decr = 0, test_curv = 0, step = 0, a = 0;
+  double lo_a = 0, lo_f = 0, lo_g = 0, hi_a = 0, hi_f = 0;
   bool final_is_xp = false;
 };
 
@@ -266,6 +271,9 @@ struct LbfgsPool final : CtmPool::Impl {
   std::vector<char> pending_dg0;
   int mb_pair = 0;
   std::unique_ptr<EvalGraph> graph;
+  // events around the round's pair-kernel launch when the context times it
+  hipEvent_t pair_ev[2] = {nullptr, nullptr};
+  bool pair_timed = false;
   const LineSearchParams lp;
   const int m_max;
 
@@ -300,6 +308,12 @@ struct LbfgsPool final : CtmPool::Impl {
     rq.ga.theta = x.as<float>();
     rq.ga.dir = d.as<float>();
     graph = std::make_unique<EvalGraph>(plan, S, rq);
+    LITHO_HIP(hipEventCreate(&pair_ev[0]));
+    LITHO_HIP(hipEventCreate(&pair_ev[1]));
+  }
+  ~LbfgsPool() override {
+    (void)hipEventDestroy(pair_ev[0]);
+    (void)hipEventDestroy(pair_ev[1]);
   }
 
   bool admit(const CtmJob& j) override {
@@ -319,18 +333,16 @@ struct LbfgsPool final : CtmPool::Impl {
     LbfgsTile& t = st[b];
     t.f0 = t.f;
     t.dg0 = dg0;
-    t.trials = 0;
-    t.a_prev = 0.0;
-    t.f_prev = t.f0;
-    t.dg_prev = dg0;
+    t.test_decr = lp.ftol * dg0;
+    t.test_curv = -lp.wolfe * dg0;
+    t.ls_iter = 0;
+    t.lo_a = 0.0;
+    t.lo_f = t.f0;
+    t.lo_g = dg0;
+    t.hi_a = 0.0;
+    t.hi_f = t.f0;
     t.a = t.step;
-    if (dg0 >= 0.0) {  // LBFGS++: "the moving direction does not decrease the objective"
-      t.phase = LbfgsTile::DONE;
-      t.final_is_xp = true;
-      res[b].status = 2;
-      return;
-    }
-    if (t.a < lp.min_step) {
+    if (dg0 > 0.0) {  // LBFGS++ throws: "the moving direction increases the objective function value"
       t.phase = LbfgsTile::DONE;
       t.final_is_xp = true;
       res[b].status = 2;
@@ -385,8 +397,12 @@ struct LbfgsPool final : CtmPool::Impl {
   // the round (it selects the instantiation); one more pair after it
   void launch_directions() {
     const int mb_dir = std::min(mb_pair + 1, m_max);
+    // timed: launches over a full history (the k_lbfgs_pair<m> instantiation)
+    pair_timed = plan.ctx->time_pair && mb_pair == m_max;
+    if (pair_timed) LITHO_HIP(hipEventRecord(pair_ev[0], s));
     launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(), hy.as<float>(),
                       pm.d_need(), d_meta.as<int>(), n2, S, mb_pair, true, pair_part.as<double>(), s);
+    if (pair_timed) LITHO_HIP(hipEventRecord(pair_ev[1], s));
     launch_reduce_fixed(pair_part.as<double>(), kOptBlocks, 4 * (mb_pair + 1), vec_red.as<double>(), kPairDots, S,
                         s);
     launch_lbfgs_update(S, pm.d_need(), vec_red.as<double>(), mb_pair, m_max, d_meta.as<int>(), d_sy.as<double>(),
@@ -405,7 +421,7 @@ struct LbfgsPool final : CtmPool::Impl {
     for (int b = 0; b < S; ++b) {
       const LbfgsTile& t = st[b];
       const bool ev = running[b] && (t.phase == LbfgsTile::INIT || t.phase == LbfgsTile::BRACKET ||
-                                     t.phase == LbfgsTile::ZOOM);
+                                     t.phase == LbfgsTile::ZOOM || t.phase == LbfgsTile::REEVAL);
       flags[b] = ev ? 1 : 0;
       stepv[b] = (ev && t.phase != LbfgsTile::INIT) ? (float)t.a : 0.f;
       any = any || ev;
@@ -415,6 +431,12 @@ struct LbfgsPool final : CtmPool::Impl {
     if (directions) launch_directions();
     graph->launch();
     const double* red = wait_reductions(plan);
+    if (directions && pair_timed) {   // both events completed before the evaluation did
+      float ms = 0.f;
+      LITHO_HIP(hipEventElapsedTime(&ms, pair_ev[0], pair_ev[1]));
+      plan.ctx->pair_launches.fetch_add(1);
+      plan.ctx->add_pair_ms(ms);
+    }
     std::vector<int> started;
     for (int b = 0; b < S; ++b) {
       if (!flags[b]) continue;
@@ -430,6 +452,15 @@ struct LbfgsPool final : CtmPool::Impl {
       const double f = red[4 * b + 0];
       const double gn2 = red[4 * b + 2];
       const double dg = red[4 * b + 3];
+      if (t.phase == LbfgsTile::REEVAL) {
+        // the same theta as the earlier trial at lo_a, so the same f; the
+        // reference keeps that point's gradient instead of evaluating again,
+        // so this evaluation is neither counted nor recorded
+        t.f = f;
+        t.gnorm = std::sqrt(gn2);
+        accept(b);
+        continue;
+      }
       res[b].history.push_back(f);
       res[b].evaluations++;
       if (t.phase == LbfgsTile::INIT) {
@@ -445,48 +476,65 @@ struct LbfgsPool final : CtmPool::Impl {
         started.push_back(b);             // d = -g, then the first line search
         continue;
       }
-      t.trials += 1;
       const double a = t.a;
       if (t.phase == LbfgsTile::BRACKET) {
-        if (f > t.f0 + lp.ftol * a * t.dg0 || (t.trials > 1 && f >= t.f_prev)) {
-          t.lo_a = t.a_prev; t.lo_f = t.f_prev; t.lo_g = t.dg_prev;
-          t.hi_a = a; t.hi_f = f; t.hi_g = dg;
+        // Alg. 3.5: leave for the zoom on a failed decrease test (or no
+        // decrease against the last trial), stop at strong Wolfe, else expand
+        if (f - t.f0 > a * t.test_decr || (0.0 < t.lo_a && f >= t.lo_f)) {
+          t.hi_a = a; t.hi_f = f;
           t.phase = LbfgsTile::ZOOM;
-        } else if (std::abs(dg) <= -lp.wolfe * t.dg0) {
+        } else if (std::abs(dg) <= t.test_curv) {
           t.f = f; t.gnorm = std::sqrt(gn2);
           accept(b);
           continue;
-        } else if (dg >= 0.0) {
+        } else {
+          t.hi_a = t.lo_a; t.hi_f = t.lo_f;
           t.lo_a = a; t.lo_f = f; t.lo_g = dg;
-          t.hi_a = t.a_prev; t.hi_f = t.f_prev; t.hi_g = t.dg_prev;
-          t.phase = LbfgsTile::ZOOM;
-        } else {
-          t.a_prev = a; t.f_prev = f; t.dg_prev = dg;
-          t.a = std::min(2.0 * a, lp.max_step);
-          if (t.trials >= lp.max_linesearch || t.a < lp.min_step) fail(b);
-          continue;
+          if (dg >= 0.0) {
+            t.phase = LbfgsTile::ZOOM;
+          } else {
+            t.ls_iter += 1;
+            if (t.ls_iter >= lp.max_linesearch) {   // trials used up: the best point is this one
+              t.f = f; t.gnorm = std::sqrt(gn2);
+              accept(b);
+            } else {
+              t.a = 2.0 * a;
+            }
+            continue;
+          }
         }
-      } else {  // ZOOM
-        if (f > t.f0 + lp.ftol * a * t.dg0 || f >= t.lo_f) {
-          t.hi_a = a; t.hi_f = f; t.hi_g = dg;
+      } else {  // ZOOM, Alg. 3.6
+        if (f - t.f0 > a * t.test_decr || f >= t.lo_f) {
+          if (a == t.hi_a) { fail(b); continue; }   // LBFGS++ throws: insufficient numeric precision
+          t.hi_a = a; t.hi_f = f;
         } else {
-          if (std::abs(dg) <= -lp.wolfe * t.dg0) {
+          if (std::abs(dg) <= t.test_curv) {
             t.f = f; t.gnorm = std::sqrt(gn2);
             accept(b);
             continue;
           }
           if (dg * (t.hi_a - t.lo_a) >= 0.0) {
-            t.hi_a = t.lo_a; t.hi_f = t.lo_f; t.hi_g = t.lo_g;
+            t.hi_a = t.lo_a; t.hi_f = t.lo_f;
           }
+          if (a == t.lo_a) { fail(b); continue; }   // LBFGS++ throws: insufficient numeric precision
           t.lo_a = a; t.lo_f = f; t.lo_g = dg;
         }
+        t.ls_iter += 1;
+        if (t.ls_iter >= lp.max_linesearch) {
+          // trials used up: the best point of sufficient decrease (lo), or
+          // LBFGS++ throws when there is none
+          if (t.lo_a <= 0.0) { fail(b); continue; }
+          if (t.lo_a == a) {
+            t.f = f; t.gnorm = std::sqrt(gn2);
+            accept(b);
+          } else {
+            t.phase = LbfgsTile::REEVAL;
+            t.a = t.lo_a;
+          }
+          continue;
+        }
       }
-      // next zoom trial
-      if (t.trials >= lp.max_linesearch) {
-        fail(b);
-        continue;
-      }
-      t.a = zoom_trial(t.lo_a, t.lo_f, t.lo_g, t.hi_a, t.hi_f, t.hi_g);
+      t.a = ls_quad_interp(t.lo_a, t.hi_a, t.lo_f, t.hi_f, t.lo_g);
     }
     if (started.empty()) return;
     // slots whose first evaluation just ended: d = -g and the first line search

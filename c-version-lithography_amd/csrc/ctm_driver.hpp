@@ -41,14 +41,15 @@ struct CtmTileResult {
 // Strong-Wolfe line-search parameters (LBFGS++ LBFGSParam defaults).
 struct LineSearchParams {
   int max_linesearch = 20;
-  double min_step = 1e-20;
-  double max_step = 1e20;
   double ftol = 1e-4;
   double wolfe = 0.9;
 };
 
-// Safeguarded cubic trial inside a zoom bracket (Nocedal & Wright eq. 3.59).
-double zoom_trial(double a_lo, double f_lo, double g_lo, double a_hi, double f_hi, double g_hi);
+// Zoom trial of LBFGS++'s LineSearchNocedalWright: the minimiser of the
+// quadratic through (step_lo, fx_lo) with slope dg_lo and (step_hi, fx_hi),
+// or the bracket midpoint when that is not finite, not strictly inside the
+// bracket, or within 1 % of the bracket width of an end.
+double ls_quad_interp(double step_lo, double step_hi, double fx_lo, double fx_hi, double dg_lo);
 
 // One tile for a pool slot: device N x N buffers.  The target is copied into
 // the slot at admission; outputs are written when the tile finishes (best-PE

@@ -2,6 +2,7 @@
 // into status codes and a thread-local message.
 #include "../../include/litho_amd.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -32,6 +33,7 @@ struct litho_server {
 namespace {
 
 thread_local std::string g_last_error;
+thread_local double g_last_eigen_gap = 0.0;
 
 template <class F>
 int guarded(F&& f) {
@@ -128,6 +130,10 @@ int litho_ctx_create(int device, litho_ctx** out) {
 int litho_ctx_destroy(litho_ctx* ctx) {
   return guarded([&] {
     if (ctx == nullptr) return;
+    const int live = ctx->c.live_plans.load();
+    if (live != 0)
+      throw litho::Error(LITHO_EINVAL, "context still has " + std::to_string(live) +
+                                           " plan(s): destroy them before their context");
     (void)hipSetDevice(ctx->c.device);
     (void)hipStreamSynchronize(ctx->c.stream);
     (void)hipStreamDestroy(ctx->c.stream);
@@ -151,6 +157,32 @@ int litho_ctx_set_graphs(litho_ctx* ctx, int enable) {
   });
 }
 
+int litho_ctx_graph_counts(litho_ctx* ctx, long long* captured, long long* failed, long long* direct) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    if (captured != nullptr) *captured = c.graphs_captured.load();
+    if (failed != nullptr) *failed = c.graphs_failed.load();
+    if (direct != nullptr) *direct = c.graphs_direct.load();
+  });
+}
+
+int litho_ctx_time_pair_kernel(litho_ctx* ctx, int enable) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    c.time_pair = enable != 0;
+    c.pair_launches.store(0);
+    c.pair_ms.store(0.0);
+  });
+}
+
+int litho_ctx_pair_kernel_time(litho_ctx* ctx, long long* launches, double* total_ms) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    if (launches != nullptr) *launches = c.pair_launches.load();
+    if (total_ms != nullptr) *total_ms = c.pair_ms.load();
+  });
+}
+
 int litho_socs_window(int n, double pixel_nm, double wavelength_nm, double na, double sigma_in, double sigma_out,
                       int* win_rows, int* win_cols, int* win_y0, int* win_x0, int* num_source_points,
                       double* weight_sum) {
@@ -167,6 +199,8 @@ int litho_socs_window(int n, double pixel_nm, double wavelength_nm, double na, d
   });
 }
 
+double litho_socs_last_eigen_gap(void) { return g_last_eigen_gap; }
+
 int litho_socs_precompute(litho_ctx* ctx, int n, double pixel_nm, double wavelength_nm, double na, double sigma_in,
                           double sigma_out, int k, double* kernels_out, double* weights_out,
                           double* singular_values_out) {
@@ -175,7 +209,9 @@ int litho_socs_precompute(litho_ctx* ctx, int n, double pixel_nm, double wavelen
     require(kernels_out != nullptr && weights_out != nullptr, "null output arrays");
     LITHO_HIP(hipSetDevice(ctx->c.device));
     litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, {}};
-    litho::socs_precompute_device(op, k, ctx->c.stream, kernels_out, weights_out, singular_values_out);
+    g_last_eigen_gap = 0.0;
+    litho::socs_precompute_device(op, k, ctx->c.stream, kernels_out, weights_out, singular_values_out,
+                                  &g_last_eigen_gap);
   });
 }
 
@@ -191,7 +227,9 @@ int litho_socs_precompute_zernike(litho_ctx* ctx, int n, double pixel_nm, double
     LITHO_HIP(hipSetDevice(ctx->c.device));
     litho::OpticsParams op{n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, {}};
     for (int i = 0; i < num_terms; ++i) op.zernike.emplace_back(zernike_index[i], zernike_coeff[i]);
-    litho::socs_precompute_device(op, k, ctx->c.stream, kernels_out, weights_out, singular_values_out);
+    g_last_eigen_gap = 0.0;
+    litho::socs_precompute_device(op, k, ctx->c.stream, kernels_out, weights_out, singular_values_out,
+                                  &g_last_eigen_gap);
   });
 }
 
@@ -202,6 +240,7 @@ int litho_plan_create(litho_ctx* ctx, int n, int k, int win_rows, int win_cols, 
     require(ctx != nullptr && out != nullptr, "null context or output handle");
     litho::Plan* p = litho::make_plan(&ctx->c, n, k, win_rows, win_cols, win_y0, win_x0, kernels, weights,
                                       weight_sum, batch_capacity);
+    ctx->c.live_plans.fetch_add(1);
     *out = new litho_plan{p};
   });
 }
@@ -210,9 +249,11 @@ int litho_plan_destroy(litho_plan* plan) {
   return guarded([&] {
     if (plan == nullptr) return;
     if (plan->p) {
-      (void)hipSetDevice(plan->p->ctx->device);
-      (void)hipStreamSynchronize(plan->p->ctx->stream);
+      litho::Context* c = plan->p->ctx;
+      (void)hipSetDevice(c->device);
+      (void)hipStreamSynchronize(c->stream);
       delete plan->p;
+      c->live_plans.fetch_sub(1);
     }
     delete plan;
   });
@@ -444,16 +485,30 @@ int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, cons
   return guarded([&] {
     litho::Plan& p = plan_of(plan);
     require(theta != nullptr && target != nullptr && stage_ms != nullptr, "null argument");
+    require(batch >= 1 && batch <= p.capacity, "batch exceeds the plan capacity");
     set_device(p);
+    hipStream_t s = p.ctx->stream;
+    // the evaluation of an L-BFGS trial (LbfgsPool in ctm_driver.cpp):
+    // theta = base + step * d (step 0, d = 0), theta written out, the theta
+    // gradient stored and g . d formed
+    const size_t n2 = p.tile_elems() * (size_t)batch;
+    litho::DevMem x(n2 * sizeof(float)), g(n2 * sizeof(float)), d(n2 * sizeof(float)), step(batch * sizeof(float));
+    LITHO_HIP(hipMemsetAsync(d.ptr, 0, d.bytes, s));
+    LITHO_HIP(hipMemsetAsync(step.ptr, 0, step.bytes, s));
     litho::EvalRequest rq;
     rq.mode = 1;
     rq.base = theta;
+    rq.dir = d.as<float>();
+    rq.coef = step.as<float>();
+    rq.theta_out = x.as<float>();
     rq.im.thr = (float)threshold;
     rq.im.alpha = (float)alpha;
     rq.im.target = target;
     rq.im.pe_weight = 1.f;
     rq.ga.scale = (float)(4.0 * alpha);
-    rq.ga.theta = theta;
+    rq.ga.theta = x.as<float>();
+    rq.ga.grad_theta = g.as<float>();
+    rq.ga.dir = d.as<float>();
     litho::profile_eval(p, batch, rq, repeats, stage_ms);
   });
 }
@@ -502,6 +557,76 @@ int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, doubl
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = (double)t / repeats;
+  });
+}
+
+int litho_lbfgs_direction(litho_ctx* ctx, size_t n, int m, int num_points, const float* x_points,
+                          const float* g_points, float* d_out, double* gd_out, int* kept_out) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    require(x_points != nullptr && g_points != nullptr && d_out != nullptr, "null argument");
+    require(n > 0 && n % 4 == 0, "n must be a positive multiple of 4");
+    require(m >= 1 && m <= litho::kMaxHist, "history m out of range");
+    require(num_points >= 1, "at least one point is required");
+    hipStream_t s = c.stream;
+    LITHO_HIP(hipSetDevice(c.device));
+    // the buffers of one L-BFGS slot (LbfgsPool in ctm_driver.cpp)
+    litho::DevMem x, xp, g, gp, hs, hy, need, meta, sy, yy, coef, pair_part, dir_part, vec_red, gd;
+    for (litho::DevMem* v : {&x, &xp, &g, &gp}) v->alloc(n * sizeof(float));
+    for (litho::DevMem* v : {&hs, &hy}) {
+      v->alloc(litho::hist_tile_stride(n) * sizeof(float));
+      LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
+    }
+    need.alloc(sizeof(int));
+    meta.alloc(litho::kMetaStride * sizeof(int));
+    sy.alloc(litho::kMaxHist * litho::kMaxHist * sizeof(double));
+    yy.alloc(litho::kMaxHist * litho::kMaxHist * sizeof(double));
+    coef.alloc(litho::kCoefStride * sizeof(double));
+    pair_part.alloc((size_t)litho::kOptBlocks * litho::kPairDots * sizeof(double));
+    dir_part.alloc((size_t)litho::kOptBlocks * sizeof(double));
+    vec_red.alloc(litho::kPairDots * sizeof(double));
+    gd.alloc(sizeof(double));
+    for (litho::DevMem* v : {&meta, &sy, &yy, &coef}) LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
+    const int one = 1;
+    LITHO_HIP(hipMemcpyAsync(need.ptr, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    auto copy = [&](litho::DevMem& dst, const float* src) {
+      LITHO_HIP(hipMemcpyAsync(dst.ptr, src, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    };
+    int mb = 0;
+    for (int j = 1; j < num_points; ++j) {
+      // pair j: s = x_j - x_{j-1}, y = g_j - g_{j-1}; the dots of the history
+      // with g_j give the direction coefficients of the newest point
+      copy(x, x_points + (size_t)j * n);
+      copy(xp, x_points + (size_t)(j - 1) * n);
+      copy(g, g_points + (size_t)j * n);
+      copy(gp, g_points + (size_t)(j - 1) * n);
+      mb = std::min(j - 1, m);
+      litho::launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(),
+                               hy.as<float>(), need.as<int>(), meta.as<int>(), n, 1, mb, false,
+                               pair_part.as<double>(), s);
+      litho::launch_reduce_fixed(pair_part.as<double>(), litho::kOptBlocks, 4 * (mb + 1), vec_red.as<double>(),
+                                 litho::kPairDots, 1, s);
+      litho::launch_lbfgs_update(1, need.as<int>(), vec_red.as<double>(), mb, m, meta.as<int>(), sy.as<double>(),
+                                 yy.as<double>(), coef.as<double>(), s);
+    }
+    if (num_points == 1) {
+      // no pair: d = -g (cg = -1)
+      const double cg = -1.0;
+      LITHO_HIP(hipMemcpyAsync(coef.ptr, &cg, sizeof(double), hipMemcpyHostToDevice, s));
+      mb = -1;
+    }
+    const int mb_dir = std::min(mb + 1, m);
+    litho::launch_lbfgs_direction(g_points + (size_t)(num_points - 1) * n, hs.as<float>(), hy.as<float>(),
+                                  need.as<int>(), meta.as<int>(), coef.as<double>(), n, d_out, 1, mb_dir,
+                                  dir_part.as<double>(), s);
+    litho::launch_reduce_fixed(dir_part.as<double>(), litho::kOptBlocks, 1, gd.as<double>(), 1, 1, s);
+    double gdv = 0.0;
+    int hm = 0;
+    LITHO_HIP(hipMemcpyAsync(&gdv, gd.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+    LITHO_HIP(hipMemcpyAsync(&hm, meta.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    LITHO_HIP(hipStreamSynchronize(s));
+    if (gd_out != nullptr) *gd_out = gdv;
+    if (kept_out != nullptr) *kept_out = hm;
   });
 }
 

@@ -39,9 +39,11 @@ struct Window {
 SourceSet make_source_set(const OpticsParams& op);
 Window socs_window(const OpticsParams& op, const SourceSet& src);
 
-// Device precompute (rocBLAS zherk + rocSOLVER zheevdx); outputs as in
-// litho_socs_precompute (include/litho_amd.h).
+// Device precompute (rocBLAS zherk + rocSOLVER zheevx); outputs as in
+// litho_socs_precompute (include/litho_amd.h).  eigen_gap_out (nullable):
+// (lambda_K - lambda_{K+1}) / lambda_K, 1 when K is the number of source
+// points; a gap at rounding level (a split degenerate eigenvalue) throws.
 void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, double* kernels_out,
-                            double* weights_out, double* singular_values_out);
+                            double* weights_out, double* singular_values_out, double* eigen_gap_out);
 
 }  // namespace litho

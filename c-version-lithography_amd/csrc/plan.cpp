@@ -168,7 +168,19 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
 
 EvalGraph::EvalGraph(Plan& plan, int batch, const EvalRequest& rq) : plan_(plan), batch_(batch), rq_(rq) {
   if (batch <= 0 || batch > plan.capacity) throw Error(1, "batch exceeds the plan capacity");
-  if (!plan.ctx->graphs) return;
+  if (!plan.ctx->graphs) {
+    plan.ctx->graphs_direct.fetch_add(1);
+    return;
+  }
+  capture();
+  if (exec_ != nullptr)
+    plan.ctx->graphs_captured.fetch_add(1);
+  else
+    plan.ctx->graphs_failed.fetch_add(1);
+}
+
+void EvalGraph::capture() {
+  Plan& plan = plan_;
   hipStream_t s = plan.ctx->stream;
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     (void)hipGetLastError();
@@ -176,7 +188,7 @@ EvalGraph::EvalGraph(Plan& plan, int batch, const EvalRequest& rq) : plan_(plan)
   }
   bool ok = true;
   try {
-    run_eval(plan, batch, rq);
+    run_eval(plan, batch_, rq_);
   } catch (const Error&) {
     ok = false;
   }

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdio>
 #include <stdexcept>
@@ -86,6 +87,20 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool graphs = true;    // replay repeated evaluations from a captured hipGraph
+  // evaluation graphs built on this context: captured, failed to capture
+  // (launched directly instead) and direct by choice (graphs off)
+  std::atomic<long long> graphs_captured{0}, graphs_failed{0}, graphs_direct{0};
+  std::atomic<int> live_plans{0};   // plans made on this context and not yet destroyed
+  // HIP-event timing of the L-BFGS pair-kernel launches of this context's
+  // optimizer rounds (off by default)
+  bool time_pair = false;
+  std::atomic<long long> pair_launches{0};
+  std::atomic<double> pair_ms{0.0};
+  void add_pair_ms(double v) {
+    double cur = pair_ms.load();
+    while (!pair_ms.compare_exchange_weak(cur, cur + v)) {
+    }
+  }
   DevMem scratch;        // grow-only workspace of the level-set and tiling calls
   // At least `bytes` of workspace.  Growing it waits for the stream first, so
   // work still queued on the old buffer finishes before it is freed.
@@ -187,6 +202,7 @@ class EvalGraph {
   bool captured() const { return exec_ != nullptr; }
 
  private:
+  void capture();
   Plan& plan_;
   int batch_;
   EvalRequest rq_;

@@ -64,6 +64,11 @@ __global__ void k_scale_columns(double2* __restrict__ u, size_t ld, int rows, co
   }
 }
 
+// Smallest accepted relative gap (lambda_K - lambda_{K+1}) / lambda_K.  An
+// exactly degenerate pair comes back with a gap of a few FP64 ulps, far
+// below this.
+constexpr double kMinEigenGap = 1e-9;
+
 void rb(rocblas_status st, const char* what) {
   if (st != rocblas_status_success) throw Error(3, std::string(what) + " failed with rocblas status " +
                                                        std::to_string((int)st));
@@ -72,7 +77,7 @@ void rb(rocblas_status st, const char* what) {
 }  // namespace
 
 void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, double* kernels_out,
-                            double* weights_out, double* singular_values_out) {
+                            double* weights_out, double* singular_values_out, double* eigen_gap_out) {
   const SourceSet src = make_source_set(op);
   const Window win = socs_window(op, src);
   const int n = op.n;
@@ -160,22 +165,37 @@ void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, d
     LITHO_HIP(hipMemcpyAsync(all.data(), d_w.ptr, ns * sizeof(double), hipMemcpyDeviceToHost, stream));
     LITHO_HIP(hipStreamSynchronize(stream));
   }
+  // The top K + 1 eigenpairs (K when K = Ns): eigenvalue K + 1 gives the
+  // relative gap (lambda_K - lambda_{K+1}) / lambda_K, and a truncation that
+  // splits a degenerate eigenvalue (gap at rounding level, e.g. the symmetric
+  // pairs of an annular source) is rejected: its kernel set would depend on
+  // the solver's choice of basis inside the eigenspace.
+  const int kk = k < ns ? k + 1 : k;
+  const int off = kk - k;
+  ev.resize(kk);
   DevMem d_zx((size_t)ns * ns * sizeof(double2));
   {
     DevMem d_nev(sizeof(rocblas_int)), d_ifail((size_t)ns * sizeof(rocblas_int));
     rb(rocsolver_zheevx(h, rocblas_evect_original, rocblas_erange_index, rocblas_fill_lower, ns,
-                        d_g.as<rocblas_double_complex>(), ns, 0.0, 0.0, ns - k + 1, ns, 0.0, d_nev.as<rocblas_int>(),
+                        d_g.as<rocblas_double_complex>(), ns, 0.0, 0.0, ns - kk + 1, ns, 0.0, d_nev.as<rocblas_int>(),
                         d_w.as<double>(), d_zx.as<rocblas_double_complex>(), ns, d_ifail.as<rocblas_int>(),
                         d_info.as<rocblas_int>()),
        "rocsolver_zheevx");
     read_info("rocsolver_zheevx");
     rocblas_int nev = 0;
     LITHO_HIP(hipMemcpyAsync(&nev, d_nev.ptr, sizeof(nev), hipMemcpyDeviceToHost, stream));
-    LITHO_HIP(hipMemcpyAsync(ev.data(), d_w.ptr, k * sizeof(double), hipMemcpyDeviceToHost, stream));
+    LITHO_HIP(hipMemcpyAsync(ev.data(), d_w.ptr, kk * sizeof(double), hipMemcpyDeviceToHost, stream));
     LITHO_HIP(hipStreamSynchronize(stream));
-    if (nev != k) throw Error(5, "rocsolver_zheevx returned " + std::to_string(nev) + " eigenpairs");
+    if (nev != kk) throw Error(5, "rocsolver_zheevx returned " + std::to_string(nev) + " eigenpairs");
   }
-  const rocblas_double_complex* d_z = d_zx.as<rocblas_double_complex>();
+  // ascending: ev[off .. off + k) are the top K, ev[0] the next one when off = 1
+  const double lam_k = ev[off];
+  const double gap = off ? (lam_k - ev[0]) / lam_k : 1.0;
+  if (eigen_gap_out != nullptr) *eigen_gap_out = gap;
+  if (!(gap > kMinEigenGap))
+    throw Error(1, "SOCS: K = " + std::to_string(k) + " splits a degenerate eigenvalue (relative gap " +
+                       std::to_string(gap) + "); choose another K");
+  const rocblas_double_complex* d_z = d_zx.as<rocblas_double_complex>() + (size_t)off * ns;
   if (singular_values_out != nullptr)
     for (int i = 0; i < ns; ++i) singular_values_out[i] = std::sqrt(std::max(all[ns - 1 - i], 0.0));
   {
@@ -195,8 +215,8 @@ void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, d
      "rocblas_zgemm");
   std::vector<double> inv(k);
   for (int j = 0; j < k; ++j) {
-    if (!(ev[j] > 0.0)) throw Error(5, "SOCS: non-positive eigenvalue among the top K");
-    inv[j] = 1.0 / std::sqrt(ev[j]);
+    if (!(ev[off + j] > 0.0)) throw Error(5, "SOCS: non-positive eigenvalue among the top K");
+    inv[j] = 1.0 / std::sqrt(ev[off + j]);
   }
   DevMem d_inv(k * sizeof(double));
   LITHO_HIP(hipMemcpyAsync(d_inv.ptr, inv.data(), k * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -214,7 +234,7 @@ void socs_precompute_device(const OpticsParams& op, int k, hipStream_t stream, d
   // descending order: output kernel j = eigen column k-1-j
   for (int j = 0; j < k; ++j) {
     const int col = k - 1 - j;
-    weights_out[j] = ev[col];
+    weights_out[j] = ev[off + col];
     for (size_t p = 0; p < lda; ++p) {
       kernels_out[2 * ((size_t)j * lda + p)] = u[(size_t)col * lda + p].x;
       kernels_out[2 * ((size_t)j * lda + p) + 1] = u[(size_t)col * lda + p].y;

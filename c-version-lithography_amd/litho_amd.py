@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -120,7 +121,11 @@ SIGNATURES = {
     "litho_ctx_synchronize": (_I, [_P]),
     "litho_ctx_stream": (_P, [_P]),
     "litho_ctx_set_graphs": (_I, [_P, _I]),
+    "litho_ctx_time_pair_kernel": (_I, [_P, _I]),
+    "litho_ctx_pair_kernel_time": (_I, [_P, C.POINTER(C.c_longlong), _DP]),
+    "litho_ctx_graph_counts": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "litho_socs_window": (_I, [_I, _D, _D, _D, _D, _D, _IP, _IP, _IP, _IP, _IP, _DP]),
+    "litho_socs_last_eigen_gap": (_D, []),
     "litho_socs_precompute": (_I, [_P, _I, _D, _D, _D, _D, _D, _I, _DP, _DP, _DP]),
     "litho_socs_precompute_zernike": (_I, [_P, _I, _D, _D, _D, _D, _D, _IP, _DP, _I, _I, _DP, _DP, _DP]),
     "litho_plan_create": (_I, [_P, _I, _I, _I, _I, _I, _I, _DP, _DP, _D, _I, C.POINTER(_P)]),
@@ -140,6 +145,7 @@ SIGNATURES = {
     "litho_initialize_theta": (_I, [_P, _P, _P, C.c_size_t]),
     "litho_profile_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _I, _DP]),
     "litho_profile_lbfgs_pair": (_I, [_P, C.c_size_t, _I, _I, _DP]),
+    "litho_lbfgs_direction": (_I, [_P, C.c_size_t, _I, _I, _P, _P, _P, _DP, _IP]),
     "litho_ls_terms": (_I, [_P, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P]),
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
     "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
@@ -204,9 +210,12 @@ class Context:
         h = _P()
         check(self.lib.litho_ctx_create(device, C.byref(h)))
         self.h = h
+        self._plans = weakref.WeakSet()   # plans made on this context: closed before it
 
     def close(self):
         if self.h:
+            for p in list(self._plans):
+                p.close()
             check(self.lib.litho_ctx_destroy(self.h))
             self.h = None
 
@@ -216,11 +225,38 @@ class Context:
     def set_graphs(self, enable: bool):
         check(self.lib.litho_ctx_set_graphs(self.h, 1 if enable else 0))
 
+    def time_pair_kernel(self, enable: bool):
+        """Start (or stop) HIP-event timing of the L-BFGS pair kernel in this context's solves; resets totals."""
+        check(self.lib.litho_ctx_time_pair_kernel(self.h, 1 if enable else 0))
+
+    def pair_kernel_time(self):
+        """(launches, total ms) of the pair-kernel launches timed since time_pair_kernel(True)."""
+        n = C.c_longlong(0)
+        ms = C.c_double(0.0)
+        check(self.lib.litho_ctx_pair_kernel_time(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def graph_counts(self):
+        """Evaluation graphs built on this context: dict(captured, failed, direct)."""
+        v = [C.c_longlong(0) for _ in range(3)]
+        check(self.lib.litho_ctx_graph_counts(self.h, *[C.byref(x) for x in v]))
+        return dict(captured=v[0].value, failed=v[1].value, direct=v[2].value)
+
     def profile_lbfgs_pair(self, n, m=10, repeats=20):
         """HIP-event time (ms) of one L-BFGS pair-kernel launch over n floats with m pairs."""
         ms = C.c_double(0.0)
         check(self.lib.litho_profile_lbfgs_pair(self.h, n, m, repeats, C.byref(ms)))
         return ms.value
+
+    def lbfgs_direction(self, x_points, g_points, d_out, m=10):
+        """litho_lbfgs_direction: x_points, g_points (P, n) float32 device tensors; d_out (n,).
+        Returns (g . d, pairs kept)."""
+        p, n = x_points.shape
+        gd = C.c_double(0.0)
+        kept = C.c_int(0)
+        check(self.lib.litho_lbfgs_direction(self.h, n, m, p, ptr(x_points), ptr(g_points), ptr(d_out),
+                                             C.byref(gd), C.byref(kept)))
+        return gd.value, kept.value
 
     def __del__(self):
         try:
@@ -246,7 +282,8 @@ class Context:
             check(self.lib.litho_socs_precompute(self.h, n, pixel_nm, wavelength_nm, na, sigma_in, sigma_out, k,
                                                  _dp(kern), _dp(w), _dp(sv)))
         out = dict(win)
-        out.update(kernels=kern[..., 0] + 1j * kern[..., 1], weights=w, singular_values=sv)
+        out.update(kernels=kern[..., 0] + 1j * kern[..., 1], weights=w, singular_values=sv,
+                   eigen_gap=self.lib.litho_socs_last_eigen_gap())
         return out
 
 
@@ -408,6 +445,7 @@ class Plan:
         check(self.lib.litho_plan_create(ctx.h, n, k, wr, wc, win_y0, win_x0, _dp(kd), _dp(w),
                                          float(weight_sum), capacity, C.byref(h)))
         self.h = h
+        ctx._plans.add(self)
         self.n, self.k, self.capacity = n, k, capacity
 
     @classmethod

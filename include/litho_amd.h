@@ -51,6 +51,7 @@ int litho_supported_size(int n);
 
 /* Device context: one HIP device and one stream (default priority). */
 int litho_ctx_create(int device, litho_ctx** out);
+/* Fails (LITHO_EINVAL, nothing destroyed) while plans made on the context exist. */
 int litho_ctx_destroy(litho_ctx* ctx);
 int litho_ctx_synchronize(litho_ctx* ctx);
 /* The context's hipStream_t, for callers that order their own work on it. */
@@ -58,6 +59,19 @@ void* litho_ctx_stream(litho_ctx* ctx);
 /* Optimizer loops replay each repeated evaluation from a captured hipGraph
  * (default on); 0 launches the same kernels one by one.  Results are identical. */
 int litho_ctx_set_graphs(litho_ctx* ctx, int enable);
+/* Evaluation graphs the optimizer pools of this context have built so far
+ * (no reference counterpart): captured and replayed, failed to capture and
+ * launched kernel by kernel instead, and launched directly because graphs are
+ * off.  Any pointer may be null. */
+int litho_ctx_graph_counts(litho_ctx* ctx, long long* captured, long long* failed, long long* direct);
+/* HIP-event timing of the L-BFGS pair kernel inside the L-BFGS solves run on
+ * this context (litho_ctm_optimize, litho_batch_ctm, the server; no reference
+ * counterpart): with enable != 0 each round's pair-kernel launch over a full
+ * history (m pairs held: the steady-state instantiation) is bracketed by two
+ * events on the context stream and its duration accumulated.  The call
+ * resets the totals; litho_ctx_pair_kernel_time reads them (nullable). */
+int litho_ctx_time_pair_kernel(litho_ctx* ctx, int enable);
+int litho_ctx_pair_kernel_time(litho_ctx* ctx, long long* launches, double* total_ms);
 
 /* ---------------------------------------------------------------- optics
  * Compact SOCS frequency window of the reference optics
@@ -79,6 +93,15 @@ int litho_socs_window(int n, double pixel_nm, double wavelength_nm, double na, d
 int litho_socs_precompute(litho_ctx* ctx, int n, double pixel_nm, double wavelength_nm, double na,
                           double sigma_in, double sigma_out, int k, double* kernels_out,
                           double* weights_out, double* singular_values_out);
+
+/* Relative eigen-gap at the truncation, (lambda_K - lambda_{K+1}) / lambda_K, of
+ * the last litho_socs_precompute[_zernike] call on this thread (1 when K is the
+ * number of source points; 0 before any call).  The reference keeps the top K
+ * singular vectors without a check (litho_prepare.cpp:298-330); here a K that
+ * splits a degenerate eigenvalue (gap below 1e-9, as the symmetric pairs of an
+ * annular source give) fails with LITHO_EINVAL, since its kernel set would
+ * depend on the solver's basis inside the eigenspace. */
+double litho_socs_last_eigen_gap(void);
 
 /* The same with pupil aberrations, Pupil (pupil.cpp:5-46) with
  * ZernikeGenerator::wavefront (zernike.cpp:11-53): P = [rho^2 <= 1]
@@ -233,6 +256,18 @@ int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, cons
  * 4 written).  *ms = mean over `repeats` back-to-back launches, by HIP events
  * on the context stream. */
 int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, double* ms);
+
+/* The L-BFGS history and inverse-Hessian product of CTM_Optimizer::optimize_lbfgs
+ * (CTM_Optimizer.cpp:194-212), i.e. LBFGS++'s BFGSMat::add_correction and
+ * apply_Hv, on the device kernels that litho_ctm_optimize runs: for the
+ * iterates (x_j, g_j), j < num_points, the pairs s = x_j - x_{j-1},
+ * y = g_j - g_{j-1} are added oldest first (kept when s.y > eps y.y; beyond m
+ * the oldest leaves), then d = -H g_last with H0 = (s'y / y'y) I of the newest
+ * kept pair (d = -g_last when none is kept).  x_points, g_points: device
+ * num_points x n floats (n a multiple of 4); d_out: device n floats.
+ * gd_out (host, nullable): g_last . d in FP64; kept_out (host, nullable): pairs held. */
+int litho_lbfgs_direction(litho_ctx* ctx, size_t n, int m, int num_points, const float* x_points,
+                          const float* g_points, float* d_out, double* gd_out, int* kept_out);
 
 /* CTM_Optimizer::initialize_theta (CTM_Optimizer.cpp:66-73): 0.3 pi where
  * pattern > 0, else 0.7 pi.  Device arrays of n*n floats. */

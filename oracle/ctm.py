@@ -3,13 +3,20 @@
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 Restates source/optimizer/CTM_Optimizer.cpp.  The L-BFGS solver is the
-reference's third-party LBFGS++ (yixuan/LBFGSpp @ c524a407, "v0.4.0"), which is
-not vendored: this file restates its published algorithm (two-loop recursion
-with H0 = (s'y / y'y) I, pairs kept only when s'y > eps*y'y, first step
-1/||d||, later steps 1, strong-Wolfe line search of Nocedal & Wright
-Alg. 3.5/3.6 with ftol 1e-4, wolfe 0.9, max 20 trials).  PARITY UNPINNED for
-the line-search trajectory beyond the reference invariants of
-tests/test_ctm_optimizer.cpp:181-196.
+reference's third-party LBFGS++ (yixuan/LBFGSpp @ c524a407, "v0.4.0",
+CMakeLists.txt:98-115), which is not vendored.  CTM_Optimizer.cpp:194-200
+constructs LBFGSSolver<double> with the default line-search class,
+LineSearchNocedalWright, and LBFGSParam defaults apart from m, epsilon,
+epsilon_rel = 0 and max_iterations.  This file restates that published
+algorithm: two-loop recursion with H0 = (s'y / y'y) I of the newest kept pair,
+pairs kept only when s'y > eps*y'y, first step 1/||d||, later steps 1, and the
+Nocedal-Wright strong-Wolfe search with expansion 2, quadratic-interpolation
+zoom, ftol 1e-4, wolfe 0.9, max_linesearch 20.
+Pinned here by mathematics (tests/test_oracle_ctm.py): the two-loop direction
+equals the explicit BFGS inverse-Hessian product, every accepted step meets
+the strong-Wolfe conditions, and a 1-D trace follows the interpolation rule
+computed in exact rational arithmetic.  PARITY UNPINNED against LBFGS++ itself
+(its source is absent, so the restatement cannot be run against it).
 """
 from __future__ import annotations
 
@@ -68,134 +75,179 @@ class LBFGSParams:
         self.epsilon_rel = 0.0
         self.max_iterations = max_iterations
         self.max_linesearch = 20
-        self.min_step = 1e-20
-        self.max_step = 1e20
         self.ftol = 1e-4
         self.wolfe = 0.9
 
 
-def _cubic_min(a_lo, f_lo, g_lo, a_hi, f_hi, g_hi):
-    """Minimiser of the cubic interpolating (a, f, f') at both ends
-    (Nocedal & Wright eq. 3.59); None when it does not exist."""
-    d1 = g_lo + g_hi - 3.0 * (f_lo - f_hi) / (a_lo - a_hi)
-    rad = d1 * d1 - g_lo * g_hi
-    if rad < 0.0:
-        return None
-    d2 = math.copysign(math.sqrt(rad), a_hi - a_lo)
-    den = g_hi - g_lo + 2.0 * d2
-    if den == 0.0:
-        return None
-    return a_hi - (a_hi - a_lo) * (g_hi + d2 - d1) / den
+def quad_interp(step_lo, step_hi, fx_lo, fx_hi, dg_lo):
+    """Zoom trial of LBFGS++'s LineSearchNocedalWright (v0.4.0, restated):
+    the minimiser of the quadratic through (step_lo, fx_lo) with slope dg_lo
+    at step_lo and through (step_hi, fx_hi), written as
+        (fdiff * step_lo - smid * sdiff * dg_lo) / (fdiff - sdiff * dg_lo)
+    with fdiff = fx_hi - fx_lo, sdiff = step_hi - step_lo, smid the midpoint.
+    The midpoint (bisection) is taken instead when that value is not finite,
+    lies on or outside the bracket, or within 1 % of the bracket width of
+    either end."""
+    fdiff = fx_hi - fx_lo
+    sdiff = step_hi - step_lo
+    smid = (step_hi + step_lo) / 2.0
+    num = fdiff * step_lo - smid * sdiff * dg_lo
+    den = fdiff - sdiff * dg_lo
+    if den == 0.0 or not math.isfinite(num) or not math.isfinite(den):
+        cand = math.nan
+    else:
+        cand = num / den
+    if not math.isfinite(cand):
+        return smid
+    end_dist = min(abs(cand - step_lo), abs(cand - step_hi))
+    near_end = end_dist < 0.01 * abs(sdiff)
+    if cand <= min(step_lo, step_hi) or cand >= max(step_lo, step_hi) or near_end:
+        return smid
+    return cand
 
 
-def zoom_trial(a_lo, f_lo, g_lo, a_hi, f_hi, g_hi):
-    """Safeguarded cubic trial inside the bracket (shared rule of oracle and host)."""
-    lo, hi = min(a_lo, a_hi), max(a_lo, a_hi)
-    width = hi - lo
-    a = _cubic_min(a_lo, f_lo, g_lo, a_hi, f_hi, g_hi)
-    if a is None or not math.isfinite(a) or a < lo + 0.1 * width or a > hi - 0.1 * width:
-        a = 0.5 * (lo + hi)
-    return a
+def line_search(evalf, xp, fx_init, g_init, d, step, p: LBFGSParams, trace=None):
+    """LineSearchNocedalWright::LineSearch restated (LBFGS++ v0.4.0; Nocedal &
+    Wright Alg. 3.5 bracketing with expansion 2, Alg. 3.6 zoom with
+    quad_interp trials).
 
-
-def line_search(evalf, x0, f0, g0, d, step, p: LBFGSParams):
-    """Strong-Wolfe line search (Nocedal & Wright Alg. 3.5 + 3.6).
-
-    evalf(x) -> (f, g).  Returns (step, x, f, g).  Raises LineSearchError when
-    max_linesearch trials pass without an acceptable point.
-    """
-    dg0 = float(np.dot(g0.ravel(), d.ravel()))
-    if dg0 >= 0.0:
-        raise LineSearchError("the moving direction does not decrease the objective function value")
-    a_prev, f_prev, dg_prev = 0.0, f0, dg0
-    a = step
-    trials = 0
+    evalf(x) -> (f, g).  Returns (step, x, f, g).  The search ends at the first
+    trial meeting the strong-Wolfe conditions.  When max_linesearch
+    non-terminal trials are used up it returns the best point of sufficient
+    decrease seen (step_lo), as LBFGS++ does; it raises LineSearchError where
+    LBFGS++ throws (ascent direction, no step of sufficient decrease, a zoom
+    trial equal to a bracket end).  trace (a list, optional) receives every
+    trial step."""
+    if step <= 0.0:
+        raise ValueError("'step' must be positive")
+    dg_init = float(np.dot(g_init.ravel(), d.ravel()))
+    if dg_init > 0.0:
+        raise LineSearchError("the moving direction increases the objective function value")
+    test_decr = p.ftol * dg_init
+    test_curv = -p.wolfe * dg_init
+    step_lo, fx_lo, dg_lo, x_lo, g_lo = 0.0, fx_init, dg_init, xp, g_init
+    step_hi = fx_hi = None
 
     def trial(alpha):
-        x = x0 + alpha * d
+        if trace is not None:
+            trace.append(alpha)
+        x = xp + alpha * d
         f, g = evalf(x)
         return x, f, g, float(np.dot(g.ravel(), d.ravel()))
 
-    def zoom(a_lo, f_lo, g_lo, a_hi, f_hi, g_hi, trials):
-        while trials < p.max_linesearch:
-            aj = zoom_trial(a_lo, f_lo, g_lo, a_hi, f_hi, g_hi)
-            x, f, g, dg = trial(aj)
-            trials += 1
-            if f > f0 + p.ftol * aj * dg0 or f >= f_lo:
-                a_hi, f_hi, g_hi = aj, f, dg
-            else:
-                if abs(dg) <= -p.wolfe * dg0:
-                    return aj, x, f, g
-                if dg * (a_hi - a_lo) >= 0.0:
-                    a_hi, f_hi, g_hi = a_lo, f_lo, g_lo
-                a_lo, f_lo, g_lo = aj, f, dg
-        raise LineSearchError("the line search routine reached the maximum number of iterations")
-
-    while trials < p.max_linesearch:
-        if a < p.min_step:
-            raise LineSearchError("the line search step became smaller than the minimum value allowed")
-        x, f, g, dg = trial(a)
-        trials += 1
-        if f > f0 + p.ftol * a * dg0 or (trials > 1 and f >= f_prev):
-            return zoom(a_prev, f_prev, dg_prev, a, f, dg, trials)
-        if abs(dg) <= -p.wolfe * dg0:
-            return a, x, f, g
+    it = 0
+    # bracketing (Alg. 3.5)
+    while True:
+        x, fx, g, dg = trial(step)
+        if fx - fx_init > step * test_decr or (0.0 < step_lo and fx >= fx_lo):
+            step_hi, fx_hi = step, fx
+            break
+        if abs(dg) <= test_curv:
+            return step, x, fx, g
+        step_hi, fx_hi = step_lo, fx_lo
+        step_lo, fx_lo, dg_lo, x_lo, g_lo = step, fx, dg, x, g
         if dg >= 0.0:
-            return zoom(a, f, dg, a_prev, f_prev, dg_prev, trials)
-        a_prev, f_prev, dg_prev = a, f, dg
-        a = min(2.0 * a, p.max_step)
-    raise LineSearchError("the line search routine reached the maximum number of iterations")
+            break
+        it += 1
+        if it >= p.max_linesearch:
+            return step_lo, x_lo, fx_lo, g_lo
+        step *= 2.0
+    # zoom (Alg. 3.6)
+    while True:
+        step = quad_interp(step_lo, step_hi, fx_lo, fx_hi, dg_lo)
+        x, fx, g, dg = trial(step)
+        if fx - fx_init > step * test_decr or fx >= fx_lo:
+            if step == step_hi:
+                raise LineSearchError("the line search routine failed, possibly due to insufficient numeric "
+                                      "precision")
+            step_hi, fx_hi = step, fx
+        else:
+            if abs(dg) <= test_curv:
+                return step, x, fx, g
+            if dg * (step_hi - step_lo) >= 0.0:
+                step_hi, fx_hi = step_lo, fx_lo
+            if step == step_lo:
+                raise LineSearchError("the line search routine failed, possibly due to insufficient numeric "
+                                      "precision")
+            step_lo, fx_lo, dg_lo, x_lo, g_lo = step, fx, dg, x, g
+        it += 1
+        if it >= p.max_linesearch:
+            if step_lo <= 0.0:
+                raise LineSearchError("the line search routine failed, unable to sufficiently decrease the "
+                                      "function value")
+            return step_lo, x_lo, fx_lo, g_lo
 
 
-def lbfgs_minimize(evalf, x, p: LBFGSParams):
+class LbfgsHistory:
+    """BFGSMat restated (LBFGS++ BFGSMat.h): at most m correction pairs,
+    add_correction keeps a pair only when s'y > eps y'y (the check in
+    LBFGSSolver::minimize) and drops the oldest beyond m; theta = y'y / s'y of
+    the newest kept pair (1 before the first); apply_Hv(g, -1) is the two-loop
+    recursion (Nocedal & Wright Alg. 7.4) with H0 = I / theta."""
+
+    def __init__(self, m):
+        self.m = m
+        self.s, self.y = [], []
+        self.theta = 1.0
+
+    def add(self, s, y) -> bool:
+        s = np.asarray(s, dtype=np.float64).ravel()
+        y = np.asarray(y, dtype=np.float64).ravel()
+        ys = float(np.dot(s, y))
+        yy = float(np.dot(y, y))
+        if not ys > np.finfo(np.float64).eps * yy:
+            return False
+        self.s.append(s)
+        self.y.append(y)
+        if len(self.s) > self.m:
+            self.s.pop(0)
+            self.y.pop(0)
+        self.theta = yy / ys
+        return True
+
+    def direction(self, g):
+        """d = -H g."""
+        q = -np.asarray(g, dtype=np.float64).ravel().copy()
+        ys = [float(np.dot(s, y)) for s, y in zip(self.s, self.y)]
+        alphas = [0.0] * len(self.s)
+        for i in range(len(self.s) - 1, -1, -1):
+            alphas[i] = float(np.dot(self.s[i], q)) / ys[i]
+            q -= alphas[i] * self.y[i]
+        q /= self.theta
+        for i in range(len(self.s)):
+            beta = float(np.dot(self.y[i], q)) / ys[i]
+            q += (alphas[i] - beta) * self.s[i]
+        return q
+
+
+def lbfgs_minimize(evalf, x, p: LBFGSParams, on_step=None):
     """LBFGSSolver::minimize restated.  evalf(x) -> (f, g); returns
     (iterations, x, f, gnorm, status) where status is 'converged',
-    'max_iterations' or 'line_search_failed'."""
+    'max_iterations' or 'line_search_failed' (where LBFGS++ throws).
+    on_step(step, xp, fp, gp, d, x, f, g), optional, sees every accepted step."""
     f, g = evalf(x)
     gnorm = float(np.linalg.norm(g))
     if gnorm <= p.epsilon or gnorm <= p.epsilon_rel * float(np.linalg.norm(x)):
         return 1, x, f, gnorm, "converged"
     d = -g
     step = 1.0 / float(np.linalg.norm(d))
-    s_list, y_list, ys_list = [], [], []
-    theta = 1.0
-    eps = np.finfo(np.float64).eps
+    hist = LbfgsHistory(p.m)
     k = 1
     while True:
-        xp, gp = x, g
+        xp, gp, fp = x, g, f
         try:
             step, x, f, g = line_search(evalf, xp, f, gp, d, step, p)
         except LineSearchError:
-            return k, xp, f, gnorm, "line_search_failed"
+            return k, xp, fp, gnorm, "line_search_failed"
+        if on_step is not None:
+            on_step(step, xp, fp, gp, d, x, f, g)
         gnorm = float(np.linalg.norm(g))
         if gnorm <= p.epsilon or gnorm <= p.epsilon_rel * float(np.linalg.norm(x)):
             return k, x, f, gnorm, "converged"
         if p.max_iterations != 0 and k >= p.max_iterations:
             return k, x, f, gnorm, "max_iterations"
-        s = (x - xp).ravel()
-        y = (g - gp).ravel()
-        ys = float(np.dot(s, y))
-        yy = float(np.dot(y, y))
-        if ys > eps * yy:
-            s_list.append(s)
-            y_list.append(y)
-            ys_list.append(ys)
-            if len(s_list) > p.m:
-                s_list.pop(0)
-                y_list.pop(0)
-                ys_list.pop(0)
-            theta = yy / ys
-        # two-loop recursion, H0 = I / theta
-        q = -g.ravel().copy()
-        alphas = [0.0] * len(s_list)
-        for i in range(len(s_list) - 1, -1, -1):
-            alphas[i] = float(np.dot(s_list[i], q)) / ys_list[i]
-            q -= alphas[i] * y_list[i]
-        q /= theta
-        for i in range(len(s_list)):
-            beta = float(np.dot(y_list[i], q)) / ys_list[i]
-            q += (alphas[i] - beta) * s_list[i]
-        d = q.reshape(g.shape)
+        hist.add(x - xp, g - gp)
+        d = hist.direction(g).reshape(g.shape)
         step = 1.0
         k += 1
 

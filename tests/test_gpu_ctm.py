@@ -2,9 +2,12 @@
 FP64 oracle and the reference's optimizer-control invariants
 (tests/test_ctm_optimizer.cpp:181-196).
 
-Tolerances: GD PE history relative <= 1e-4 per iteration for 10 iterations;
-L-BFGS final PE within 0.5 % of the oracle's (the line-search path itself is
-not pinned: LBFGS++ is not vendored).
+Tolerances: GD PE history relative <= 1e-4 per iteration; L-BFGS: the same
+number of objective evaluations as the oracle, the per-evaluation PE history
+within 1e-4 relative for the first 20 evaluations, final PE within 0.5 %; the
+device L-BFGS direction within 1e-5 relative L2 of the oracle's two-loop
+direction on the same history.  (The oracle restates LBFGS++, which is not
+vendored: agreement with LBFGS++ itself is not pinned.)
 """
 import numpy as np
 import pytest
@@ -67,6 +70,21 @@ def test_optimizer_control_invariants(gpu_ctx):
     plan.close()
 
 
+def assert_lbfgs_trajectory(r, hist, info, label):
+    """Same evaluation count, PE history within 1e-4 for the first 20
+    evaluations, final PE within 0.5 %."""
+    h = np.asarray(r["history"])
+    ref = np.asarray(hist)
+    k = min(20, len(ref))
+    dev_k = float(np.max(np.abs(h[:k] - ref[:k]) / ref[:k]))
+    dev_all = float(np.max(np.abs(h[:len(ref)] - ref[:len(h)]) / ref[:len(h)])) if len(h) == len(ref) else None
+    print(f"{label}: evaluations gpu {r['evaluations']} oracle {len(ref)}; history rel dev first {k}: {dev_k:.2e}, "
+          f"all: {dev_all}; final PE {r['final_pe']:.6f} vs {info['final_pe']:.6f}", flush=True)
+    assert r["evaluations"] == len(ref)
+    assert dev_k <= 1e-4
+    assert abs(r["final_pe"] - info["final_pe"]) <= 5e-3 * info["final_pe"]
+
+
 @pytest.mark.parametrize("n,iters", [(128, 15), (512, 10)])
 def test_lbfgs_final_pe(gpu_ctx, n, iters):
     print(f"case lbfgs n={n} iterations={iters}", flush=True)
@@ -74,11 +92,14 @@ def test_lbfgs_final_pe(gpu_ctx, n, iters):
     s, cache, tgt = case(n, 24)
     plan = la.Plan.from_socs(gpu_ctx, s)
     out = empty(n)
+    before = gpu_ctx.graph_counts()
     r = plan.ctm_optimize(dev(tgt), out[0], method="lbfgs", iterations=iters, history=10, tol=1e-2,
                           threshold=0.225, alpha=50.0)
+    after = gpu_ctx.graph_counts()
+    # the optimizer's evaluations were replayed from a captured graph
+    assert after["captured"] == before["captured"] + 1 and after["failed"] == before["failed"]
     mask, hist, theta, info = oc.lbfgs(cache, tgt.astype(np.float64), 0.225, 50.0, iters, 10, 1e-2)
-    assert abs(r["final_pe"] - info["final_pe"]) <= 5e-3 * info["final_pe"]
-    assert r["history"][0] == pytest.approx(hist[0], rel=1e-5)
+    assert_lbfgs_trajectory(r, hist, info, f"lbfgs n={n}")
     # the returned mask is the mask of the returned point
     _, w_gpu, _ = im.forward(cache, host(out)[0], 0.225, 50.0)
     assert abs(im.pe_loss(w_gpu, tgt) - r["final_pe"]) <= 1e-4 * r["final_pe"]
@@ -184,7 +205,7 @@ def test_lbfgs_2048_final_pe(gpu_ctx, kernels2048):
     print(f"2048 lbfgs: gpu PE {r['final_pe']:.4f} ({r['evaluations']} evals, status {r['status']}) "
           f"oracle PE {info['final_pe']:.4f} ({len(hist)} evals, {info['status']}); "
           f"binary mismatches {mism} of {n*n} (oracle pixels within 1e-4 of 0.5: {near})")
-    assert abs(r["final_pe"] - info["final_pe"]) <= 5e-3 * info["final_pe"]
+    assert_lbfgs_trajectory(r, hist, info, "lbfgs 2048")
     plan.close()
     im.set_threads(1)
 
@@ -200,3 +221,97 @@ def test_profile_lbfgs_pair(gpu_ctx):
         gpu_ctx.profile_lbfgs_pair(64 * 64 + 2, 3, 2)
     with pytest.raises(la.LithoError):
         gpu_ctx.profile_lbfgs_pair(64 * 64, 0, 2)
+
+
+@pytest.mark.parametrize("m,points", [(5, 14), (10, 4), (3, 1)])
+def test_lbfgs_direction_matches_two_loop(gpu_ctx, m, points):
+    """The device history update and compact-form direction (the kernels of
+    litho_ctm_optimize) against the oracle's two-loop recursion on the same
+    iterates: more pairs than m (the oldest leave), one pair with s.y < 0
+    (rejected), and no pair at all (d = -g)."""
+    import torch
+    print(f"case lbfgs_direction m={m} points={points}", flush=True)
+    n = 64 * 64
+    rng = np.random.default_rng(m * 100 + points)
+    diag = (0.5 + rng.random(n)).astype(np.float32)
+    xs = [rng.standard_normal(n).astype(np.float32)]
+    gs = [(diag * xs[0] + 0.1 * rng.standard_normal(n)).astype(np.float32)]
+    for j in range(1, points):
+        x = (xs[-1] + 0.3 * rng.standard_normal(n)).astype(np.float32)
+        sgn = -1.0 if j == 2 else 1.0          # the pair (x_2 - x_1, g_2 - g_1) has s.y < 0
+        g = (gs[-1] + sgn * diag * (x - xs[-1]) + 0.01 * rng.standard_normal(n)).astype(np.float32)
+        xs.append(x)
+        gs.append(g)
+    hist = oc.LbfgsHistory(m)
+    for j in range(1, points):
+        s = (xs[j] - xs[j - 1]).astype(np.float64)      # float32 differences, as the device forms them
+        y = (gs[j] - gs[j - 1]).astype(np.float64)
+        hist.add(s, y)
+    want = hist.direction(gs[-1].astype(np.float64))
+    d = torch.zeros(n, dtype=torch.float32, device="cuda")
+    gd, kept = gpu_ctx.lbfgs_direction(dev(np.stack(xs)), dev(np.stack(gs)), d, m=m)
+    got = host(d)
+    err = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    print(f"lbfgs_direction m={m} points={points}: kept {kept}/{len(hist.s)}, rel L2 {err:.2e}, "
+          f"g.d {gd:.6e} vs {float(np.dot(gs[-1], want)):.6e}")
+    assert kept == len(hist.s)
+    assert err <= 1e-5
+    assert abs(gd - float(np.dot(gs[-1].astype(np.float64), want))) <= 1e-5 * abs(gd)
+
+
+def test_gd_c1_trajectory_512(gpu_ctx):
+    """The CPU configuration: 512^2, K = 24 device-precomputed kernels, GD lr 0.9,
+    50 iterations, tol 0.
+
+    This fixed-step trajectory amplifies rounding: the FP64 oracle itself,
+    with theta only rounded to float32 after each step (a 6e-8 relative
+    change), moves its PE history by 1.4e-7 at iteration 20, 9e-4 at 35 and its
+    masks by up to 0.05.  So the per-iteration bound 1e-4 is asserted over the
+    first 20 iterations; over all 50 the history stays within 1e-2, the best
+    PE within 0.5 % of the oracle's, at the same iteration."""
+    import os
+    print("case gd C1 512 k=24 50 iterations", flush=True)
+    la = litho_amd()
+    n = 512
+    dk = gpu_ctx.socs_precompute(n, 4.0, 193.0, 1.35, 0.6, 0.9, 24)
+    sk = op.SocsKernels(n=n, win_y0=dk["y0"], win_x0=dk["x0"], kernels=dk["kernels"], weights=dk["weights"],
+                        weight_sum=dk["weight_sum"])
+    plan = la.Plan.from_socs(gpu_ctx, sk)
+    tgt = targets().manhattan_target(n, 512)
+    out = empty(n)
+    r = plan.ctm_optimize(dev(tgt), out[0], method="gradient_descent", iterations=50, learning_rate=0.9, tol=0.0,
+                          threshold=0.225, alpha=50.0)
+    plan.close()
+    im.set_threads(len(os.sched_getaffinity(0)))
+    th0 = oc.initialize_theta(tgt).astype(np.float32).astype(np.float64)   # the device's float32 theta0
+    best, hist, theta = oc.gradient_descent(im.Cache.from_socs(sk), tgt.astype(np.float64), 0.225, 50.0, 50, 0.9,
+                                            0.0, theta0=th0)
+    im.set_threads(1)
+    h = np.asarray(r["history"])
+    ref = np.asarray(hist)
+    rel = np.abs(h - ref) / ref
+    mism = int(np.sum((host(out)[0] >= 0.5) != (best >= 0.5)))
+    print(f"C1 GD 512: PE {h[0]:.3f} -> {h[-1]:.3f}; history rel dev first 20 {rel[:20].max():.2e}, all "
+          f"{rel.max():.2e} (iterations 25/35/49: {rel[25]:.1e}/{rel[35]:.1e}/{rel[49]:.1e}); best iteration "
+          f"{int(np.argmin(h))} vs {int(np.argmin(ref))}, best PE {h.min():.3f} vs {ref.min():.3f}; "
+          f"binary mismatches of the best masks {mism} of {n * n}")
+    assert r["evaluations"] == 50
+    assert rel[:20].max() <= 1e-4
+    assert rel.max() <= 1e-2
+    assert abs(h.min() - ref.min()) <= 5e-3 * ref.min()
+    assert int(np.argmin(h)) == int(np.argmin(ref))
+
+
+def test_precompute_eigen_gap(gpu_ctx):
+    """The relative gap at K equals the oracle SVD's, and a K that splits a
+    degenerate eigenvalue pair (K = 21 at 512^2: lambda_21 = lambda_22) is
+    rejected."""
+    print("case eigen gap 512", flush=True)
+    la = litho_amd()
+    dk = gpu_ctx.socs_precompute(512, 4.0, 193.0, 1.35, 0.6, 0.9, 24)
+    lam = socs(512, 24).singular_values ** 2
+    want = (lam[23] - lam[24]) / lam[23]
+    print(f"eigen gap 512 K=24: gpu {dk['eigen_gap']:.6f} oracle {want:.6f}")
+    assert abs(dk["eigen_gap"] - want) <= 1e-6 * want
+    with pytest.raises(la.LithoError):
+        gpu_ctx.socs_precompute(512, 4.0, 193.0, 1.35, 0.6, 0.9, 21)
