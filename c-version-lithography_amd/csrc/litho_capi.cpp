@@ -12,6 +12,7 @@
 #include "ctm_driver.hpp"
 #include "fullchip.hpp"
 #include "lsm.hpp"
+#include "noalloc.hpp"
 #include "optics.hpp"
 #include "opt_kernels.hpp"
 #include "plan.hpp"
@@ -117,6 +118,7 @@ int litho_ctx_create(int device, litho_ctx** out) {
     LITHO_HIP(hipGetDeviceCount(&count));
     require(device >= 0 && device < count, "device index out of range");
     LITHO_HIP(hipSetDevice(device));
+    litho::noalloc_init(device);   // this device's address of the allocation-free entry points' scratch
     auto* c = new litho_ctx();
     c->c.device = device;
     if (hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking) != hipSuccess) {
@@ -658,6 +660,15 @@ int litho_ls_reinit_sdf(litho_ctx* ctx, int batch, int rows, int cols, const flo
     litho::Context& c = ctx_of(ctx);
     litho::ls_reinit_sdf(c, batch, rows, cols, phi, dx, dy, sdf);
     LITHO_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int litho_ls_reinit_sdf_noalloc(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, double dx,
+                                double dy, float* sdf) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    LITHO_HIP(hipSetDevice(c.device));
+    litho::ls_reinit_sdf_noalloc(c, batch, rows, cols, phi, dx, dy, sdf);
   });
 }
 

@@ -149,6 +149,7 @@ SIGNATURES = {
     "litho_ls_terms": (_I, [_P, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P]),
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
     "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
+    "litho_ls_reinit_sdf_noalloc": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
     "litho_render_levelset_mask": (_I, [_P, _I, _I, _I, _P, _I, _I, _P]),
     "litho_micro_imaging": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _D, _D, _P, _P, _P, _P]),
     "litho_lsm_step": (_I, [_P, _I, _I, _P, _P, _IP, C.POINTER(LsmOptions), _I, _P, _P, _DP]),
@@ -297,6 +298,11 @@ class Context:
     def reinit_sdf(self, phi, sdf, dx=1.0, dy=1.0):
         b, r, c = _shape3(phi)
         check(self.lib.litho_ls_reinit_sdf(self.h, b, r, c, ptr(phi), dx, dy, ptr(sdf)))
+
+    def reinit_sdf_noalloc(self, phi, sdf, dx=1.0, dy=1.0):
+        """litho_ls_reinit_sdf_noalloc: reinit_sdf on the static per-GPU scratch (no device allocation)."""
+        b, r, c = _shape3(phi)
+        check(self.lib.litho_ls_reinit_sdf_noalloc(self.h, b, r, c, ptr(phi), dx, dy, ptr(sdf)))
 
     def render_levelset_mask(self, phi, mask_out, mask_type="Binary", scale=4):
         """render_levelset_mask of (batch, rows, cols) tiles; mask_type "Binary" or "Gray"."""

@@ -297,6 +297,17 @@ double litho_ls_dt(double cfl, double dx, double dy, double max_h1, double max_h
 int litho_ls_reinit_sdf(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, double dx, double dy,
                         float* sdf);
 
+/* litho_ls_reinit_sdf without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its scratch (chamfer row buffers and distance planes, 16 bytes per pixel) is
+ * a static 64 MiB device array per GPU, held until the process ends: calls
+ * with batch * rows * cols above 4 Mi pixels fail with LITHO_EINVAL.  Calls
+ * that overlap on one GPU (any contexts) share that array and race.  Results
+ * equal litho_ls_reinit_sdf's bit for bit. */
+int litho_ls_reinit_sdf_noalloc(litho_ctx* ctx, int batch, int rows, int cols, const float* phi, double dx,
+                                double dy, float* sdf);
+
 typedef struct {
   double dx, dy;             /* FullChipLSMOptions dx / dy (full_chip_lsm.h:8-66) */
   double cfl;                /* CFL number */
