@@ -277,13 +277,52 @@ void check_chip(const Plan& plan, const litho_tile* tiles, int T, int height, in
 
 }  // namespace
 
+namespace {
+
+// Imaging::compute of T local tiles in batches of the plan capacity.
+void image_tiles(Plan& plan, int T, const float* masks, double threshold, double alpha, float* aerial,
+                 float* wafer) {
+  const size_t plane = plan.tile_elems();
+  for (int c = 0; c < T; c += plan.capacity) {
+    const int b = std::min(plan.capacity, T - c);
+    EvalRequest rq;
+    rq.mode = 0;
+    rq.base = masks + (size_t)c * plane;
+    rq.adjoint = false;
+    rq.im.thr = (float)threshold;
+    rq.im.alpha = (float)alpha;
+    rq.im.aerial = aerial + (size_t)c * plane;
+    rq.im.wafer = wafer + (size_t)c * plane;
+    run_eval(plan, b, rq);
+  }
+}
+
+int max_core_elems(const litho_tile* tiles, int T) {
+  int m = 0;
+  for (int i = 0; i < T; ++i) m = std::max(m, area(tiles[i].core_local));
+  return m;
+}
+
+// Loss::pe_loss (loss.cpp:16-23) of two device images, FP64, fixed order.
+double pe_loss_device(const float* wafer, const float* target, size_t count, hipStream_t s) {
+  constexpr int kBlocks = 1024;
+  DevMem part(kBlocks * sizeof(double)), out(sizeof(double));
+  launch_sq_diff_partials(wafer, target, count, part.as<double>(), kBlocks, s);
+  launch_reduce_fixed(part.as<double>(), kBlocks, 1, out.as<double>(), 1, 1, s);
+  double v = 0.0;
+  LITHO_HIP(hipMemcpyAsync(&v, out.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+  LITHO_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+}  // namespace
+
 void micro_imaging(Plan& plan, const litho_tile* tiles, int T, int height, int width, const float* global_mask,
                    double threshold, double alpha, float* global_aerial, float* global_wafer, float* local_aerial,
                    float* local_wafer) {
   check_chip(plan, tiles, T, height, width, global_mask, global_mask);
   if (!std::isfinite(threshold) || !std::isfinite(alpha)) throw Error(1, "Imaging: threshold and alpha must be finite");
   const int n = plan.n;
-  const size_t plane = (size_t)n * n;
   hipStream_t s = plan.ctx->stream;
   const std::vector<int> tab = tile_table(tiles, T);
   DevMem table(tab.size() * sizeof(int)), masks, la, lw;
@@ -299,27 +338,116 @@ void micro_imaging(Plan& plan, const litho_tile* tiles, int T, int height, int w
     alloc_tiles(lw, T, n);
     wafer = lw.as<float>();
   }
-  int max_core = 0;
-  for (int i = 0; i < T; ++i) max_core = std::max(max_core, area(tiles[i].core_local));
+  const int max_core = max_core_elems(tiles, T);
   // every Micro reads the global mask with periodic Halo indices (mirco_imaging.cpp:15-17)
   launch_extract(global_mask, height, width, table.as<int>(), T, n, masks.as<float>(), s);
-  for (int c = 0; c < T; c += plan.capacity) {
-    const int b = std::min(plan.capacity, T - c);
-    EvalRequest rq;
-    rq.mode = 0;
-    rq.base = masks.as<float>() + (size_t)c * plane;
-    rq.adjoint = false;
-    rq.im.thr = (float)threshold;
-    rq.im.alpha = (float)alpha;
-    rq.im.aerial = aerial + (size_t)c * plane;
-    rq.im.wafer = wafer + (size_t)c * plane;
-    run_eval(plan, b, rq);
-  }
+  image_tiles(plan, T, masks.as<float>(), threshold, alpha, aerial, wafer);
   // each Core writes the part of the chip it owns (mirco_imaging.cpp:23-25)
   if (global_aerial != nullptr) launch_assemble(aerial, table.as<int>(), T, n, width, global_aerial, max_core, s);
   if (global_wafer != nullptr) launch_assemble(wafer, table.as<int>(), T, n, width, global_wafer, max_core, s);
   LITHO_HIP(hipGetLastError());
   LITHO_HIP(hipStreamSynchronize(s));
+}
+
+MicroComparison compare_micro_imaging(Plan& plan, const litho_tile* tiles, int T, int height, int width,
+                                      const float* global_target, const float* optimized_local_masks,
+                                      double threshold, double alpha, const MicroComparisonOutputs& out) {
+  check_chip(plan, tiles, T, height, width, global_target, optimized_local_masks);
+  if (!std::isfinite(threshold) || !std::isfinite(alpha)) throw Error(1, "Imaging: threshold and alpha must be finite");
+  const int n = plan.n;
+  const size_t chip = (size_t)height * width;
+  hipStream_t s = plan.ctx->stream;
+  const std::vector<int> tab = tile_table(tiles, T);
+  DevMem table(tab.size() * sizeof(int)), ta, tw, oa, ow, la, lw;
+  LITHO_HIP(hipMemcpyAsync(table.ptr, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  // the global images every PE needs, in the caller's buffers or here; the
+  // Cores tile the chip, and pixels outside every Core count as 0
+  auto global_buf = [&](DevMem& m, float* given) {
+    if (given != nullptr) {
+      LITHO_HIP(hipMemsetAsync(given, 0, chip * sizeof(float), s));
+      return given;
+    }
+    m.alloc(chip * sizeof(float));
+    LITHO_HIP(hipMemsetAsync(m.ptr, 0, chip * sizeof(float), s));
+    return m.as<float>();
+  };
+  float* t_ai = global_buf(ta, out.target_aerial);
+  float* t_wi = global_buf(tw, out.target_wafer);
+  float* o_ai = global_buf(oa, out.optimized_aerial);
+  float* o_wi = global_buf(ow, out.optimized_wafer);
+  // the target: every Micro's periodic extract imaged (full_chip_imaging_compare.cpp:51-52)
+  micro_imaging(plan, tiles, T, height, width, global_target, threshold, alpha, t_ai, t_wi, nullptr, nullptr);
+  // the optimized Micros as given, Guard and Halo included (:53)
+  float* l_ai = out.optimized_local_aerial;
+  if (l_ai == nullptr) {
+    alloc_tiles(la, T, n);
+    l_ai = la.as<float>();
+  }
+  alloc_tiles(lw, T, n);
+  image_tiles(plan, T, optimized_local_masks, threshold, alpha, l_ai, lw.as<float>());
+  const int max_core = max_core_elems(tiles, T);
+  launch_assemble(l_ai, table.as<int>(), T, n, width, o_ai, max_core, s);
+  launch_assemble(lw.as<float>(), table.as<int>(), T, n, width, o_wi, max_core, s);
+  if (out.optimized_mask != nullptr) {
+    LITHO_HIP(hipMemsetAsync(out.optimized_mask, 0, chip * sizeof(float), s));
+    launch_assemble(optimized_local_masks, table.as<int>(), T, n, width, out.optimized_mask, max_core, s);
+  }
+  LITHO_HIP(hipGetLastError());
+  MicroComparison r;
+  // Loss::pe_loss of the Core-assembled wafer images against the global target (:62-63)
+  r.target_pe = pe_loss_device(t_wi, global_target, chip, s);
+  r.optimized_pe = pe_loss_device(o_wi, global_target, chip, s);
+  return r;
+}
+
+litho_mask_continuity mask_continuity(Context& ctx, const litho_tile* tiles, int T, int tile_size, int height,
+                                      int width, const float* local_masks, const float* target) {
+  if (height <= 0 || width <= 0) throw Error(1, "chip size must be positive");
+  if (local_masks == nullptr || target == nullptr) throw Error(1, "mask continuity: null masks or target");
+  validate_tiles(tiles, T, tile_size);
+  for (int i = 0; i < T; ++i) {
+    const litho_rect& c = tiles[i].core_global;
+    if (c.y0 < 0 || c.x0 < 0 || c.y1 > height || c.x1 > width) throw Error(1, "Micro Core outside the chip");
+  }
+  hipStream_t s = ctx.stream;
+  const std::vector<int> tab = tile_table(tiles, T);
+  DevMem table(tab.size() * sizeof(int)), global((size_t)height * width * sizeof(float));
+  const size_t np = (size_t)T * 4 * kExchangeBlocks * kContinuityValues;
+  DevMem part(np * sizeof(double));
+  LITHO_HIP(hipMemcpyAsync(table.ptr, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  LITHO_HIP(hipMemsetAsync(global.ptr, 0, global.bytes, s));
+  launch_assemble(local_masks, table.as<int>(), T, tile_size, width, global.as<float>(), max_core_elems(tiles, T), s);
+  launch_mask_continuity(table.as<int>(), T, tile_size, width, local_masks, global.as<float>(), target,
+                         part.as<double>(), s);
+  LITHO_HIP(hipGetLastError());
+  std::vector<double> h(np);
+  LITHO_HIP(hipMemcpyAsync(h.data(), part.ptr, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  LITHO_HIP(hipStreamSynchronize(s));
+  // fixed order: Micros, neighbour slots, blocks
+  double v[kContinuityValues] = {0};
+  for (size_t r = 0; r < (size_t)T * 4 * kExchangeBlocks; ++r)
+    for (int i = 0; i < kContinuityValues; ++i) {
+      const double x = h[r * kContinuityValues + i];
+      v[i] = i == 3 ? std::max(v[i], x) : v[i] + x;
+    }
+  litho_mask_continuity m{};
+  m.overlap_samples = (long long)v[0];
+  m.overlap_sse = v[1];
+  m.overlap_max_abs = v[3];
+  if (m.overlap_samples > 0) {
+    m.overlap_rmse = std::sqrt(v[1] / v[0]);
+    m.overlap_mae = v[2] / v[0];
+  }
+  m.seam_samples = (long long)v[4];
+  m.flat_seam_samples = (long long)v[7];
+  m.foreground_seam_samples = (long long)v[9];
+  if (m.seam_samples > 0) {
+    m.seam_jump_mae = v[5] / v[4];
+    m.seam_reference_error_mae = v[6] / v[4];
+  }
+  if (m.flat_seam_samples > 0) m.flat_seam_jump_mae = v[8] / v[7];
+  if (m.foreground_seam_samples > 0) m.foreground_seam_jump_mae = v[10] / v[9];
+  return m;
 }
 
 FullChipResult fullchip_ctm(Plan& plan, const litho_tile* tiles, int T, int height, int width,

@@ -50,6 +50,33 @@ void micro_imaging(Plan& plan, const litho_tile* tiles, int num_tiles, int heigh
                    double threshold, double alpha, float* global_aerial, float* global_wafer, float* local_aerial,
                    float* local_wafer);
 
+// compare_micro_imaging (full_chip_imaging_compare.cpp:19-71): the target's
+// Micros (periodic extracts of the global target) and the optimized local
+// Micros (as given, with their own Guard and Halo) imaged in batches, each set
+// of Cores assembled into global aerial / wafer images, and the PE of both
+// against the global target (FP64).  Outputs nullable: global images
+// height x width, optimized_local_aerial [T][n][n], optimized_mask the
+// Core-assembled optimized Micros.
+struct MicroComparisonOutputs {
+  float* optimized_mask = nullptr;
+  float* target_aerial = nullptr;
+  float* target_wafer = nullptr;
+  float* optimized_aerial = nullptr;
+  float* optimized_wafer = nullptr;
+  float* optimized_local_aerial = nullptr;
+};
+struct MicroComparison {
+  double target_pe = 0.0, optimized_pe = 0.0;
+};
+MicroComparison compare_micro_imaging(Plan& plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                                      const float* global_target, const float* optimized_local_masks,
+                                      double threshold, double alpha, const MicroComparisonOutputs& out);
+
+// evaluate_mask_continuity (mask_continuity.cpp:8-61) of device Micros
+// [num_tiles][tile_size^2] against the device height x width target.
+litho_mask_continuity mask_continuity(Context& ctx, const litho_tile* tiles, int num_tiles, int tile_size, int height,
+                                      int width, const float* local_masks, const float* target);
+
 struct FullChipResult {
   int best_iteration = 0;
   double best_loss = 0.0;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "binarize.hpp"
 #include "ctm_driver.hpp"
 #include "fullchip.hpp"
 #include "lsm.hpp"
@@ -796,3 +797,62 @@ int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
 }
 
 }  // extern "C"
+
+int litho_binarize_advanced(litho_ctx* ctx, int height, int width, const float* gray_mask, const float* target_pattern,
+                            double peak_min_intensity, double quantile_val, int area_threshold,
+                            int min_spacing_pixels, float* final_mask, float* initial_binary_mask,
+                            float* filtered_binary_mask, float* sraf_mask_initial, float* sraf_mask_final,
+                            float* keep_out_zone, litho_binarize_info* info) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    LITHO_HIP(hipSetDevice(c.device));
+    litho::BinarizeOutputs o;
+    o.final_mask = final_mask;
+    o.initial_mask = initial_binary_mask;
+    o.filtered_mask = filtered_binary_mask;
+    o.sraf_initial = sraf_mask_initial;
+    o.sraf_final = sraf_mask_final;
+    o.keep_out = keep_out_zone;
+    const litho::BinarizeInfo r = litho::binarize_advanced(c, height, width, gray_mask, target_pattern,
+                                                           peak_min_intensity, quantile_val, area_threshold,
+                                                           min_spacing_pixels, o);
+    if (info != nullptr) {
+      info->threshold = r.threshold;
+      info->num_peaks = r.num_peaks;
+    }
+  });
+}
+
+int litho_compare_micro_imaging(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                                const float* global_target, const float* optimized_local_masks, double threshold,
+                                double alpha, float* optimized_mask, float* target_aerial, float* target_wafer,
+                                float* optimized_aerial, float* optimized_wafer, float* optimized_local_aerial,
+                                double* target_pe, double* optimized_pe) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    set_device(p);
+    require(tiles != nullptr && num_tiles > 0, "no Micro tiles");
+    litho::MicroComparisonOutputs o;
+    o.optimized_mask = optimized_mask;
+    o.target_aerial = target_aerial;
+    o.target_wafer = target_wafer;
+    o.optimized_aerial = optimized_aerial;
+    o.optimized_wafer = optimized_wafer;
+    o.optimized_local_aerial = optimized_local_aerial;
+    const litho::MicroComparison r = litho::compare_micro_imaging(p, tiles, num_tiles, height, width, global_target,
+                                                                  optimized_local_masks, threshold, alpha, o);
+    if (target_pe != nullptr) *target_pe = r.target_pe;
+    if (optimized_pe != nullptr) *optimized_pe = r.optimized_pe;
+  });
+}
+
+int litho_evaluate_mask_continuity(litho_ctx* ctx, const litho_tile* tiles, int num_tiles, int tile_size, int height,
+                                   int width, const float* local_masks, const float* target,
+                                   litho_mask_continuity* out) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    require(tiles != nullptr && num_tiles > 0 && out != nullptr, "null tiles or output");
+    LITHO_HIP(hipSetDevice(c.device));
+    *out = litho::mask_continuity(c, tiles, num_tiles, tile_size, height, width, local_masks, target);
+  });
+}

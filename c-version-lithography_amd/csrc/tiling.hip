@@ -276,6 +276,103 @@ __global__ void k_mask_from_theta_tiles(const float* __restrict__ theta, size_t 
     mask[i] = 0.5f * (1.0f + cosf(theta[i]));
 }
 
+// evaluate_mask_continuity (mask_continuity.cpp:8-61): block (blk, slot, t)
+// takes a grid-strided share of the Guard strip of neighbour slot `slot` of
+// Micro t (my_guard_local against the neighbour's neighbor_overlap_local) and,
+// for a right / bottom neighbour, of the Core seam pixel pairs of the
+// assembled mask.  partial[((t * 4 + slot) * nblk + blk) * kContinuityValues + v].
+__global__ void __launch_bounds__(256)
+k_mask_continuity(const int* __restrict__ table, int n, int width, const float* __restrict__ masks,
+                  const float* __restrict__ global, const float* __restrict__ target, double* __restrict__ partial) {
+  const int t = blockIdx.z, slot = blockIdx.y;
+  const int* e = table + (size_t)t * kTileStride + 20 + 20 * slot;
+  double v[kContinuityValues];
+#pragma unroll
+  for (int i = 0; i < kContinuityValues; ++i) v[i] = 0.0;
+  if (e[0] != 0) {
+    const int nb = e[1], dir = e[2];
+    const int* a = e + 4;
+    const int* b = e + 8;
+    const int rows = rows_of(a), cols = cols_of(a);
+    const size_t cnt = (rows > 0 && cols > 0) ? (size_t)rows * cols : 0;
+    const size_t mine = (size_t)t * n * n, theirs = (size_t)nb * n * n;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < cnt; q += (size_t)gridDim.x * blockDim.x) {
+      const int y = (int)(q / cols), x = (int)(q - (size_t)y * cols);
+      const double d = (double)masks[mine + (size_t)(a[0] + y) * n + a[2] + x] -
+                       (double)masks[theirs + (size_t)(b[0] + y) * n + b[2] + x];
+      v[0] += 1.0;
+      v[1] += d * d;
+      v[2] += fabs(d);
+      v[3] = fmax(v[3], fabs(d));
+    }
+    if (dir == 1 || dir == 3) {
+      const int* core = table + (size_t)t * kTileStride + 4;        // my core_global y0, y1, x0, x1
+      const int* ncore = table + (size_t)nb * kTileStride + 4;
+      const int len = dir == 1 ? core[1] - core[0] : core[3] - core[2];
+      for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < len; q += gridDim.x * blockDim.x) {
+        const int by = dir == 1 ? core[0] + q : ncore[0], bx = dir == 1 ? ncore[2] : core[2] + q;
+        const int ay = dir == 1 ? by : by - 1, ax = dir == 1 ? bx - 1 : bx;
+        const size_t ia = (size_t)ay * width + ax, ib = (size_t)by * width + bx;
+        const double jump = (double)global[ib] - (double)global[ia];
+        const double ref = (double)target[ib] - (double)target[ia];
+        v[4] += 1.0;
+        v[5] += fabs(jump);
+        v[6] += fabs(jump - ref);
+        if (ref == 0.0) {
+          v[7] += 1.0;
+          v[8] += fabs(jump);
+        }
+        if ((double)target[ia] > 0.5 && (double)target[ib] > 0.5) {
+          v[9] += 1.0;
+          v[10] += fabs(jump);
+        }
+      }
+    }
+  }
+  __shared__ double w[4][kContinuityValues];
+#pragma unroll
+  for (int i = 0; i < kContinuityValues; ++i) {
+    double x = v[i];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(x, off, 64);
+      x = i == 3 ? fmax(x, o) : x + o;
+    }
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6][i] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < kContinuityValues) {
+    const int i = threadIdx.x;
+    const double r = i == 3 ? fmax(fmax(w[0][i], w[1][i]), fmax(w[2][i], w[3][i])) : ((w[0][i] + w[1][i]) + w[2][i]) + w[3][i];
+    partial[(((size_t)t * 4 + slot) * gridDim.x + blockIdx.x) * kContinuityValues + i] = r;
+  }
+}
+
+void launch_mask_continuity(const int* table, int num_tiles, int n, int width, const float* masks,
+                            const float* global, const float* target, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_mask_continuity, dim3(kExchangeBlocks, 4, num_tiles), dim3(256), 0, s, table, n, width, masks,
+                     global, target, partial);
+}
+
+// FP64 partial sums of (a - b)^2: block `blk` sums elements blk*256 + t,
+// strided by the grid, then a fixed shuffle tree and a fixed sum over waves.
+__global__ void __launch_bounds__(256) k_sq_diff_partials(const float* __restrict__ a, const float* __restrict__ b,
+                                                          size_t n, double* __restrict__ partial) {
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double d = (double)a[i] - (double)b[i];
+    acc = fma(d, d, acc);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = ((w[0] + w[1]) + w[2]) + w[3];
+}
+
+void launch_sq_diff_partials(const float* a, const float* b, size_t n, double* partial, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_sq_diff_partials, dim3(blocks), dim3(256), 0, s, a, b, n, partial);
+}
+
 void launch_mask_from_theta_tiles(const float* theta, size_t count, float* mask, hipStream_t s) {
   const size_t blocks = (count + 255) / 256;
   hipLaunchKernelGGL(k_mask_from_theta_tiles, dim3(blocks < 4096 ? (blocks > 0 ? blocks : 1) : 4096), dim3(256), 0, s,

@@ -53,5 +53,16 @@ void launch_core_metrics(const int* table, int num_tiles, int n, const float* wa
 void launch_theta_step(const int* table, int num_tiles, int n, const float* grad, double lr, float* theta,
                        float* mask, int max_update_elems, hipStream_t s);
 void launch_mask_from_theta_tiles(const float* theta, size_t count, float* mask, hipStream_t s);
+// Mask continuity partials, kContinuityValues per (Micro, neighbour slot, block
+// of kExchangeBlocks): Guard-vs-neighbour-Core count, sum d^2, sum |d|, max |d|;
+// Core seam pairs (right / bottom neighbours, on the assembled global mask and
+// target): count, sum |jump|, sum |jump - target jump|, flat count, flat sum
+// |jump|, foreground count, foreground sum |jump|.
+constexpr int kContinuityValues = 11;
+void launch_mask_continuity(const int* table, int num_tiles, int n, int width, const float* masks,
+                            const float* global, const float* target, double* partial, hipStream_t s);
+// partial[blk] = FP64 sum of (a - b)^2 over the grid-strided elements of block
+// blk (blocks of 256 threads), summed in a fixed order.
+void launch_sq_diff_partials(const float* a, const float* b, size_t n, double* partial, int blocks, hipStream_t s);
 
 }  // namespace litho

@@ -102,6 +102,21 @@ class TileLoss(C.Structure):
                 ("core_pe_loss", C.c_double), ("core_bg_penalty_raw", C.c_double)]
 
 
+class BinarizeInfo(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("num_peaks", C.c_longlong)]
+
+
+class MaskContinuity(C.Structure):
+    _fields_ = [("overlap_samples", C.c_longlong), ("overlap_sse", C.c_double), ("overlap_rmse", C.c_double),
+                ("overlap_mae", C.c_double), ("overlap_max_abs", C.c_double), ("seam_samples", C.c_longlong),
+                ("flat_seam_samples", C.c_longlong), ("foreground_seam_samples", C.c_longlong),
+                ("seam_jump_mae", C.c_double), ("seam_reference_error_mae", C.c_double),
+                ("flat_seam_jump_mae", C.c_double), ("foreground_seam_jump_mae", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BatchTile(C.Structure):
     _fields_ = [("plan_index", C.c_int), ("target", _P_), ("theta0", _P_), ("out_mask", _P_), ("out_theta", _P_)]
 
@@ -150,6 +165,11 @@ SIGNATURES = {
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
     "litho_ls_reinit_sdf": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
     "litho_ls_reinit_sdf_noalloc": (_I, [_P, _I, _I, _I, _P, _D, _D, _P]),
+    "litho_compare_micro_imaging": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P,
+                                         _DP, _DP]),
+    "litho_evaluate_mask_continuity": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _I, _P, _P, C.POINTER(MaskContinuity)]),
+    "litho_binarize_advanced": (_I, [_P, _I, _I, _P, _P, _D, _D, _I, _I, _P, _P, _P, _P, _P, _P,
+                                     C.POINTER(BinarizeInfo)]),
     "litho_render_levelset_mask": (_I, [_P, _I, _I, _I, _P, _I, _I, _P]),
     "litho_micro_imaging": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _D, _D, _P, _P, _P, _P]),
     "litho_lsm_step": (_I, [_P, _I, _I, _P, _P, _IP, C.POINTER(LsmOptions), _I, _P, _P, _DP]),
@@ -304,6 +324,18 @@ class Context:
         b, r, c = _shape3(phi)
         check(self.lib.litho_ls_reinit_sdf_noalloc(self.h, b, r, c, ptr(phi), dx, dy, ptr(sdf)))
 
+    def binarize_advanced(self, gray, target, final_mask, peak_min_intensity=0.1, quantile_val=0.2,
+                          area_threshold=50, min_spacing_pixels=15, initial=None, filtered=None, sraf_initial=None,
+                          sraf_final=None, keep_out=None):
+        """Imbinarize::binarize_advanced on device (H, W) float32 tensors; returns (threshold, num_peaks)."""
+        h, w = gray.shape
+        info = BinarizeInfo()
+        check(self.lib.litho_binarize_advanced(self.h, h, w, ptr(gray), ptr(target), peak_min_intensity,
+                                               quantile_val, area_threshold, min_spacing_pixels, ptr(final_mask),
+                                               ptr(initial), ptr(filtered), ptr(sraf_initial), ptr(sraf_final),
+                                               ptr(keep_out), C.byref(info)))
+        return info.threshold, info.num_peaks
+
     def render_levelset_mask(self, phi, mask_out, mask_type="Binary", scale=4):
         """render_levelset_mask of (batch, rows, cols) tiles; mask_type "Binary" or "Gray"."""
         b, r, c = _shape3(phi)
@@ -328,6 +360,14 @@ class Context:
     def partition_extract(self, global_, tiles, tile_size, out):
         h, w = global_.shape
         check(self.lib.litho_partition_extract(self.h, ptr(global_), h, w, tiles, len(tiles), tile_size, ptr(out)))
+
+    def mask_continuity(self, tiles, tile_size, local_masks, target):
+        """evaluate_mask_continuity of device Micros against the device global target; returns a dict."""
+        h, w = target.shape
+        m = MaskContinuity()
+        check(self.lib.litho_evaluate_mask_continuity(self.h, tiles, len(tiles), tile_size, h, w, ptr(local_masks),
+                                                      ptr(target), C.byref(m)))
+        return m.as_dict()
 
     def partition_assemble(self, local, tiles, tile_size, out):
         h, w = out.shape
@@ -484,6 +524,19 @@ class Plan:
         check(self.lib.litho_tile_exchange(self.h, tiles, len(tiles), ptr(masks), ptr(targets), C.byref(options),
                                            ptr(mask_gradient), ptr(raw_gradient), ptr(wafer), per, _dp(tot)))
         return list(per), tot
+
+    def compare_micro_imaging(self, tiles, global_target, optimized_local_masks, threshold=0.25, alpha=50.0,
+                              optimized_mask=None, target_aerial=None, target_wafer=None, optimized_aerial=None,
+                              optimized_wafer=None, optimized_local_aerial=None):
+        """compare_micro_imaging; returns (target_pe, optimized_pe)."""
+        h, w = global_target.shape
+        tpe, ope = C.c_double(0.0), C.c_double(0.0)
+        check(self.lib.litho_compare_micro_imaging(self.h, tiles, len(tiles), h, w, ptr(global_target),
+                                                   ptr(optimized_local_masks), threshold, alpha, ptr(optimized_mask),
+                                                   ptr(target_aerial), ptr(target_wafer), ptr(optimized_aerial),
+                                                   ptr(optimized_wafer), ptr(optimized_local_aerial), C.byref(tpe),
+                                                   C.byref(ope)))
+        return tpe.value, ope.value
 
     def fullchip_ctm(self, tiles, initial_pattern, target, options, iterations, learning_rate, global_mask_out,
                      local_masks_out=None):

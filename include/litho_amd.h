@@ -425,6 +425,40 @@ int litho_micro_imaging(litho_plan* plan, const litho_tile* tiles, int num_tiles
                         const float* global_mask, double threshold, double alpha, float* global_aerial,
                         float* global_wafer, float* local_aerial, float* local_wafer);
 
+/* compare_micro_imaging(params, meta, cache, target, optimized_local_masks)
+ * (full_chip_imaging_compare.cpp:19-71), the post-run evaluation: the target's
+ * Micros (periodic extracts of global_target) and the optimized Micros as
+ * given (device [num_tiles][n][n], their own Guard and Halo kept) are imaged
+ * through the plan, each set's Cores assembled into global height x width
+ * aerial / wafer images (zero outside every Core, as assemble_global), and
+ * target_pe / optimized_pe = Loss::pe_loss(wafer, global_target) in FP64
+ * (host, nullable).  optimized_mask: the Core-assembled optimized Micros.
+ * Every output is nullable; optimized_local_aerial is [num_tiles][n][n]. */
+int litho_compare_micro_imaging(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
+                                const float* global_target, const float* optimized_local_masks, double threshold,
+                                double alpha, float* optimized_mask, float* target_aerial, float* target_wafer,
+                                float* optimized_aerial, float* optimized_wafer, float* optimized_local_aerial,
+                                double* target_pe, double* optimized_pe);
+
+/* MaskContinuityMetrics (mask_continuity.h:9-19). */
+typedef struct {
+  long long overlap_samples;       /* directed Guard -> neighbour Core pixel pairs */
+  double overlap_sse, overlap_rmse, overlap_mae, overlap_max_abs;
+  long long seam_samples, flat_seam_samples, foreground_seam_samples;
+  double seam_jump_mae, seam_reference_error_mae, flat_seam_jump_mae, foreground_seam_jump_mae;
+} litho_mask_continuity;
+
+/* evaluate_mask_continuity(meta, local_masks, target) (mask_continuity.cpp:8-61):
+ * for every Micro and neighbour, its Guard strip of the local masks against the
+ * same pixels in the neighbour's local mask; for every right / bottom
+ * neighbour, the pixel pairs across the Core seam of the assembled mask, with
+ * the target's own change across the seam separated out.  local_masks: device
+ * [num_tiles][tile_size^2]; target: device height x width; out: host.  FP64
+ * sums in a fixed order. */
+int litho_evaluate_mask_continuity(litho_ctx* ctx, const litho_tile* tiles, int num_tiles, int tile_size, int height,
+                                   int width, const float* local_masks, const float* target,
+                                   litho_mask_continuity* out);
+
 /* FullChipCTMOptimizer::optimize(initial_pattern, global_target, options)
  * (full_chip_ctm.cpp:25-152): theta per Micro from initialize_theta of the
  * pattern, fixed-step gradient descent on theta inside Core + Guard (Halo frozen),
@@ -449,6 +483,35 @@ int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
                        const float* initial_binary, const float* global_target, const litho_exchange_options* options,
                        const litho_lsm_options* lsm, int iterations, int reinit_interval, float* global_mask_out,
                        float* global_phi_out, double* history, int* best_iteration, double* best_loss);
+
+/* ---------------------------------------------------------------- CTM -> binary
+ * Imbinarize::binarize_advanced(gray_mask, target_pattern, peak_min_intensity,
+ * quantile_val, area_threshold, min_spacing_pixels) (imbinarize.cpp:84-190),
+ * the hand-off from the CTM gray mask to the level-set stage
+ * (demo_full_chip_ctm.cpp:109 calls it with 0.125, 0.1, 5, 5):
+ *   threshold = quantile_val-quantile (linear) of the gray values at local
+ *               maxima (>= their in-image 3 x 3 neighbourhood) above
+ *               peak_min_intensity;
+ *   initial   = gray >= threshold;  filtered = initial without 8-connected
+ *               components of area < area_threshold;
+ *   sraf_initial = filtered and not (target > 0.5);  keep_out = (target > 0.5)
+ *               dilated by the elliptical element of radius min_spacing_pixels
+ *               (0: none); sraf_final = sraf_initial and not keep_out;
+ *   final     = (target > 0.5) or sraf_final, without components of area
+ *               < area_threshold.
+ * With no peak every mask is empty (final included), as the reference.
+ * height x width device float32 arrays; outputs 0/1; all outputs except
+ * final_mask nullable.  info (host, nullable).  Synchronises the context. */
+typedef struct {
+  double threshold;        /* the adaptive threshold (0 when no peak) */
+  long long num_peaks;
+} litho_binarize_info;
+
+int litho_binarize_advanced(litho_ctx* ctx, int height, int width, const float* gray_mask, const float* target_pattern,
+                            double peak_min_intensity, double quantile_val, int area_threshold,
+                            int min_spacing_pixels, float* final_mask, float* initial_binary_mask,
+                            float* filtered_binary_mask, float* sraf_mask_initial, float* sraf_mask_final,
+                            float* keep_out_zone, litho_binarize_info* info);
 
 #ifdef __cplusplus
 }

@@ -139,6 +139,62 @@ def assemble_global(locals_, meta: Meta) -> np.ndarray:
     return out
 
 
+def evaluate_mask_continuity(meta: Meta, local_masks, target) -> dict:
+    """evaluate_mask_continuity (mask_continuity.cpp:8-61): Guard-vs-neighbour
+    Core differences of the local masks (every directed neighbour pair) and
+    the jumps across the assembled Cores' right / bottom seams, split by the
+    target's own change across the seam."""
+    r = dict(overlap_samples=0, overlap_sse=0.0, overlap_rmse=0.0, overlap_mae=0.0, overlap_max_abs=0.0,
+             seam_samples=0, flat_seam_samples=0, foreground_seam_samples=0, seam_jump_mae=0.0,
+             seam_reference_error_mae=0.0, flat_seam_jump_mae=0.0, foreground_seam_jump_mae=0.0)
+    g = assemble_global(local_masks, meta)
+    t = np.asarray(target, dtype=np.float64)
+    overlap_abs = 0.0
+
+    def seam(ay, ax, by, bx):
+        jump = g[by, bx] - g[ay, ax]
+        ref = t[by, bx] - t[ay, ax]
+        r["seam_samples"] += 1
+        r["seam_jump_mae"] += abs(jump)
+        r["seam_reference_error_mae"] += abs(jump - ref)
+        if ref == 0.0:
+            r["flat_seam_samples"] += 1
+            r["flat_seam_jump_mae"] += abs(jump)
+        if t[ay, ax] > 0.5 and t[by, bx] > 0.5:
+            r["foreground_seam_samples"] += 1
+            r["foreground_seam_jump_mae"] += abs(jump)
+
+    for tile in meta.tiles:
+        for e in tile.neighbors:
+            a, b = e.my_guard_local, e.neighbor_overlap_local
+            d = np.asarray(local_masks[tile.id], np.float64)[a.sl()] - np.asarray(local_masks[e.id], np.float64)[b.sl()]
+            r["overlap_samples"] += d.size
+            r["overlap_sse"] += float(np.sum(d * d))
+            overlap_abs += float(np.sum(np.abs(d)))
+            if d.size:
+                r["overlap_max_abs"] = max(r["overlap_max_abs"], float(np.max(np.abs(d))))
+            nb = meta.tiles[e.id]
+            if e.direction == RIGHT:
+                x = nb.core_global.x0
+                for y in range(tile.core_global.y0, tile.core_global.y1):
+                    seam(y, x - 1, y, x)
+            elif e.direction == BOTTOM:
+                y = nb.core_global.y0
+                for x in range(tile.core_global.x0, tile.core_global.x1):
+                    seam(y - 1, x, y, x)
+    if r["overlap_samples"]:
+        r["overlap_rmse"] = float(np.sqrt(r["overlap_sse"] / r["overlap_samples"]))
+        r["overlap_mae"] = overlap_abs / r["overlap_samples"]
+    if r["seam_samples"]:
+        r["seam_jump_mae"] /= r["seam_samples"]
+        r["seam_reference_error_mae"] /= r["seam_samples"]
+    if r["flat_seam_samples"]:
+        r["flat_seam_jump_mae"] /= r["flat_seam_samples"]
+    if r["foreground_seam_samples"]:
+        r["foreground_seam_jump_mae"] /= r["foreground_seam_samples"]
+    return r
+
+
 def update_window(tile: Tile, size: int) -> np.ndarray:
     """1 inside Core + Guard (partition.cpp:115-120)."""
     w = np.zeros((size, size))
@@ -356,6 +412,24 @@ def micro_imaging(meta: Meta, cache: Cache, global_mask, threshold: float, alpha
         la.append(a)
         lw.append(w)
     return assemble_global(la, meta), assemble_global(lw, meta), la, lw
+
+
+def compare_micro_imaging(meta: Meta, cache: Cache, global_target, optimized_local_masks, threshold: float,
+                          alpha: float) -> dict:
+    """compare_micro_imaging (full_chip_imaging_compare.cpp:19-71): the
+    target's Micros from periodic extracts, the optimized Micros as given,
+    both Core-assembled; PE of each against the global target."""
+    ta, tw, _, _ = micro_imaging(meta, cache, global_target, threshold, alpha)
+    oa_l, ow_l = [], []
+    for m in optimized_local_masks:
+        a, w, _ = forward(cache, m, threshold, alpha)
+        oa_l.append(a)
+        ow_l.append(w)
+    oa, ow = assemble_global(oa_l, meta), assemble_global(ow_l, meta)
+    t = np.asarray(global_target, dtype=np.float64)
+    return dict(optimized_mask=assemble_global(optimized_local_masks, meta), target_aerial=ta, target_wafer=tw,
+                optimized_aerial=oa, optimized_wafer=ow, optimized_local_aerial=oa_l,
+                target_pe=float(np.sum((tw - t) ** 2)), optimized_pe=float(np.sum((ow - t) ** 2)))
 
 
 def lsm_update_tile(phi, mask_gradient, tile: Tile, size: int, lo: LSMOptions, iteration: int):

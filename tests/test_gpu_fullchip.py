@@ -113,6 +113,81 @@ def test_micro_imaging_matches_oracle(capacity):
     ctx.close()
 
 
+def test_compare_micro_imaging_matches_oracle():
+    """compare_micro_imaging: target Micros from periodic extracts, optimized
+    Micros as given (their Guards differ from the neighbours' Cores), both
+    Core-assembled; images at the forward tolerances, PE within 1e-5."""
+    import torch
+    la, ctx, plan, ts, tiles, meta, cache = setup(3)
+    target = chip_target(96, 96, 7)
+    rng = np.random.default_rng(5)
+    local = [f32(np.clip(m + 0.1 * rng.standard_normal(m.shape), 0, 1)) for m in fc.extract_all(target, meta)]
+    print("case compare_micro_imaging", flush=True)
+    z = lambda: torch.zeros((96, 96), dtype=torch.float32, device="cuda")
+    om, ta, tw, oa, ow = z(), z(), z(), z(), z()
+    loc = torch.zeros((len(tiles), ts, ts), dtype=torch.float32, device="cuda")
+    tpe, ope = plan.compare_micro_imaging(tiles, dev(target), dev(np.stack(local)), threshold=THR, alpha=ALPHA,
+                                          optimized_mask=om, target_aerial=ta, target_wafer=tw, optimized_aerial=oa,
+                                          optimized_wafer=ow, optimized_local_aerial=loc)
+    ref = fc.compare_micro_imaging(meta, cache, target, local, THR, ALPHA)
+    print(f"compare_micro_imaging: PE target {tpe:.6f} vs {ref['target_pe']:.6f}, optimized {ope:.6f} vs "
+          f"{ref['optimized_pe']:.6f}")
+    assert abs(tpe - ref["target_pe"]) <= 1e-5 * ref["target_pe"]
+    assert abs(ope - ref["optimized_pe"]) <= 1e-5 * ref["optimized_pe"]
+    assert np.array_equal(host(om), ref["optimized_mask"].astype(np.float32).astype(np.float64))
+    for got, want in [(ta, ref["target_aerial"]), (oa, ref["optimized_aerial"])]:
+        assert np.max(np.abs(host(got) - want)) <= 1e-5 * np.max(np.abs(want))
+    for got, want in [(tw, ref["target_wafer"]), (ow, ref["optimized_wafer"])]:
+        assert np.max(np.abs(host(got) - want)) <= 1e-4
+    assert np.max(np.abs(host(loc) - np.stack(ref["optimized_local_aerial"]))) <= 1e-5 * np.max(
+        np.abs(ref["optimized_aerial"]))
+    # without optional outputs the PE is the same
+    tpe2, ope2 = plan.compare_micro_imaging(tiles, dev(target), dev(np.stack(local)), threshold=THR, alpha=ALPHA)
+    assert (tpe2, ope2) == (tpe, ope)
+    plan.close()
+    ctx.close()
+
+
+def test_mask_continuity_reference_numbers(gpu_ctx):
+    """evaluate_mask_continuity on the device: the reference's known answers
+    (tests/test_mask_continuity.cpp:13-39) and a random case against the oracle."""
+    import torch
+    la = litho_amd()
+    ts, tiles = la.partition_build(12, 16, 2, 2, 2, 1)
+    meta = fc.build(12, 16, 2, 2, 2, 1)
+    print("case mask continuity", flush=True)
+    target = np.zeros((12, 16))
+    target[:, 8:] = 1.0
+    locals_ = fc.extract_all(target, meta)
+    r = gpu_ctx.mask_continuity(tiles, ts, dev(np.stack(locals_)), dev(target))
+    assert r["overlap_samples"] == 56 and r["seam_samples"] == 28
+    assert r["overlap_sse"] == 0.0 and r["seam_reference_error_mae"] == 0.0
+    assert abs(r["seam_jump_mae"] - 12.0 / 28.0) < 1e-14
+    assert r["flat_seam_samples"] == 16 and r["foreground_seam_samples"] == 8
+    target[:] = 0.0
+    for t in meta.tiles:
+        locals_[t.id][:] = t.col
+    b = gpu_ctx.mask_continuity(tiles, ts, dev(np.stack(locals_)), dev(target))
+    assert b["overlap_sse"] == 24.0 and b["overlap_max_abs"] == 1.0
+    assert abs(b["overlap_rmse"] - np.sqrt(24.0 / 56.0)) < 1e-14
+    assert abs(b["seam_reference_error_mae"] - 12.0 / 28.0) < 1e-14
+    locals_ = fc.extract_all(target, meta)
+    g = meta.tiles[0].neighbors[0].my_guard_local
+    locals_[0][g.sl()] = 0.8
+    c = gpu_ctx.mask_continuity(tiles, ts, dev(np.stack(locals_)), dev(target))
+    assert abs(c["overlap_sse"] - 6 * float(np.float32(0.8)) ** 2) < 1e-12 and c["seam_jump_mae"] == 0.0
+    # random masks on a 3 x 2 partition with guard 6
+    ts2, tiles2 = la.partition_build(100, 90, 3, 2, 12, 6)
+    meta2 = fc.build(100, 90, 3, 2, 12, 6)
+    rng = np.random.default_rng(4)
+    tgt2 = (rng.random((100, 90)) < 0.4).astype(np.float64)
+    loc2 = [f32(rng.random((ts2, ts2))) for _ in range(len(tiles2))]
+    got = gpu_ctx.mask_continuity(tiles2, ts2, dev(np.stack(loc2)), dev(tgt2))
+    ref = fc.evaluate_mask_continuity(meta2, loc2, tgt2)
+    for k, v in ref.items():
+        assert abs(got[k] - v) <= 1e-12 * max(1.0, abs(v)), (k, got[k], v)
+
+
 CASES = {
     "pe_core": dict(),
     "penalty_coreguard_cept2_avg": dict(loss_type="PEPenalty", loss_region="CoreGuard", enable_cept=True,

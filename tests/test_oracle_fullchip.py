@@ -172,3 +172,29 @@ def test_full_chip_lsm_single_tile_matches_manual_rk2():
         phi, _, _ = fc.lsm_update_tile(phi, ev.locals[0].mask_gradient, t, n, lo, it + 1)
     assert np.allclose([x[1] for x in res["history"]], losses, rtol=1e-10, atol=0)
     assert np.allclose(res["final_phi"][0], phi, rtol=0, atol=1e-11)
+
+
+def test_mask_continuity_reference_numbers():
+    """The reference's own known answers (tests/test_mask_continuity.cpp:13-39)."""
+    meta = fc.build(12, 16, 2, 2, 2, 1)
+    target = np.zeros((12, 16))
+    target[:, 8:] = 1.0
+    locals_ = fc.extract_all(target, meta)
+    r = fc.evaluate_mask_continuity(meta, locals_, target)
+    assert r["overlap_samples"] == 56 and r["seam_samples"] == 28
+    assert r["overlap_sse"] == 0.0 and r["seam_reference_error_mae"] == 0.0
+    assert abs(r["seam_jump_mae"] - 12.0 / 28.0) < 1e-14
+    assert r["flat_seam_samples"] == 16 and r["foreground_seam_samples"] == 8
+    target[:] = 0.0
+    for t in meta.tiles:
+        locals_[t.id][:] = t.col
+    b = fc.evaluate_mask_continuity(meta, locals_, target)
+    assert b["overlap_sse"] == 24.0 and b["overlap_max_abs"] == 1.0
+    assert abs(b["overlap_rmse"] - np.sqrt(24.0 / 56.0)) < 1e-14
+    assert abs(b["seam_reference_error_mae"] - 12.0 / 28.0) < 1e-14
+    locals_ = fc.extract_all(target, meta)
+    g = meta.tiles[0].neighbors[0].my_guard_local
+    locals_[0][g.sl()] = 0.8
+    c = fc.evaluate_mask_continuity(meta, locals_, target)
+    assert abs(c["overlap_sse"] - 3.84) < 1e-14
+    assert c["seam_jump_mae"] == 0.0
