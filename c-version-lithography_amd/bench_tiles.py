@@ -231,9 +231,15 @@ def fullchip(args):
     r_ctm = plan.fullchip_ctm(tiles, target, target, opts, args.ctm_iterations, 0.9, gmask)
     torch.cuda.synchronize()
     t_ctm = time.perf_counter() - t0
-    binary = (gmask >= 0.5).float()
+    # the reference's hand-off: Imbinarize::binarize_advanced(mask, target, 0.125, 0.1, 5, 5)
+    # (demo_full_chip_ctm.cpp:109); its final mask seeds the level set
+    binary = torch.zeros_like(target)
+    t0 = time.perf_counter()
+    thr, npk = ctx.binarize_advanced(gmask, target, binary, 0.125, 0.1, 5, 5)
+    t_bin = time.perf_counter() - t0
     print(f"CTM: {args.ctm_iterations} iterations in {t_ctm:.3f} s, best loss {r_ctm['best_loss']:.1f} "
-          f"(iteration {r_ctm['best_iteration']})", file=sys.stderr, flush=True)
+          f"(iteration {r_ctm['best_iteration']}); binarize_advanced {1e3 * t_bin:.1f} ms: threshold {thr:.4f} "
+          f"from {npk} peaks, {int(binary.sum().item())} mask pixels", file=sys.stderr, flush=True)
     lmask = torch.zeros_like(target)
     lphi = torch.zeros_like(target)
     plan.fullchip_lsm(tiles, binary, target, opts, 1, lmask, lphi, cfl=0.5, curvature_weight=0.01,
@@ -256,7 +262,8 @@ def fullchip(args):
             "k": K, "window": d["rows"], "micro_pixels": px,
             "stencil_GBps_at_40B_per_px": round(40 * px / (t_lsm / args.updates) / 1e9, 1),
             "lsm_loss_first_last": [round(h[0][0], 3), round(h[-1][0], 3)],
-            "ctm_best_loss": round(r_ctm["best_loss"], 3)}
+            "ctm_best_loss": round(r_ctm["best_loss"], 3), "binarize_ms": round(1e3 * t_bin, 2),
+            "binarize_threshold": round(thr, 6)}
     plan.close()
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -134,11 +134,11 @@ Exchanger::Exchanger(Plan& plan, const litho_tile* tiles, int num_tiles)
   pen_win_.alloc((size_t)T_ * plane * sizeof(float));
   raw_.alloc((size_t)T_ * plane * sizeof(float));
   guard_part_.alloc((size_t)T_ * 4 * kExchangeBlocks * sizeof(double));
-  core_part_.alloc((size_t)T_ * kMetricBlocks * 2 * sizeof(double));
+  core_part_.alloc((size_t)T_ * kMetricBlocks * kCoreMetrics * sizeof(double));
   red_all_.alloc((size_t)T_ * 4 * sizeof(double));
   cept_red_.alloc((size_t)T_ * sizeof(double));
-  core_red_.alloc((size_t)T_ * 2 * sizeof(double));
-  host_ = new HostPinned((size_t)T_ * 7 * sizeof(double));
+  core_red_.alloc((size_t)T_ * kCoreMetrics * sizeof(double));
+  host_ = new HostPinned((size_t)T_ * (5 + kCoreMetrics) * sizeof(double));
   LITHO_HIP(hipMemcpyAsync(table_.ptr, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, plan_.ctx->stream));
   LITHO_HIP(hipStreamSynchronize(plan_.ctx->stream));
 }
@@ -209,7 +209,8 @@ void Exchanger::run(const float* masks, const litho_exchange_options& o, float* 
   if (o.collect_core_metrics) {
     launch_core_metrics(table(), T_, n_, wafer, targets_, pe_win_.as<float>(), pen_win_.as<float>(),
                         o.penalty_threshold, max_core_, core_part_.as<double>(), s);
-    launch_reduce_fixed(core_part_.as<double>(), kMetricBlocks, 2, core_red_.as<double>(), 2, T_, s);
+    launch_reduce_fixed(core_part_.as<double>(), kMetricBlocks, kCoreMetrics, core_red_.as<double>(), kCoreMetrics,
+                        T_, s);
   }
   LITHO_HIP(hipGetLastError());
   // FP64 scalars of every Micro: one transfer
@@ -218,7 +219,8 @@ void Exchanger::run(const float* masks, const litho_exchange_options& o, float* 
   if (any_strip)
     LITHO_HIP(hipMemcpyAsync(h + (size_t)T_ * 4, cept_red_.ptr, (size_t)T_ * sizeof(double), hipMemcpyDeviceToHost, s));
   if (o.collect_core_metrics)
-    LITHO_HIP(hipMemcpyAsync(h + (size_t)T_ * 5, core_red_.ptr, (size_t)T_ * 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    LITHO_HIP(hipMemcpyAsync(h + (size_t)T_ * 5, core_red_.ptr, (size_t)T_ * kCoreMetrics * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
   LITHO_HIP(hipStreamSynchronize(s));
   double loss = 0.0, data = 0.0, cept = 0.0;
   for (int t = 0; t < T_; ++t) {
@@ -229,8 +231,12 @@ void Exchanger::run(const float* masks, const litho_exchange_options& o, float* 
     r.data_loss = pe_w * r.pe_loss + r.penalty_loss;
     r.cept_loss = prm.cept ? h[(size_t)T_ * 4 + t] : 0.0;
     if (o.collect_core_metrics) {
-      r.core_pe_loss = h[(size_t)T_ * 5 + 2 * t];
-      r.core_bg_penalty_raw = h[(size_t)T_ * 5 + 2 * t + 1];
+      const double* cm = h + (size_t)T_ * 5 + (size_t)kCoreMetrics * t;
+      r.core_pe_loss = cm[0];
+      r.core_bg_penalty_raw = cm[1];
+      // CoreLossMetrics (micro_gradient_exchange.cpp:225-243)
+      r.core_pe_penalty_loss = o.pe_weight * cm[2] + o.penalty_weight * cm[1];
+      r.core_data_loss = pe_w * cm[0] + pen_w * cm[1];
     }
     r.loss = r.data_loss + (prm.cept ? o.cept_weight * r.cept_loss : 0.0);
     loss += r.loss;
@@ -253,6 +259,31 @@ struct ChipBuffers {
 };
 
 void alloc_tiles(DevMem& m, int T, int n) { m.alloc((size_t)T * n * n * sizeof(float)); }
+
+// One history row from the exchange's per-Micro scalars, summed in Micro
+// order (full_chip_ctm.cpp:84-104, full_chip_lsm.cpp:152-182).
+litho_fullchip_row history_row(int it, const double tot[3], const std::vector<litho_tile_loss>& per,
+                               const litho_exchange_options& o, double min_dt, double max_dt) {
+  litho_fullchip_row r{};
+  r.iteration = it;
+  r.loss = tot[0];
+  r.data_loss = tot[1];
+  r.cept_loss = tot[2];
+  r.min_dt = min_dt;
+  r.max_dt = max_dt;
+  for (const litho_tile_loss& l : per) {
+    r.pe_contribution += o.pe_weight * l.pe_loss;
+    r.penalty_contribution += l.penalty_loss;
+    r.pe_loss_raw += l.pe_loss;
+    r.bg_penalty_raw += l.bg_penalty_raw;
+    r.core_pe += l.core_pe_loss;
+    r.core_bg_penalty_raw += l.core_bg_penalty_raw;
+    r.core_pe_penalty += l.core_pe_penalty_loss;
+    r.core_data_loss += l.core_data_loss;
+  }
+  r.cept_contribution = o.enable_cept ? o.cept_weight * tot[2] : 0.0;
+  return r;
+}
 
 void record_best(int it, double loss, FullChipResult& res, bool& have, const float* masks, const float* state,
                  float* best_masks, float* best_state, size_t bytes, hipStream_t s) {
@@ -453,7 +484,7 @@ litho_mask_continuity mask_continuity(Context& ctx, const litho_tile* tiles, int
 FullChipResult fullchip_ctm(Plan& plan, const litho_tile* tiles, int T, int height, int width,
                             const float* initial_pattern, const float* global_target,
                             const litho_exchange_options& options, int iterations, double learning_rate,
-                            float* global_mask_out, float* local_masks_out, double* history) {
+                            float* global_mask_out, float* local_masks_out, litho_fullchip_row* history) {
   check_chip(plan, tiles, T, height, width, initial_pattern, global_target);
   if (iterations < 0) throw Error(1, "FullChipCTMOptions: iterations must be non-negative");
   if (!std::isfinite(learning_rate)) throw Error(1, "FullChipCTMOptions: learning_rate must be finite");
@@ -483,15 +514,12 @@ FullChipResult fullchip_ctm(Plan& plan, const litho_tile* tiles, int T, int heig
   int max_update = 0;
   for (int i = 0; i < T; ++i) max_update = std::max(max_update, area(tiles[i].update_local));
   double tot[3];
+  std::vector<litho_tile_loss> per(T);
   for (int it = 0; it <= iterations; ++it) {
     if (it > 0)   // full_chip_ctm.cpp:133-141: theta step on Core + Guard
       launch_theta_step(ex.table(), T, n, b.grad.as<float>(), learning_rate, theta, masks, max_update, s);
-    ex.run(masks, o, b.grad.as<float>(), nullptr, nullptr, nullptr, tot);
-    if (history != nullptr) {
-      history[3 * it + 0] = tot[0];
-      history[3 * it + 1] = tot[1];
-      history[3 * it + 2] = tot[2];
-    }
+    ex.run(masks, o, b.grad.as<float>(), nullptr, nullptr, history != nullptr ? per.data() : nullptr, tot);
+    if (history != nullptr) history[it] = history_row(it, tot, per, o, 0.0, 0.0);
     record_best(it, tot[0], res, have, masks, nullptr, b.best_masks.as<float>(), nullptr, bytes, s);
   }
   if (global_mask_out != nullptr)
@@ -506,7 +534,8 @@ FullChipResult fullchip_ctm(Plan& plan, const litho_tile* tiles, int T, int heig
 FullChipResult fullchip_lsm(Plan& plan, const litho_tile* tiles, int T, int height, int width,
                             const float* initial_binary, const float* global_target,
                             const litho_exchange_options& options, const LsmOptions& lo, int iterations,
-                            int reinit_interval, float* global_mask_out, float* global_phi_out, double* history) {
+                            int reinit_interval, float* global_mask_out, float* global_phi_out,
+                            litho_fullchip_row* history) {
   check_chip(plan, tiles, T, height, width, initial_binary, global_target);
   if (iterations < 0) throw Error(1, "FullChipLSMOptions: iterations must be non-negative");
   if (reinit_interval < 0) throw Error(1, "FullChipLSMOptions: reinit_interval must be non-negative");
@@ -554,6 +583,7 @@ FullChipResult fullchip_lsm(Plan& plan, const litho_tile* tiles, int T, int heig
   FullChipResult res;
   bool have = false;
   double tot[3];
+  std::vector<litho_tile_loss> per(T);
   for (int it = 0; it <= iterations; ++it) {
     double mn = 0.0, mx = 0.0;
     if (it > 0) {
@@ -562,15 +592,8 @@ FullChipResult fullchip_lsm(Plan& plan, const litho_tile* tiles, int T, int heig
       mn = *std::min_element(dt.begin(), dt.end());
       mx = *std::max_element(dt.begin(), dt.end());
     }
-    ex.run(masks, o, b.grad.as<float>(), nullptr, nullptr, nullptr, tot);
-    if (history != nullptr) {
-      double* h = history + 5 * (size_t)it;
-      h[0] = tot[0];
-      h[1] = tot[1];
-      h[2] = tot[2];
-      h[3] = mn;
-      h[4] = mx;
-    }
+    ex.run(masks, o, b.grad.as<float>(), nullptr, nullptr, history != nullptr ? per.data() : nullptr, tot);
+    if (history != nullptr) history[it] = history_row(it, tot, per, o, mn, mx);
     record_best(it, tot[0], res, have, masks, phi, b.best_masks.as<float>(), b.best_state.as<float>(), bytes, s);
   }
   if (global_mask_out != nullptr)

@@ -86,12 +86,13 @@ struct FullChipResult {
 FullChipResult fullchip_ctm(Plan& plan, const litho_tile* tiles, int num_tiles, int height, int width,
                             const float* initial_pattern, const float* global_target,
                             const litho_exchange_options& options, int iterations, double learning_rate,
-                            float* global_mask_out, float* local_masks_out, double* history);
+                            float* global_mask_out, float* local_masks_out, litho_fullchip_row* history);
 
 // FullChipLSMOptimizer::optimize, Binary render (full_chip_lsm.cpp:39-255).
 FullChipResult fullchip_lsm(Plan& plan, const litho_tile* tiles, int num_tiles, int height, int width,
                             const float* initial_binary, const float* global_target,
                             const litho_exchange_options& options, const LsmOptions& lsm, int iterations,
-                            int reinit_interval, float* global_mask_out, float* global_phi_out, double* history);
+                            int reinit_interval, float* global_mask_out, float* global_phi_out,
+                            litho_fullchip_row* history);
 
 }  // namespace litho

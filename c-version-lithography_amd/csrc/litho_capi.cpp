@@ -10,8 +10,10 @@
 
 #include "batch.hpp"
 #include "binarize.hpp"
+#include "epe.hpp"
 #include "ctm_driver.hpp"
 #include "fullchip.hpp"
+#include "io.hpp"
 #include "lsm.hpp"
 #include "noalloc.hpp"
 #include "optics.hpp"
@@ -22,6 +24,10 @@
 
 struct litho_ctx {
   litho::Context c;
+};
+
+struct litho_ep_set {
+  litho::EpSet* s;
 };
 
 struct litho_plan {
@@ -755,7 +761,7 @@ int litho_tile_exchange(litho_plan* plan, const litho_tile* tiles, int num_tiles
 int litho_fullchip_ctm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
                        const float* initial_pattern, const float* global_target, const litho_exchange_options* options,
                        int iterations, double learning_rate, float* global_mask_out, float* local_masks_out,
-                       double* history, int* best_iteration, double* best_loss) {
+                       litho_fullchip_row* history, int* best_iteration, double* best_loss) {
   return guarded([&] {
     litho::Plan& p = plan_of(plan);
     require(options != nullptr, "null options");
@@ -782,7 +788,7 @@ int litho_micro_imaging(litho_plan* plan, const litho_tile* tiles, int num_tiles
 int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
                        const float* initial_binary, const float* global_target, const litho_exchange_options* options,
                        const litho_lsm_options* lsm, int iterations, int reinit_interval, float* global_mask_out,
-                       float* global_phi_out, double* history, int* best_iteration, double* best_loss) {
+                       float* global_phi_out, litho_fullchip_row* history, int* best_iteration, double* best_loss) {
   return guarded([&] {
     litho::Plan& p = plan_of(plan);
     require(options != nullptr && lsm != nullptr, "null options");
@@ -796,7 +802,6 @@ int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
   });
 }
 
-}  // extern "C"
 
 int litho_binarize_advanced(litho_ctx* ctx, int height, int width, const float* gray_mask, const float* target_pattern,
                             double peak_min_intensity, double quantile_val, int area_threshold,
@@ -856,3 +861,93 @@ int litho_evaluate_mask_continuity(litho_ctx* ctx, const litho_tile* tiles, int 
     *out = litho::mask_continuity(c, tiles, num_tiles, tile_size, height, width, local_masks, target);
   });
 }
+
+int litho_ep_set_create(litho_plan* plan, int count, const double* points_yx, const double* weights,
+                        litho_ep_set** out) {
+  return guarded([&] {
+    litho::Plan& p = plan_of(plan);
+    require(out != nullptr, "null output handle");
+    set_device(p);
+    litho::EpSet* e = litho::make_ep_set(p, count, points_yx, weights);
+    *out = new litho_ep_set{e};
+  });
+}
+
+int litho_ep_set_destroy(litho_ep_set* set) {
+  return guarded([&] {
+    if (set == nullptr) return;
+    if (set->s != nullptr) {
+      (void)hipSetDevice(set->s->plan->ctx->device);
+      (void)hipStreamSynchronize(set->s->plan->ctx->stream);
+      delete set->s;
+    }
+    delete set;
+  });
+}
+
+int litho_epe_gradient(litho_ep_set* set, const float* mask, const float* target, const float* pe_window,
+                       double pe_weight, double epe_weight, double threshold, double alpha, double pixel_size_nm,
+                       float* grad_mask, float* aerial, float* wafer, litho_epe_result* result) {
+  return guarded([&] {
+    require(set != nullptr && set->s != nullptr, "null edge-point set");
+    set_device(*set->s->plan);
+    const litho::EpeResult r = litho::epe_gradient(*set->s, mask, target, pe_window, pe_weight, epe_weight, threshold,
+                                                   alpha, pixel_size_nm, grad_mask, aerial, wafer);
+    if (result != nullptr) {
+      result->pe_loss = r.pe_loss;
+      result->epe_loss = r.epe_loss;
+      result->epe_abs_sum_nm = r.epe_abs_sum_nm;
+      result->active_count = r.active;
+    }
+  });
+}
+
+int litho_save_txt(const char* path, const double* matrix, int rows, int cols, int precision) {
+  return guarded([&] { litho::save_txt(path, matrix, rows, cols, precision); });
+}
+
+int litho_save_txt_int(const char* path, const double* matrix, int rows, int cols) {
+  return guarded([&] { litho::save_txt_int(path, matrix, rows, cols); });
+}
+
+int litho_load_txt(const char* path, double* matrix, size_t capacity, int* rows, int* cols) {
+  return guarded([&] {
+    require(rows != nullptr && cols != nullptr, "null row / column outputs");
+    std::vector<double> v;
+    int r = 0, c = 0;
+    litho::load_txt(path, v, &r, &c);
+    *rows = r;
+    *cols = c;
+    if (matrix == nullptr) return;
+    require(capacity >= v.size(), "load_txt: the output array is smaller than the matrix");
+    std::copy(v.begin(), v.end(), matrix);
+  });
+}
+
+int litho_save_matrix_txt(const char* path, const double* matrix, int rows, int cols) {
+  return guarded([&] { litho::save_matrix_txt(path, matrix, rows, cols); });
+}
+
+int litho_write_history_csv(const char* path, int stage, const litho_fullchip_row* rows, int count) {
+  return guarded([&] { litho::write_history_csv(path, stage, rows, count); });
+}
+
+int litho_write_summary_csv(const char* path, int stage, int best_iteration, double best_loss, int final_iteration,
+                            double final_loss, double seconds) {
+  return guarded(
+      [&] { litho::write_summary_csv(path, stage, best_iteration, best_loss, final_iteration, final_loss, seconds); });
+}
+
+int litho_write_imaging_metrics_csv(const char* path, int stage, double target_pe, double optimized_pe) {
+  return guarded([&] { litho::write_imaging_metrics_csv(path, stage, target_pe, optimized_pe); });
+}
+
+int litho_write_sampling_csv(const char* path, int grid_size, int source_grid_size, int source_points, int kernels,
+                             double pixel_size_nm, int micro_core, int image_rows, int image_cols, int micro_count) {
+  return guarded([&] {
+    litho::write_sampling_csv(path, grid_size, source_grid_size, source_points, kernels, pixel_size_nm, micro_core,
+                              image_rows, image_cols, micro_count);
+  });
+}
+
+}  // extern "C"

@@ -387,6 +387,7 @@ k_col_image_adj(SocsArgs a, ImageArgs im) {
           drive += im.penalty * bgm * ex * bw;
         }
         g[s] = drive * w * (1.0f - w);
+        if (im.extra_dldi != nullptr) g[s] += im.extra_dldi[gi];
         if (im.dldi != nullptr) im.dldi[gi] = g[s];
       }
     }
@@ -856,6 +857,7 @@ k_col_image_adj_pruned(SocsArgs a, ImageArgs im) {
           drive += im.penalty * bgm * ex * bw;
         }
         g[s][r] = drive * w * (1.0f - w);
+        if (im.extra_dldi != nullptr) g[s][r] += im.extra_dldi[gi];
         if (im.dldi != nullptr) im.dldi[gi] = g[s][r];
       }
     }
@@ -962,6 +964,7 @@ __device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, flo
       drive += im.penalty * bgm * ex * bw;
     }
     g = drive * w * (1.0f - w);
+    if (im.extra_dldi != nullptr) g += im.extra_dldi[gi];
     if (im.dldi != nullptr) im.dldi[gi] = g;
   }
   return g;

@@ -57,6 +57,7 @@ struct ImageArgs {
   const float* pen_win;           // null = no background penalty
   float pe_weight, penalty, pen_thr;
   float* dldi;                    // optional debug output
+  const float* extra_dldi;        // optional: added to dL/dI (before the adjoint's scale) where target is set
   double* partials;               // [b][image blocks][2]: PE, background penalty
 };
 

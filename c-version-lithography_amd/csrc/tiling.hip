@@ -210,29 +210,27 @@ k_core_metrics(const int* __restrict__ table, int n, const float* __restrict__ w
   const int cols = cols_of(c);
   const size_t cnt = (size_t)rows_of(c) * cols;
   const size_t base = (size_t)t * n * n;
-  double pe = 0.0, bg = 0.0;
+  double v[3] = {0.0, 0.0, 0.0};   // PE (loss window), background penalty, PE on the main pattern (target > 0.5)
   for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < cnt; q += (size_t)gridDim.x * blockDim.x) {
     const int y = (int)(q / cols), x = (int)(q - (size_t)y * cols);
     const size_t p = base + (size_t)(c[0] + y) * n + c[2] + x;
     const double w = wafer[p], tv = targets[p];
     const double d = w - tv;
-    pe += d * d * (double)pe_win[p];
+    v[0] += d * d * (double)pe_win[p];
     const double ex = w - pen_thr > 0.0 ? w - pen_thr : 0.0;
-    bg += (double)pen_win[p] * (tv < 0.5 ? 1.0 : 0.0) * ex * ex;
+    v[1] += (double)pen_win[p] * (tv < 0.5 ? 1.0 : 0.0) * ex * ex;
+    v[2] += tv > 0.5 ? d * d : 0.0;
   }
-  __shared__ double sh[2][4];
-  for (int off = 32; off > 0; off >>= 1) {
-    pe += __shfl_down(pe, off, 64);
-    bg += __shfl_down(bg, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][threadIdx.x >> 6] = pe;
-    sh[1][threadIdx.x >> 6] = bg;
+  __shared__ double sh[kCoreMetrics][4];
+  for (int i = 0; i < kCoreMetrics; ++i) {
+    double x = v[i];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[i][threadIdx.x >> 6] = x;
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
-    const double* v = sh[threadIdx.x];
-    partial[((size_t)t * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = ((v[0] + v[1]) + v[2]) + v[3];
+  if (threadIdx.x < kCoreMetrics) {
+    const double* r = sh[threadIdx.x];
+    partial[((size_t)t * gridDim.x + blockIdx.x) * kCoreMetrics + threadIdx.x] = ((r[0] + r[1]) + r[2]) + r[3];
   }
 }
 

@@ -45,7 +45,9 @@ void launch_guard_exchange(const int* table, int num_tiles, int n, const float* 
 // Two-sided CEPT, Core side of neighbour slot `slot`: mask_gradient[ov] += cept_weight * 2 (m[ov] - m_nb[os]).
 void launch_overlap_cept(const int* table, int num_tiles, int n, int slot, const float* masks, float* mask_gradient,
                          double cept_weight, int max_strip, hipStream_t s);
-// Core metrics: partial[(t * kMetricBlocks + blk) * 2 + {0, 1}] = PE, background penalty over the Core.
+// Core metrics: partial[(t * kMetricBlocks + blk) * kCoreMetrics + {0, 1, 2}] = PE (loss window),
+// background penalty and PE on the main pattern (target > 0.5) over the Core.
+constexpr int kCoreMetrics = 3;
 void launch_core_metrics(const int* table, int num_tiles, int n, const float* wafer, const float* targets,
                          const float* pe_win, const float* pen_win, double penalty_threshold, int max_core_elems,
                          double* partial, hipStream_t s);

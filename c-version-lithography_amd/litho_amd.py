@@ -99,7 +99,8 @@ def exchange_options(loss_type="PE", loss_region="Core", pe_weight=1.0, penalty_
 class TileLoss(C.Structure):
     _fields_ = [("loss", C.c_double), ("data_loss", C.c_double), ("pe_loss", C.c_double),
                 ("penalty_loss", C.c_double), ("bg_penalty_raw", C.c_double), ("cept_loss", C.c_double),
-                ("core_pe_loss", C.c_double), ("core_bg_penalty_raw", C.c_double)]
+                ("core_pe_loss", C.c_double), ("core_bg_penalty_raw", C.c_double),
+                ("core_pe_penalty_loss", C.c_double), ("core_data_loss", C.c_double)]
 
 
 class BinarizeInfo(C.Structure):
@@ -112,6 +113,25 @@ class MaskContinuity(C.Structure):
                 ("flat_seam_samples", C.c_longlong), ("foreground_seam_samples", C.c_longlong),
                 ("seam_jump_mae", C.c_double), ("seam_reference_error_mae", C.c_double),
                 ("flat_seam_jump_mae", C.c_double), ("foreground_seam_jump_mae", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EpeResult(C.Structure):
+    _fields_ = [("pe_loss", C.c_double), ("epe_loss", C.c_double), ("epe_abs_sum_nm", C.c_double),
+                ("active_count", C.c_int)]
+
+
+class FullChipRow(C.Structure):
+    _fields_ = [("iteration", C.c_int), ("loss", C.c_double), ("data_loss", C.c_double), ("cept_loss", C.c_double),
+                ("min_dt", C.c_double), ("max_dt", C.c_double), ("pe_contribution", C.c_double),
+                ("penalty_contribution", C.c_double), ("epe_contribution", C.c_double),
+                ("cept_contribution", C.c_double), ("pe_loss_raw", C.c_double), ("bg_penalty_raw", C.c_double),
+                ("epe_loss_raw", C.c_double), ("epe_abs_sum_nm", C.c_double), ("epe_point_count", C.c_int),
+                ("core_pe", C.c_double), ("core_bg_penalty_raw", C.c_double), ("core_pe_penalty", C.c_double),
+                ("core_epe_loss", C.c_double), ("core_epe_abs_sum_nm", C.c_double), ("core_ep_count", C.c_int),
+                ("core_epe_mean", C.c_double), ("core_data_loss", C.c_double), ("epe_mean", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -168,6 +188,9 @@ SIGNATURES = {
     "litho_compare_micro_imaging": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P,
                                          _DP, _DP]),
     "litho_evaluate_mask_continuity": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _I, _P, _P, C.POINTER(MaskContinuity)]),
+    "litho_ep_set_create": (_I, [_P, _I, _DP, _DP, C.POINTER(_P)]),
+    "litho_ep_set_destroy": (_I, [_P]),
+    "litho_epe_gradient": (_I, [_P, _P, _P, _P, _D, _D, _D, _D, _D, _P, _P, _P, C.POINTER(EpeResult)]),
     "litho_binarize_advanced": (_I, [_P, _I, _I, _P, _P, _D, _D, _I, _I, _P, _P, _P, _P, _P, _P,
                                      C.POINTER(BinarizeInfo)]),
     "litho_render_levelset_mask": (_I, [_P, _I, _I, _I, _P, _I, _I, _P]),
@@ -179,9 +202,17 @@ SIGNATURES = {
     "litho_tile_exchange": (_I, [_P, C.POINTER(Tile), _I, _P, _P, C.POINTER(ExchangeOptions), _P, _P, _P,
                                  C.POINTER(TileLoss), _DP]),
     "litho_fullchip_ctm": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _P, C.POINTER(ExchangeOptions), _I, _D, _P, _P,
-                                _DP, _IP, _DP]),
+                                C.POINTER(FullChipRow), _IP, _DP]),
     "litho_fullchip_lsm": (_I, [_P, C.POINTER(Tile), _I, _I, _I, _P, _P, C.POINTER(ExchangeOptions),
-                                C.POINTER(LsmOptions), _I, _I, _P, _P, _DP, _IP, _DP]),
+                                C.POINTER(LsmOptions), _I, _I, _P, _P, C.POINTER(FullChipRow), _IP, _DP]),
+    "litho_save_txt": (_I, [C.c_char_p, _DP, _I, _I, _I]),
+    "litho_save_txt_int": (_I, [C.c_char_p, _DP, _I, _I]),
+    "litho_load_txt": (_I, [C.c_char_p, _DP, C.c_size_t, _IP, _IP]),
+    "litho_save_matrix_txt": (_I, [C.c_char_p, _DP, _I, _I]),
+    "litho_write_history_csv": (_I, [C.c_char_p, _I, C.POINTER(FullChipRow), _I]),
+    "litho_write_summary_csv": (_I, [C.c_char_p, _I, _I, _D, _I, _D, _D]),
+    "litho_write_imaging_metrics_csv": (_I, [C.c_char_p, _I, _D, _D]),
+    "litho_write_sampling_csv": (_I, [C.c_char_p, _I, _I, _I, _I, _D, _I, _I, _I, _I]),
 }
 
 _lib = None
@@ -541,12 +572,13 @@ class Plan:
     def fullchip_ctm(self, tiles, initial_pattern, target, options, iterations, learning_rate, global_mask_out,
                      local_masks_out=None):
         h, w = target.shape
-        hist = np.zeros((iterations + 1, 3))
+        rows = (FullChipRow * (iterations + 1))()
         bi, bl = C.c_int(), C.c_double()
         check(self.lib.litho_fullchip_ctm(self.h, tiles, len(tiles), h, w, ptr(initial_pattern), ptr(target),
                                           C.byref(options), iterations, learning_rate, ptr(global_mask_out),
-                                          ptr(local_masks_out), _dp(hist), C.byref(bi), C.byref(bl)))
-        return dict(best_iteration=bi.value, best_loss=bl.value, history=hist)
+                                          ptr(local_masks_out), rows, C.byref(bi), C.byref(bl)))
+        hist = np.array([[r.loss, r.data_loss, r.cept_loss] for r in rows])
+        return dict(best_iteration=bi.value, best_loss=bl.value, history=hist, rows=rows)
 
     def micro_imaging(self, tiles, global_mask, global_aerial=None, global_wafer=None, local_aerial=None,
                       local_wafer=None, threshold=0.25, alpha=50.0):
@@ -560,14 +592,15 @@ class Plan:
                      dx=1.0, dy=1.0, cfl=0.7, curvature_weight=0.01, reinit_interval=20, mask_type="Binary",
                      render_scale=4):
         h, w = target.shape
-        hist = np.zeros((iterations + 1, 5))
+        rows = (FullChipRow * (iterations + 1))()
         bi, bl = C.c_int(), C.c_double()
         lo = LsmOptions(dx, dy, cfl, curvature_weight, _MASK_TYPES[mask_type], render_scale)
         check(self.lib.litho_fullchip_lsm(self.h, tiles, len(tiles), h, w, ptr(initial_binary), ptr(target),
                                           C.byref(options), C.byref(lo), iterations, reinit_interval,
-                                          ptr(global_mask_out), ptr(global_phi_out), _dp(hist), C.byref(bi),
+                                          ptr(global_mask_out), ptr(global_phi_out), rows, C.byref(bi),
                                           C.byref(bl)))
-        return dict(best_iteration=bi.value, best_loss=bl.value, history=hist)
+        hist = np.array([[r.loss, r.data_loss, r.cept_loss, r.min_dt, r.max_dt] for r in rows])
+        return dict(best_iteration=bi.value, best_loss=bl.value, history=hist, rows=rows)
 
     def pe_gradient(self, mask, target, grad_mask=None, aerial=None, wafer=None, pe_window=None,
                     penalty_window=None, pe_weight=1.0, penalty=0.0, penalty_threshold=0.01,
@@ -610,6 +643,92 @@ class Plan:
 
 def supported_size(n: int) -> bool:
     return bool(load().litho_supported_size(n))
+
+
+class EpSet:
+    """litho_ep_set: edge points (count, 2) (y, x) and weights of one plan's tile size."""
+
+    def __init__(self, plan, points_yx, weights):
+        self.lib = plan.lib
+        self.plan = plan
+        p = np.ascontiguousarray(points_yx, dtype=np.float64).reshape(-1, 2)
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        h = _P()
+        check(self.lib.litho_ep_set_create(plan.h, len(w), _dp(p), _dp(w), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self.lib.litho_ep_set_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gradient(self, mask, target=None, grad_mask=None, aerial=None, wafer=None, pe_window=None, pe_weight=1.0,
+                 epe_weight=1.0, threshold=0.25, alpha=50.0, pixel_size_nm=4.0):
+        r = EpeResult()
+        check(self.lib.litho_epe_gradient(self.h, ptr(mask), ptr(target), ptr(pe_window), pe_weight, epe_weight,
+                                          threshold, alpha, pixel_size_nm, ptr(grad_mask), ptr(aerial), ptr(wafer),
+                                          C.byref(r)))
+        return dict(pe_loss=r.pe_loss, epe_loss=r.epe_loss, epe_abs_sum_nm=r.epe_abs_sum_nm,
+                    active_count=r.active_count)
+
+
+# ------------------------------------------------------------------ on-disk formats
+def _path(p) -> bytes:
+    return os.fsencode(os.fspath(p))
+
+
+def save_txt(path, matrix, precision=4):
+    """SaveTxt::save_mat of a 2-D array."""
+    m = np.ascontiguousarray(matrix, dtype=np.float64)
+    check(load().litho_save_txt(_path(path), _dp(m), m.shape[0], m.shape[1], precision))
+
+
+def save_txt_int(path, matrix):
+    m = np.ascontiguousarray(matrix, dtype=np.float64)
+    check(load().litho_save_txt_int(_path(path), _dp(m), m.shape[0], m.shape[1]))
+
+
+def load_txt(path) -> np.ndarray:
+    """SaveTxt::load_txt: the matrix of a TXT file (rows x cols float64)."""
+    lib = load()
+    r, c = C.c_int(), C.c_int()
+    check(lib.litho_load_txt(_path(path), None, 0, C.byref(r), C.byref(c)))
+    out = np.zeros((r.value, c.value))
+    check(lib.litho_load_txt(_path(path), _dp(out), out.size, C.byref(r), C.byref(c)))
+    return out
+
+
+def save_matrix_txt(path, matrix):
+    """The full-chip demos' "# rows= cols=" matrix TXT (precision 17, aligned)."""
+    m = np.ascontiguousarray(matrix, dtype=np.float64)
+    check(load().litho_save_matrix_txt(_path(path), _dp(m), m.shape[0], m.shape[1]))
+
+
+def write_history_csv(path, stage, rows):
+    """history.csv of the full-chip CTM ("CTM") or LSM ("LSM") demo from FullChipRow rows."""
+    arr = (FullChipRow * len(rows))(*rows)
+    check(load().litho_write_history_csv(_path(path), {"CTM": 0, "LSM": 1}[stage], arr, len(rows)))
+
+
+def write_summary_csv(path, stage, best_iteration, best_loss, final_iteration, final_loss, seconds=0.0):
+    check(load().litho_write_summary_csv(_path(path), {"CTM": 0, "LSM": 1}[stage], best_iteration, best_loss,
+                                         final_iteration, final_loss, seconds))
+
+
+def write_imaging_metrics_csv(path, stage, target_pe, optimized_pe):
+    check(load().litho_write_imaging_metrics_csv(_path(path), {"CTM": 0, "LSM": 1}[stage], target_pe, optimized_pe))
+
+
+def write_sampling_csv(path, grid_size, source_grid_size, source_points, kernels, pixel_size_nm, micro_core,
+                       image_rows, image_cols, micro_count):
+    check(load().litho_write_sampling_csv(_path(path), grid_size, source_grid_size, source_points, kernels,
+                                          pixel_size_nm, 1 if micro_core else 0, image_rows, image_cols, micro_count))
 
 
 def initialize_theta(ctx: Context, pattern, theta) -> None:

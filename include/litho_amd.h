@@ -42,6 +42,7 @@ extern "C" {
 
 typedef struct litho_ctx litho_ctx;
 typedef struct litho_plan litho_plan;
+typedef struct litho_ep_set litho_ep_set;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* litho_last_error(void);
@@ -269,6 +270,38 @@ int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, doubl
 int litho_lbfgs_direction(litho_ctx* ctx, size_t n, int m, int num_points, const float* x_points,
                           const float* g_points, float* d_out, double* gd_out, int* kept_out);
 
+/* ---------------------------------------------------------------- EPE
+ * Edge points of one tile size (EpsResult: eps rows (y, x) in fractional
+ * pixel coordinates, weight_epe), with their bilinear stencils
+ * (make_bilinear_stencil, epe_utils.cpp:38-64).  points_yx: host count x 2
+ * (y, x); weights: host count values (points with weight <= 0 inactive).
+ * The set uses the plan; destroy it before the plan. */
+int litho_ep_set_create(litho_plan* plan, int count, const double* points_yx, const double* weights,
+                        litho_ep_set** out);
+int litho_ep_set_destroy(litho_ep_set* set);
+
+typedef struct {
+  double pe_loss;           /* sum (W - T)^2 pe_window; 0 without a target */
+  double epe_loss;          /* sum w ((I_ep - threshold) / |grad I|_ep)^2, pixel^2 */
+  double epe_abs_sum_nm;    /* sum |EPE| pixel_size_nm over the active points */
+  int active_count;
+} litho_epe_result;
+
+/* Loss::evaluate_sparse_epe (loss.cpp:125-163) and Gradient::epe_gradient
+ * (gradient.cpp:198-227) of one tile at the edge points, on the device:
+ * I_ep and |grad I|_ep (central differences, one-sided at the edges, clamped
+ * below at 1e-6 per pixel and per sample) sampled bilinearly,
+ * dL/dI_ep = 2 w (I_ep - threshold) / |grad I|_ep^2 scattered back, adjoint
+ * with factor 2.  With a target the result is added, weighted, to the PE
+ * gradient of litho_pe_gradient (pe_window nullable = ones), the PE_EPE
+ * combination of micro_gradient_exchange.cpp:208-221:
+ *   grad_mask = pe_weight * dPE/dM + epe_weight * dEPE/dM.
+ * target null: epe_weight * dEPE/dM alone.  mask, target, pe_window,
+ * grad_mask, aerial, wafer: device N x N (outputs nullable).  result: host. */
+int litho_epe_gradient(litho_ep_set* set, const float* mask, const float* target, const float* pe_window,
+                       double pe_weight, double epe_weight, double threshold, double alpha, double pixel_size_nm,
+                       float* grad_mask, float* aerial, float* wafer, litho_epe_result* result);
+
 /* CTM_Optimizer::initialize_theta (CTM_Optimizer.cpp:66-73): 0.3 pi where
  * pattern > 0, else 0.7 pi.  Device arrays of n*n floats. */
 int litho_initialize_theta(litho_ctx* ctx, const float* pattern, float* theta, size_t count);
@@ -400,7 +433,10 @@ typedef struct {
 
 typedef struct {                 /* MicroGradientData scalars (micro_gradient_exchange.h:98-112) */
   double loss, data_loss, pe_loss, penalty_loss, bg_penalty_raw, cept_loss;
-  double core_pe_loss, core_bg_penalty_raw;   /* collect_core_metrics only */
+  /* collect_core_metrics only, CoreLossMetrics (micro_gradient_exchange.h:64-73): */
+  double core_pe_loss, core_bg_penalty_raw;
+  double core_pe_penalty_loss;   /* pe_weight * PE on the Core's main pattern + penalty_weight * core_bg_penalty_raw */
+  double core_data_loss;         /* the selected loss on the Core alone, without CEPT */
 } litho_tile_loss;
 
 /* MicroGradientExchange::compute(local_masks, ...) (micro_gradient_exchange.cpp:134-334)
@@ -459,6 +495,24 @@ int litho_evaluate_mask_continuity(litho_ctx* ctx, const litho_tile* tiles, int 
                                    int width, const float* local_masks, const float* target,
                                    litho_mask_continuity* out);
 
+/* One state of a full-chip run: FullChipCTMIteration (full_chip_ctm.h:19-29)
+ * and FullChipLSMIteration (full_chip_lsm.h), the columns of the demos'
+ * history.csv.  Contributions are weighted (0 for a term that is off); raw
+ * values are summed over the Micros' loss regions; core_* sum the Micros'
+ * CoreLossMetrics (collected by the LSM, else 0).  The EPE columns stay 0:
+ * the exchange runs the PE / PEPenalty losses. */
+typedef struct {
+  int iteration;
+  double loss, data_loss, cept_loss;
+  double min_dt, max_dt;                                  /* LSM: the update's step range; CTM: 0 */
+  double pe_contribution, penalty_contribution, epe_contribution, cept_contribution;
+  double pe_loss_raw, bg_penalty_raw, epe_loss_raw, epe_abs_sum_nm;
+  int epe_point_count;
+  double core_pe, core_bg_penalty_raw, core_pe_penalty, core_epe_loss, core_epe_abs_sum_nm;
+  int core_ep_count;
+  double core_epe_mean, core_data_loss, epe_mean;
+} litho_fullchip_row;
+
 /* FullChipCTMOptimizer::optimize(initial_pattern, global_target, options)
  * (full_chip_ctm.cpp:25-152): theta per Micro from initialize_theta of the
  * pattern, fixed-step gradient descent on theta inside Core + Guard (Halo frozen),
@@ -466,11 +520,11 @@ int litho_evaluate_mask_continuity(litho_ctx* ctx, const litho_tile* tiles, int 
  * initial_pattern, global_target: device height x width.  global_mask_out:
  * device height x width (best Micros' Cores).  local_masks_out (device,
  * nullable): best Micros, [num_tiles][tile_size^2].  history (host, nullable):
- * (iterations + 1) x {loss, data_loss, cept_loss}. */
+ * iterations + 1 rows. */
 int litho_fullchip_ctm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
                        const float* initial_pattern, const float* global_target, const litho_exchange_options* options,
                        int iterations, double learning_rate, float* global_mask_out, float* local_masks_out,
-                       double* history, int* best_iteration, double* best_loss);
+                       litho_fullchip_row* history, int* best_iteration, double* best_loss);
 
 /* FullChipLSMOptimizer::optimize(initial_binary, global_target, options)
  * (full_chip_lsm.cpp:39-255) with the renderer of lsm->mask_type: phi per Micro = reinit_sdf(mask - 0.5),
@@ -478,11 +532,11 @@ int litho_fullchip_ctm(litho_plan* plan, const litho_tile* tiles, int num_tiles,
  * each iteration one RK2 update (litho_lsm_step) of every Micro with re-init
  * every reinit_interval updates (0: never), then the exchange.
  * global_mask_out / global_phi_out: device height x width (nullable).
- * history (host, nullable): (iterations + 1) x {loss, data_loss, cept_loss, min_dt, max_dt}. */
+ * history (host, nullable): iterations + 1 rows. */
 int litho_fullchip_lsm(litho_plan* plan, const litho_tile* tiles, int num_tiles, int height, int width,
                        const float* initial_binary, const float* global_target, const litho_exchange_options* options,
                        const litho_lsm_options* lsm, int iterations, int reinit_interval, float* global_mask_out,
-                       float* global_phi_out, double* history, int* best_iteration, double* best_loss);
+                       float* global_phi_out, litho_fullchip_row* history, int* best_iteration, double* best_loss);
 
 /* ---------------------------------------------------------------- CTM -> binary
  * Imbinarize::binarize_advanced(gray_mask, target_pattern, peak_min_intensity,
@@ -512,6 +566,37 @@ int litho_binarize_advanced(litho_ctx* ctx, int height, int width, const float* 
                             int min_spacing_pixels, float* final_mask, float* initial_binary_mask,
                             float* filtered_binary_mask, float* sraf_mask_initial, float* sraf_mask_final,
                             float* keep_out_zone, litho_binarize_info* info);
+
+/* ---------------------------------------------------------------- on-disk formats
+ * Host-only text output of the reference's callers.  Matrices are host
+ * row-major double arrays (row = y); paths are used as given.
+ *
+ * SaveTxt::save_mat (save_txt.cpp:23-37): "# rows=R cols=C", then one line per
+ * row of fixed-point values at `precision` decimals separated by one space. */
+int litho_save_txt(const char* path, const double* matrix, int rows, int cols, int precision);
+/* SaveTxt::save_mat_int (save_txt.cpp:39-51): the same with values rounded to integers. */
+int litho_save_txt_int(const char* path, const double* matrix, int rows, int cols);
+/* SaveTxt::load_txt (save_txt.cpp:58-105): skips blank and '#' lines, reads
+ * whitespace-separated values, requires a constant column count (else
+ * LITHO_ERUNTIME, as the reference's runtime_error).  matrix null: only
+ * *rows / *cols; else capacity >= rows * cols values are written. */
+int litho_load_txt(const char* path, double* matrix, size_t capacity, int* rows, int* cols);
+/* The matrix TXT of the full-chip demos and save_full_chip_imaging_comparison
+ * (full_chip_imaging_compare.cpp:76-80): "# rows=R cols=C", then the Eigen
+ * stream of the matrix at precision 17 (every value right-aligned to the
+ * widest value's length, one space between values). */
+int litho_save_matrix_txt(const char* path, const double* matrix, int rows, int cols);
+/* history.csv of the full-chip CTM (stage 0; demo_full_chip_ctm.cpp:112-122)
+ * or LSM demo (stage 1; demo_full_chip_lsm.cpp:170-184), precision 17. */
+int litho_write_history_csv(const char* path, int stage, const litho_fullchip_row* rows, int count);
+/* summary.csv (demo_full_chip_ctm.cpp:124-128; LSM adds `seconds`, demo_full_chip_lsm.cpp:196-199). */
+int litho_write_summary_csv(const char* path, int stage, int best_iteration, double best_loss, int final_iteration,
+                            double final_loss, double seconds);
+/* metrics.csv and sampling.csv of save_full_chip_imaging_comparison
+ * (full_chip_imaging_compare.cpp:94-101); micro_core != 0 writes "micro_core". */
+int litho_write_imaging_metrics_csv(const char* path, int stage, double target_pe, double optimized_pe);
+int litho_write_sampling_csv(const char* path, int grid_size, int source_grid_size, int source_points, int kernels,
+                             double pixel_size_nm, int micro_core, int image_rows, int image_cols, int micro_count);
 
 #ifdef __cplusplus
 }

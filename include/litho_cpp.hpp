@@ -486,8 +486,10 @@ class MicroImaging {
   const PartitionMeta& meta_;
 };
 
-struct CoreLossMetrics {
-  double pe_loss = 0.0, bg_penalty_raw = 0.0;
+struct CoreLossMetrics {             // micro_gradient_exchange.h:64-73 (EPE fields stay 0: no EPE mode here)
+  double pe_loss = 0.0, bg_penalty_raw = 0.0, pe_penalty_loss = 0.0, epe_loss = 0.0, epe_abs_sum_nm = 0.0;
+  int ep_count = 0;
+  double epe_mean = 0.0, data_loss = 0.0;
 };
 
 struct MicroGradientData {
@@ -541,6 +543,8 @@ class MicroGradientExchange {
       d.cept_loss = per[i].cept_loss;
       d.core.pe_loss = per[i].core_pe_loss;
       d.core.bg_penalty_raw = per[i].core_bg_penalty_raw;
+      d.core.pe_penalty_loss = per[i].core_pe_penalty_loss;
+      d.core.data_loss = per[i].core_data_loss;
       d.raw_mask_gradient = raw.download(n, n, plane * i);
       d.mask_gradient = mg.download(n, n, plane * i);
       r.locals.push_back(std::move(d));
@@ -566,9 +570,12 @@ struct FullChipCTMOptions {         // full_chip_ctm.h:10-17
   double learning_rate = 0.9;
   MicroGradientOptions gradient_options{};
 };
-struct FullChipCTMIteration {
+struct FullChipCTMIteration {       // full_chip_ctm.h:19-29
   int iteration = 0;
   double loss = 0.0, data_loss = 0.0, cept_loss = 0.0;
+  double pe_contribution = 0.0, penalty_contribution = 0.0, epe_contribution = 0.0, cept_contribution = 0.0;
+  double pe_loss_raw = 0.0, bg_penalty_raw = 0.0, epe_loss_raw = 0.0, epe_abs_sum_nm = 0.0;
+  int epe_point_count = 0;
 };
 struct FullChipCTMResult {
   int best_iteration = 0;
@@ -595,13 +602,16 @@ class FullChipCTMOptimizer {
     p.upload(initial_pattern);
     t.upload(global_target);
     const litho_exchange_options o = detail::to_abi(options.gradient_options, thr_, alpha_);
-    std::vector<double> h(3 * (size_t)(options.iterations + 1));
+    std::vector<litho_fullchip_row> h((size_t)(options.iterations + 1));
     FullChipCTMResult r;
     detail::check(litho_fullchip_ctm(cache_.plan(), meta_.tiles.data(), (int)meta_.tiles.size(), meta_.height,
                                      meta_.width, p.get(), t.get(), &o, options.iterations, options.learning_rate,
                                      out.get(), nullptr, h.data(), &r.best_iteration, &r.best_loss));
     r.global_mask = out.download(meta_.height, meta_.width);
-    for (int i = 0; i <= options.iterations; ++i) r.history.push_back({i, h[3 * i], h[3 * i + 1], h[3 * i + 2]});
+    for (const litho_fullchip_row& x : h)
+      r.history.push_back({x.iteration, x.loss, x.data_loss, x.cept_loss, x.pe_contribution, x.penalty_contribution,
+                           x.epe_contribution, x.cept_contribution, x.pe_loss_raw, x.bg_penalty_raw, x.epe_loss_raw,
+                           x.epe_abs_sum_nm, x.epe_point_count});
     return r;
   }
 
@@ -641,9 +651,14 @@ struct FullChipLSMOptions {         // full_chip_lsm.h:10-25
   int render_scale = 4;             // Gray: upscale per direction
   MicroGradientOptions gradient_options{};
 };
-struct FullChipLSMIteration {
+struct FullChipLSMIteration {       // full_chip_lsm.h: the columns of demo_full_chip_lsm.cpp:170-184
   int iteration = 0;
   double loss = 0.0, data_loss = 0.0, cept_loss = 0.0, min_dt = 0.0, max_dt = 0.0;
+  double pe_contribution = 0.0, penalty_contribution = 0.0, epe_contribution = 0.0, cept_contribution = 0.0;
+  CoreLossMetrics core;
+  double total_pe_loss = 0.0, total_bg_penalty_raw = 0.0, total_epe_loss = 0.0, total_epe_abs_sum_nm = 0.0;
+  int total_ep_count = 0;
+  double epe_mean = 0.0;
 };
 struct FullChipLSMResult {
   int best_iteration = 0;
@@ -669,7 +684,7 @@ class FullChipLSMOptimizer {
     const litho_exchange_options o = detail::to_abi(options.gradient_options, thr_, alpha_);
     const litho_lsm_options lo{options.dx, options.dy, options.cfl, options.curvature_weight,
                                options.mask_type == LevelSetMaskType::Gray ? 1 : 0, options.render_scale};
-    std::vector<double> h(5 * (size_t)(options.iterations + 1));
+    std::vector<litho_fullchip_row> h((size_t)(options.iterations + 1));
     FullChipLSMResult r;
     detail::check(litho_fullchip_lsm(cache_.plan(), meta_.tiles.data(), (int)meta_.tiles.size(), meta_.height,
                                      meta_.width, b.get(), t.get(), &o, &lo, options.iterations,
@@ -677,8 +692,27 @@ class FullChipLSMOptimizer {
                                      &r.best_loss));
     r.global_mask = mask.download(meta_.height, meta_.width);
     r.global_phi = phi.download(meta_.height, meta_.width);
-    for (int i = 0; i <= options.iterations; ++i)
-      r.history.push_back({i, h[5 * i], h[5 * i + 1], h[5 * i + 2], h[5 * i + 3], h[5 * i + 4]});
+    for (const litho_fullchip_row& x : h) {
+      FullChipLSMIteration it;
+      it.iteration = x.iteration;
+      it.loss = x.loss;
+      it.data_loss = x.data_loss;
+      it.cept_loss = x.cept_loss;
+      it.min_dt = x.min_dt;
+      it.max_dt = x.max_dt;
+      it.pe_contribution = x.pe_contribution;
+      it.penalty_contribution = x.penalty_contribution;
+      it.epe_contribution = x.epe_contribution;
+      it.cept_contribution = x.cept_contribution;
+      it.core.pe_loss = x.core_pe;
+      it.core.bg_penalty_raw = x.core_bg_penalty_raw;
+      it.core.pe_penalty_loss = x.core_pe_penalty;
+      it.core.data_loss = x.core_data_loss;
+      it.total_pe_loss = x.pe_loss_raw;
+      it.total_bg_penalty_raw = x.bg_penalty_raw;
+      it.epe_mean = x.epe_mean;
+      r.history.push_back(it);
+    }
     return r;
   }
 

@@ -258,6 +258,8 @@ class LocalResult:
     cept_loss: float = 0.0
     core_pe_loss: float = 0.0
     core_bg_penalty_raw: float = 0.0
+    core_pe_penalty_loss: float = 0.0
+    core_data_loss: float = 0.0
     raw_mask_gradient: np.ndarray = None
     cept_mask_gradient: np.ndarray = None
     mask_gradient: np.ndarray = None
@@ -304,6 +306,11 @@ def exchange_compute(meta: Meta, cache: Cache, local_masks, global_target, o: Op
             r.core_pe_loss = pe_loss(wafer[c.sl()], lt[c.sl()], pe_win[c.sl()])
             r.core_bg_penalty_raw = background_penalty_loss(wafer[c.sl()], lt[c.sl()],
                                                             o.penalty_threshold, pen_win[c.sl()])
+            # micro_gradient_exchange.cpp:231-243
+            main = (lt[c.sl()] > 0.5).astype(np.float64)
+            r.core_pe_penalty_loss = (o.pe_weight * pe_loss(wafer[c.sl()], lt[c.sl()], main)
+                                      + o.penalty_weight * r.core_bg_penalty_raw)
+            r.core_data_loss = pe_w * r.core_pe_loss + pen_w * r.core_bg_penalty_raw
         r.loss = r.data_loss
         r.mask_gradient = r.raw_mask_gradient.copy()
         res.append(r)

@@ -226,7 +226,7 @@ def test_tile_exchange_matches_oracle(case, capacity):
     for t in range(len(tiles)):
         r = ref.locals[t]
         for name in ("loss", "data_loss", "pe_loss", "penalty_loss", "bg_penalty_raw", "cept_loss", "core_pe_loss",
-                     "core_bg_penalty_raw"):
+                     "core_bg_penalty_raw", "core_pe_penalty_loss", "core_data_loss"):
             a, b = getattr(per[t], name), getattr(r, name)
             assert abs(a - b) <= 1e-5 * max(abs(b), 1e-9), (t, name, a, b)
         e1 = rel_l2(g_raw[t], r.raw_mask_gradient)
@@ -259,6 +259,18 @@ def test_fullchip_ctm_matches_oracle():
     assert err <= 1e-4
     assert np.max(np.abs(host(gm) - ref["global_mask"])) <= 1e-4
     assert np.max(np.abs(host(loc) - np.stack(ref["local_masks"]))) <= 1e-4
+    # the history rows' breakdown (full_chip_ctm.cpp:84-104): data = PE + penalty contributions,
+    # loss = data + weighted CEPT; and they reach history.csv
+    for row in r["rows"]:
+        assert row.data_loss == pytest.approx(row.pe_contribution + row.penalty_contribution, rel=1e-12)
+        assert row.loss == pytest.approx(row.data_loss + row.cept_contribution, rel=1e-12)
+        assert row.pe_loss_raw == pytest.approx(row.pe_contribution, rel=1e-12)     # pe_weight 1
+    import tempfile, csv, os
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "history.csv")
+        la.write_history_csv(path, "CTM", list(r["rows"]))
+        rec = list(csv.DictReader(open(path)))
+        assert len(rec) == iters + 1 and float(rec[-1]["loss"]) == r["rows"][-1].loss
     plan.close()
     ctx.close()
 
@@ -294,6 +306,11 @@ def test_fullchip_lsm_matches_oracle(mask_type):
     assert err <= 1e-4
     assert mism == 0
     assert np.max(np.abs(host(gp) - ref["global_phi"])) <= 1e-4 * np.max(np.abs(ref["global_phi"]))
+    # rows: PEPenalty contributions and the Core metrics the LSM collects (full_chip_lsm.cpp:152-182)
+    for row in r["rows"]:
+        assert row.data_loss == pytest.approx(row.pe_contribution + row.penalty_contribution, rel=1e-12)
+        assert row.loss == pytest.approx(row.data_loss + row.cept_contribution, rel=1e-12)
+        assert row.core_pe > 0.0 and row.core_data_loss > 0.0 and row.core_pe_penalty > 0.0
     plan.close()
     ctx.close()
 
