@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "binarize.hpp"
@@ -60,23 +61,43 @@ __device__ __forceinline__ bool is_peak(const float* __restrict__ g, int h, int 
   return v >= m;
 }
 
-__global__ void k_peak_hist(const float* __restrict__ g, int h, int w, double floor_v, uint32_t* __restrict__ hist) {
+// Adds cnt to hist[key] for every lane with has set, one atomic per distinct
+// key of the wave: plateaus make many pixels peaks with the same value, and
+// per-pixel atomics on one bin would serialise.  Wave-uniform control flow:
+// every lane of the wave must call it.
+__device__ __forceinline__ void wave_hist_add(uint32_t* __restrict__ hist, uint32_t key, uint32_t cnt, bool has) {
+  const int lane = (int)__lane_id();   // the hardware lane (blocks may be 2-D)
+  unsigned long long pending = __ballot(has);
+  while (pending != 0ull) {
+    const int leader = __ffsll(pending) - 1;
+    const uint32_t lkey = (uint32_t)__shfl((int)key, leader, 64);
+    const bool same = has && key == lkey;
+    uint32_t v = same ? cnt : 0u;
+    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+    if (lane == leader) atomicAdd(&hist[lkey], v);
+    has = has && !same;
+    pending = __ballot(has);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_peak_hist(const float* __restrict__ g, int h, int w, double floor_v,
+                                                   uint32_t* __restrict__ hist) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x >= w || y >= h) return;
-  float v;
-  if (is_peak(g, h, w, y, x, floor_v, v)) atomicAdd(&hist[float_key(v) >> 16], 1u);
+  float v = 0.f;
+  const bool peak = x < w && y < h && is_peak(g, h, w, y, x, floor_v, v);
+  wave_hist_add(hist, float_key(v) >> 16, 1u, peak);
 }
 
 // low-16-bit histograms of the peaks whose high 16 bits are bin_a / bin_b
-__global__ void k_peak_hist_low(const float* __restrict__ g, int h, int w, double floor_v, uint32_t bin_a,
-                                uint32_t bin_b, uint32_t* __restrict__ hist_a, uint32_t* __restrict__ hist_b) {
+__global__ void __launch_bounds__(256) k_peak_hist_low(const float* __restrict__ g, int h, int w, double floor_v,
+                                                       uint32_t bin_a, uint32_t bin_b, uint32_t* __restrict__ hist_a,
+                                                       uint32_t* __restrict__ hist_b) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x >= w || y >= h) return;
-  float v;
-  if (!is_peak(g, h, w, y, x, floor_v, v)) return;
+  float v = 0.f;
+  const bool peak = x < w && y < h && is_peak(g, h, w, y, x, floor_v, v);
   const uint32_t k = float_key(v);
-  if ((k >> 16) == bin_a) atomicAdd(&hist_a[k & 0xffffu], 1u);
-  if ((k >> 16) == bin_b) atomicAdd(&hist_b[k & 0xffffu], 1u);
+  wave_hist_add(hist_a, k & 0xffffu, 1u, peak && (k >> 16) == bin_a);
+  wave_hist_add(hist_b, k & 0xffffu, 1u, peak && (k >> 16) == bin_b);
 }
 
 __global__ void k_threshold(const float* __restrict__ g, size_t n, double thr, uint8_t* __restrict__ out) {
@@ -134,20 +155,38 @@ __global__ void k_ccl_union(const uint8_t* __restrict__ m, int h, int w, int* L)
   }
 }
 
-__global__ void k_ccl_area(size_t n, int* L, int* __restrict__ area) {
+// One pointer-jumping pass over the union-find forest, out[i] = in[in[i]]:
+// every pass halves the depth of every chain (the merge phase links roots in
+// whatever order the threads ran, so chains across a large component can be
+// long).  changed is set when some label moved.
+__global__ void k_ccl_jump(size_t n, const int* __restrict__ in, int* __restrict__ out, int* __restrict__ changed) {
+  bool moved = false;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    if (L[i] < 0) continue;
-    const int r = ccl_find(L, (int)i);
-    atomicAdd(&area[r], 1);
+    const int p = in[i];
+    const int q = p >= 0 ? in[p] : p;
+    out[i] = q;
+    moved = moved || q != p;
+  }
+  if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0) atomicOr(changed, 1);
+}
+
+// Pixel areas per root (labels fully compressed: L[i] is the root), one
+// atomic per distinct root of a wave.
+__global__ void __launch_bounds__(256) k_ccl_area(size_t n, const int* __restrict__ L, int* __restrict__ area) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t n_up = (n + stride - 1) / stride * stride;   // whole waves take part in every round
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_up; i += stride) {
+    const int r = i < n ? L[i] : -1;
+    wave_hist_add(reinterpret_cast<uint32_t*>(area), (uint32_t)(r < 0 ? 0 : r), 1u, r >= 0);
   }
 }
 
 // the area filter: keep pixels whose component has area >= thr
-__global__ void k_ccl_keep(size_t n, int* L, const int* __restrict__ area, int thr, uint8_t* __restrict__ out) {
+__global__ void k_ccl_keep(size_t n, const int* __restrict__ L, const int* __restrict__ area, int thr,
+                           uint8_t* __restrict__ out) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    uint8_t k = 0;
-    if (L[i] >= 0) k = area[ccl_find(L, (int)i)] >= thr ? 1 : 0;
-    out[i] = k;
+    const int r = L[i];
+    out[i] = r >= 0 && area[r] >= thr ? 1 : 0;
   }
 }
 
@@ -289,13 +328,29 @@ BinarizeInfo binarize_advanced(Context& ctx, int h, int w, const float* gray, co
   info.threshold = thr;
 
   DevMem initial(n), filtered(n), pre(n), fin(n), keep(spacing > 0 ? n : 0), dist(spacing > 0 ? n : 0);
-  DevMem labels(n * sizeof(int)), area(n * sizeof(int)), half((2 * 256 + 1) * sizeof(int));
+  DevMem labels(n * sizeof(int)), labels2(n * sizeof(int)), area(n * sizeof(int)), half((2 * 256 + 1) * sizeof(int));
+  DevMem changed(sizeof(int));
   auto components = [&](const uint8_t* m, uint8_t* kept) {
     hipLaunchKernelGGL(k_ccl_init, dim3(grid1(n)), dim3(256), 0, s, m, n, labels.as<int>(), area.as<int>());
     hipLaunchKernelGGL(k_ccl_union, grd, blk, 0, s, m, h, w, labels.as<int>());
-    hipLaunchKernelGGL(k_ccl_area, dim3(grid1(n)), dim3(256), 0, s, n, labels.as<int>(), area.as<int>());
-    hipLaunchKernelGGL(k_ccl_keep, dim3(grid1(n)), dim3(256), 0, s, n, labels.as<int>(), area.as<int>(), area_thr,
-                       kept);
+    // compress by pointer jumping until no label moves (at most ~log2 of the
+    // longest chain passes; checked every four passes)
+    int* a = labels.as<int>();
+    int* b = labels2.as<int>();
+    for (int round = 0;; ++round) {
+      if (round > 16) throw Error(5, "binarize: label compression did not converge");
+      LITHO_HIP(hipMemsetAsync(changed.ptr, 0, sizeof(int), s));
+      for (int k = 0; k < 4; ++k) {
+        hipLaunchKernelGGL(k_ccl_jump, dim3(grid1(n)), dim3(256), 0, s, n, a, b, changed.as<int>());
+        std::swap(a, b);
+      }
+      int moved = 0;
+      LITHO_HIP(hipMemcpyAsync(&moved, changed.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+      LITHO_HIP(hipStreamSynchronize(s));
+      if (!moved) break;
+    }
+    hipLaunchKernelGGL(k_ccl_area, dim3(grid1(n)), dim3(256), 0, s, n, a, area.as<int>());
+    hipLaunchKernelGGL(k_ccl_keep, dim3(grid1(n)), dim3(256), 0, s, n, a, area.as<int>(), area_thr, kept);
   };
   hipLaunchKernelGGL(k_threshold, dim3(grid1(n)), dim3(256), 0, s, gray, n, thr, initial.as<uint8_t>());
   components(initial.as<uint8_t>(), filtered.as<uint8_t>());

@@ -11,10 +11,13 @@
 //   reinit_sdf (:290-319)        -> k_chamfer + k_sdf_combine
 // and the RK2 update of source/optimizer/full_chip_lsm.cpp:208-242.
 //
-// The stencils read float32 phi and compute in FP64 (the reference's
-// arithmetic; the WENO epsilon 1e-6 max v^2 + 1e-99 is not representable in
-// FP32).  Edge padding by 3 (twice, as the reference pads the already padded
-// field) reduces to clamping every stencil index into the tile.
+// The stencils read float32 phi and compute in FP32 (the reference's FP64
+// arithmetic on the same float32 values agrees to ~1e-7 relative: first
+// differences of neighbouring values are exact, and the WENO weights are
+// formed on power-of-two-scaled differences, where the reference's epsilon
+// 1e-6 max v^2 + 1e-99 is representable).  Edge padding by 3 (twice, as the
+// reference pads the already padded field) reduces to clamping every stencil
+// index into the tile.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -27,37 +30,38 @@ namespace {
 
 __device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
 
-// 1/x for the WENO weights: hardware reciprocal refined by two Newton-Raphson
-// steps r <- r + r(1 - x r), which square the relative error each time (well
-// below one ulp of double from the estimate's ~2^-26 start).  x lies in
-// [1e-198, ~1e300] here, so the range scaling of a full IEEE division is not
-// needed; the result differs from 1/x by at most an ulp or so.
-__device__ __forceinline__ double rcp_nr(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(fma(-x, r, 1.0), r, r);
-  r = fma(fma(-x, r, 1.0), r, r);
-  return r;
-}
-
-// WENO5-JS combination of five one-sided differences (level_set_utils.cpp:56-111).
-__device__ __forceinline__ double weno5(double v1, double v2, double v3, double v4, double v5) {
-  const double c13 = 1.0 / 3.0, c76 = 7.0 / 6.0, c116 = 11.0 / 6.0, c16 = 1.0 / 6.0, c56 = 5.0 / 6.0;
-  const double cs1 = 13.0 / 12.0, cs2 = 0.25;
-  const double d1 = c13 * v1 - c76 * v2 + c116 * v3;
-  const double d2 = -c16 * v2 + c56 * v3 + c13 * v4;
-  const double d3 = c13 * v3 + c56 * v4 - c16 * v5;
-  const double t1 = v1 - 2.0 * v2 + v3, t2 = v2 - 2.0 * v3 + v4, t3 = v3 - 2.0 * v4 + v5;
-  const double e1 = v1 - 4.0 * v2 + 3.0 * v3, e2 = v2 - v4, e3 = 3.0 * v3 - 4.0 * v4 + v5;
-  const double s1 = cs1 * t1 * t1 + cs2 * e1 * e1;
-  const double s2 = cs1 * t2 * t2 + cs2 * e2 * e2;
-  const double s3 = cs1 * t3 * t3 + cs2 * e3 * e3;
-  const double vmax2 = fmax(fmax(fmax(v1 * v1, v2 * v2), fmax(v3 * v3, v4 * v4)), v5 * v5);
-  const double eps = 1e-6 * vmax2 + 1e-99;
-  // (s + eps)^2 >= 1e-198, and the weights stay below ~1e198
-  const double a1 = 0.1 * rcp_nr((s1 + eps) * (s1 + eps));
-  const double a2 = 0.6 * rcp_nr((s2 + eps) * (s2 + eps));
-  const double a3 = 0.3 * rcp_nr((s3 + eps) * (s3 + eps));
-  return (a1 * d1 + a2 * d2 + a3 * d3) * rcp_nr(a1 + a2 + a3);
+// WENO5-JS combination of five one-sided differences (level_set_utils.cpp:56-111)
+// in FP32.  The nonlinear weights a_i = g_i / (beta_i + eps)^2 with
+// eps = 1e-6 max v^2 (+ 1e-99) are invariant under scaling all v by one
+// factor, up to the 1e-99 term, which only matters when every v is zero or
+// within ~1e-46 of it: the differences are scaled by the power of two that
+// brings max |v| into [0.5, 1) (exact), the weights formed there, and the
+// candidate stencils combined in the original units.  All-zero differences
+// give 0, as every candidate is 0.
+__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5) {
+  const float m = fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fmaxf(fabsf(v3), fabsf(v4))), fabsf(v5));
+  if (!(m > 0.f)) return 0.f;
+  int e;
+  (void)frexpf(m, &e);
+  const float sc = ldexpf(1.f, -e);
+  const float u1 = v1 * sc, u2 = v2 * sc, u3 = v3 * sc, u4 = v4 * sc, u5 = v5 * sc;
+  const float c13 = 1.f / 3.f, c76 = 7.f / 6.f, c116 = 11.f / 6.f, c16 = 1.f / 6.f, c56 = 5.f / 6.f;
+  const float cs1 = 13.f / 12.f, cs2 = 0.25f;
+  const float d1 = c13 * v1 - c76 * v2 + c116 * v3;
+  const float d2 = -c16 * v2 + c56 * v3 + c13 * v4;
+  const float d3 = c13 * v3 + c56 * v4 - c16 * v5;
+  const float t1 = u1 - 2.f * u2 + u3, t2 = u2 - 2.f * u3 + u4, t3 = u3 - 2.f * u4 + u5;
+  const float e1 = u1 - 4.f * u2 + 3.f * u3, e2 = u2 - u4, e3 = 3.f * u3 - 4.f * u4 + u5;
+  const float s1 = cs1 * t1 * t1 + cs2 * e1 * e1;
+  const float s2 = cs1 * t2 * t2 + cs2 * e2 * e2;
+  const float s3 = cs1 * t3 * t3 + cs2 * e3 * e3;
+  const float mu = m * sc;                   // max |u| in [0.5, 1): (beta + eps)^2 >= 6e-14
+  const float eps = 1e-6f * (mu * mu);
+  const float q1 = s1 + eps, q2 = s2 + eps, q3 = s3 + eps;
+  const float a1 = 0.1f * __builtin_amdgcn_rcpf(q1 * q1);
+  const float a2 = 0.6f * __builtin_amdgcn_rcpf(q2 * q2);
+  const float a3 = 0.3f * __builtin_amdgcn_rcpf(q3 * q3);
+  return (a1 * d1 + a2 * d2 + a3 * d3) * __builtin_amdgcn_rcpf(a1 + a2 + a3);
 }
 
 // phi of one Micro at absolute (yy, xx), clamped into the tile (the
@@ -65,8 +69,8 @@ __device__ __forceinline__ double weno5(double v1, double v2, double v3, double 
 struct GlobalPhi {
   const float* t;
   int rows, cols;
-  __device__ __forceinline__ double operator()(int yy, int xx) const {
-    return (double)t[(size_t)clampi(yy, rows - 1) * cols + clampi(xx, cols - 1)];
+  __device__ __forceinline__ float operator()(int yy, int xx) const {
+    return t[(size_t)clampi(yy, rows - 1) * cols + clampi(xx, cols - 1)];
   }
 };
 
@@ -77,19 +81,17 @@ template <int W>
 struct LdsPhi {
   const float* s;
   int y0, x0;
-  __device__ __forceinline__ double operator()(int yy, int xx) const {
-    return (double)s[(yy - y0 + 3) * W + (xx - x0 + 3)];
-  }
+  __device__ __forceinline__ float operator()(int yy, int xx) const { return s[(yy - y0 + 3) * W + (xx - x0 + 3)]; }
 };
 
 // f(o) = phi at offset o in [-3, 3] along the line through the pixel.
 // minus: v_m = (f[j+m-2] - f[j+m-3]) / d, m = 0..4; plus: v_q = (f[j+4-q] - f[j+3-q]) / d.
 template <class F>
-__device__ __forceinline__ void weno_pair(const F& f, double inv_d, double& minus, double& plus) {
-  double f_[7];
+__device__ __forceinline__ void weno_pair(const F& f, float inv_d, float& minus, float& plus) {
+  float f_[7];
 #pragma unroll
   for (int o = 0; o < 7; ++o) f_[o] = f(o - 3);    // f[j-3] .. f[j+3]
-  double d[6];
+  float d[6];
 #pragma unroll
   for (int o = 0; o < 6; ++o) d[o] = (f_[o + 1] - f_[o]) * inv_d;  // d[o] = f[j+o-2] - f[j+o-3]
   minus = weno5(d[0], d[1], d[2], d[3], d[4]);
@@ -97,55 +99,61 @@ __device__ __forceinline__ void weno_pair(const F& f, double inv_d, double& minu
 }
 
 // Godunov |grad phi| for the sign of vn (level_set_utils.cpp:179-214).
-__device__ __forceinline__ double godunov(double vn, double pxm, double pxp, double pym, double pyp) {
-  double gx = 0.0, gy = 0.0;
-  if (vn > 0.0) {
-    const double a = fmax(pxm, 0.0), b = fmin(pxp, 0.0), c = fmax(pym, 0.0), e = fmin(pyp, 0.0);
-    gx = fmax(a * a, b * b);
-    gy = fmax(c * c, e * e);
-  } else if (vn < 0.0) {
-    const double a = fmin(pxm, 0.0), b = fmax(pxp, 0.0), c = fmin(pym, 0.0), e = fmax(pyp, 0.0);
-    gx = fmax(a * a, b * b);
-    gy = fmax(c * c, e * e);
+__device__ __forceinline__ float godunov(float vn, float pxm, float pxp, float pym, float pyp) {
+  float gx = 0.f, gy = 0.f;
+  if (vn > 0.f) {
+    const float a = fmaxf(pxm, 0.f), b = fminf(pxp, 0.f), c = fmaxf(pym, 0.f), e = fminf(pyp, 0.f);
+    gx = fmaxf(a * a, b * b);
+    gy = fmaxf(c * c, e * e);
+  } else if (vn < 0.f) {
+    const float a = fminf(pxm, 0.f), b = fmaxf(pxp, 0.f), c = fminf(pym, 0.f), e = fmaxf(pyp, 0.f);
+    gx = fmaxf(a * a, b * b);
+    gy = fmaxf(c * c, e * e);
   }
-  return sqrt(gx + gy);
+  return sqrtf(gx + gy);
 }
 
 // |grad phi| normal term at (y, x): Vn |grad phi|.
 template <class P>
-__device__ __forceinline__ double ls_normal_term(const P& phi, int y, int x, double vn, double inv_dx, double inv_dy) {
-  if (vn == 0.0) return 0.0;
-  double pxm, pxp, pym, pyp;
+__device__ __forceinline__ float ls_normal_term(const P& phi, int y, int x, float vn, float inv_dx, float inv_dy) {
+  if (vn == 0.f) return 0.f;
+  float pxm, pxp, pym, pyp;
   weno_pair([&](int o) { return phi(y, x + o); }, inv_dx, pxm, pxp);
   weno_pair([&](int o) { return phi(y + o, x); }, inv_dy, pym, pyp);
   return vn * godunov(vn, pxm, pxp, pym, pyp);
 }
 
-// _gradient_1d at index j of a line of length n (n >= 3) given a getter;
-// inv_d2 = 1 / (2 d) (exact for the unit grid spacing of the reference).
+// 2 d times _gradient_1d at index j of a line of length n (n >= 3) given a
+// getter: the raw central / one-sided combination, whose 1 / (2 d) scale the
+// caller applies once at the end.  Differences of neighbouring float32 values
+// within a factor of two of each other are exact in FP32, so the nested
+// differences below stay exact on smooth fields and only the final products
+// round.
+// The one-sided ends -3 g0 + 4 g1 - g2 and 3 g[n-1] - 4 g[n-2] + g[n-3] are
+// regrouped as combinations of neighbour differences, which keeps them exact
+// up to one rounding of the small result.
 template <class G>
-__device__ __forceinline__ double grad1(const G& g, int j, int n, double inv_d2) {
-  if (j == 0) return (-3.0 * g(0) + 4.0 * g(1) - g(2)) * inv_d2;
-  if (j == n - 1) return (3.0 * g(n - 1) - 4.0 * g(n - 2) + g(n - 3)) * inv_d2;
-  return (g(j + 1) - g(j - 1)) * inv_d2;
+__device__ __forceinline__ float grad1u(const G& g, int j, int n) {
+  if (j == 0) return 3.f * (g(1) - g(0)) + (g(1) - g(2));
+  if (j == n - 1) return 3.f * (g(n - 1) - g(n - 2)) - (g(n - 2) - g(n - 3));
+  return g(j + 1) - g(j - 1);
 }
 
 // evolve_kappa at (y, x) (level_set_utils.cpp:257-278).  The one-sided
 // differences at the tile edges keep every access inside the tile and
-// within 3 pixels of (y, x).
+// within 3 pixels of (y, x).  inv_dx2 = 1 / (2 dx), inv_dy2 = 1 / (2 dy).
 template <class P>
-__device__ __forceinline__ double ls_kappa_term(const P& phi, int rows, int cols, int y, int x, double dx, double dy,
-                                                double b) {
-  const double inv_dx2 = 1.0 / (2.0 * dx), inv_dy2 = 1.0 / (2.0 * dy);
-  auto px_at = [&](int yy, int xx) { return grad1([&](int k) { return phi(yy, k); }, xx, cols, inv_dx2); };
-  auto py_at = [&](int yy, int xx) { return grad1([&](int k) { return phi(k, xx); }, yy, rows, inv_dy2); };
-  const double px = px_at(y, x), py = py_at(y, x);
-  const double pxy = grad1([&](int k) { return px_at(k, x); }, y, rows, inv_dy2);
-  const double pxx = grad1([&](int k) { return px_at(y, k); }, x, cols, inv_dx2);
-  const double pyy = grad1([&](int k) { return py_at(k, x); }, y, rows, inv_dy2);
-  const double g2 = px * px + py * py;
-  const double num = pxx * py * py - 2.0 * px * py * pxy + pyy * px * px;
-  return b * (g2 > 2.220446049250313e-16 ? num / g2 : 0.0);
+__device__ __forceinline__ float ls_kappa_term(const P& phi, int rows, int cols, int y, int x, float inv_dx2,
+                                               float inv_dy2, float b) {
+  auto px_at = [&](int yy, int xx) { return grad1u([&](int k) { return phi(yy, k); }, xx, cols); };
+  auto py_at = [&](int yy, int xx) { return grad1u([&](int k) { return phi(k, xx); }, yy, rows); };
+  const float px = px_at(y, x) * inv_dx2, py = py_at(y, x) * inv_dy2;
+  const float pxy = grad1u([&](int k) { return px_at(k, x); }, y, rows) * (inv_dx2 * inv_dy2);
+  const float pxx = grad1u([&](int k) { return px_at(y, k); }, x, cols) * (inv_dx2 * inv_dx2);
+  const float pyy = grad1u([&](int k) { return py_at(k, x); }, y, rows) * (inv_dy2 * inv_dy2);
+  const float g2 = px * px + py * py;
+  const float num = pxx * py * py - 2.f * px * py * pxy + pyy * px * px;
+  return b * (g2 > 2.220446049250313e-16f ? num / g2 : 0.f);
 }
 
 __device__ __forceinline__ bool in_rect(const int* r, int y, int x) {
@@ -164,8 +172,10 @@ k_ls_terms(const float* __restrict__ phi, const float* __restrict__ vn, int rows
   for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < plane; p += (size_t)gridDim.x * blockDim.x) {
     const int y = (int)(p / cols), x = (int)(p - (size_t)y * cols);
     if (delta != nullptr)
-      delta[t * plane + p] = (float)ls_normal_term(tile, y, x, (double)vn[t * plane + p], 1.0 / dx, 1.0 / dy);
-    if (kappa != nullptr) kappa[t * plane + p] = (float)ls_kappa_term(tile, rows, cols, y, x, dx, dy, b);
+      delta[t * plane + p] = ls_normal_term(tile, y, x, vn[t * plane + p], (float)(1.0 / dx), (float)(1.0 / dy));
+    if (kappa != nullptr)
+      kappa[t * plane + p] =
+          ls_kappa_term(tile, rows, cols, y, x, (float)(1.0 / (2.0 * dx)), (float)(1.0 / (2.0 * dy)), (float)b);
   }
 }
 
@@ -242,19 +252,20 @@ k_lsm_stage(const float* __restrict__ phi_in, const float* base, const float* __
   const size_t p = (size_t)y * n + x;
   const LdsPhi<kStageW> phi{sp, y0, x0};
   const int* r = rects + 4 * t;
-  double v;
+  float vf;
   if (in_rect(r, y, x)) {
-    const double vn = (double)grad[t * plane + p];
-    const double normal = ls_kappa_term(phi, n, n, y, x, dx, dy, b) - ls_normal_term(phi, y, x, vn, 1.0 / dx, 1.0 / dy);
-    const double cur = phi(y, x);
+    const float vn = grad[t * plane + p];
+    const float normal = ls_kappa_term(phi, n, n, y, x, (float)(1.0 / (2.0 * dx)), (float)(1.0 / (2.0 * dy)), (float)b) -
+                         ls_normal_term(phi, y, x, vn, (float)(1.0 / dx), (float)(1.0 / dy));
+    const float cur = phi(y, x);
+    const float step = (float)dt[t];
     if (STAGE == 1)
-      v = cur + dt[t] * normal;
+      vf = fmaf(step, normal, cur);
     else
-      v = 0.5 * (double)base[t * plane + p] + 0.5 * (cur + dt[t] * normal);
+      vf = 0.5f * base[t * plane + p] + 0.5f * fmaf(step, normal, cur);
   } else {
-    v = STAGE == 1 ? phi(y, x) : (double)base[t * plane + p];
+    vf = STAGE == 1 ? phi(y, x) : base[t * plane + p];
   }
-  const float vf = (float)v;
   out[t * plane + p] = vf;
   if (mask_out != nullptr) mask_out[t * plane + p] = vf >= 0.f ? 1.f : 0.f;
 }
