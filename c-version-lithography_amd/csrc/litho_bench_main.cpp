@@ -5,7 +5,7 @@
 //
 //   litho_bench eval  N K REPEATS [KFILE]   one CTM grad-eval per repeat (stage timings)
 //   litho_bench ctm   N K ITERS   KFILE [S] S L-BFGS CTM solves (default 3), each timed
-//   litho_bench evalb N K REPEATS BATCH     the same for BATCH equal tiles in one launch
+//   litho_bench evalb N K REPEATS BATCH [KFILE]  the same for BATCH equal tiles in one launch
 //   litho_bench save  N K KFILE             precompute the SOCS kernels and save them
 //   litho_bench copy  MIB                   device-to-device copy of MIB MiB (counter calibration)
 // With KFILE the kernels are read from a file written by `save` (no precompute).
@@ -97,7 +97,8 @@ int main(int argc, char** argv) {
   double wsum = 0.0;
   check(litho_socs_window(n, 4.0, 193.0, 1.35, 0.6, 0.9, &wr, &wc, &y0, &x0, &ns, &wsum), "litho_socs_window");
   std::vector<double> kern((size_t)k * wr * wc * 2), lam(k);
-  const char* kfile = mode == "evalb" ? nullptr : (argc > 5 ? argv[5] : (mode == "save" ? argv[4] : nullptr));
+  const char* kfile = mode == "evalb" ? (argc > 6 ? argv[6] : nullptr)
+                                      : (argc > 5 ? argv[5] : (mode == "save" ? argv[4] : nullptr));
   if (kfile != nullptr && mode != "save") {
     FILE* f = std::fopen(kfile, "rb");
     if (f == nullptr || std::fread(kern.data(), sizeof(double), kern.size(), f) != kern.size() ||

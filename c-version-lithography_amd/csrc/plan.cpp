@@ -106,6 +106,17 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     const bool corr_ok = corr_len_supported(L) && 2 * wr - 1 <= n;
     const bool corr_off = path != nullptr && (std::string(path) == "fused" || std::string(path) == "generic");
     p->corr_len = corr_ok && !corr_off ? L : 0;
+    // the band-limited adjoint: the length-L transforms, or with
+    // LITHO_COLUMN_ADJOINT = mfma a Toeplitz GEMM on the matrix cores.  The
+    // GEMM (bf16 pairs, three products per term) is faster only up to
+    // W_r = 80 (640^2 tiles, W_r = 67: 216 against 238 us per batch of 32;
+    // 896^2: 396 against 332 us; 2048^2 alone: 82 against 62 us), and its
+    // 16-bit operands move the fixed-step GD trajectory of the 512^2 CPU
+    // configuration by 2e-4 at iteration 19, against 1e-7 for the FP32
+    // transforms, so it is not the default.
+    const char* adj = std::getenv("LITHO_COLUMN_ADJOINT");
+    const bool want_mfma = adj != nullptr && std::string(adj) == "mfma";
+    p->adj_mfma = p->corr_len > 0 && corr_adjoint_mfma_supported(wr, k) && want_mfma ? 1 : 0;
     if (p->corr_len > 0) {
       // pass twiddle tables of the register-resident length-L transforms (the inverse applies conjugates)
       const std::vector<float2> twl = corr_twiddles(L);

@@ -127,6 +127,7 @@ struct Plan {
   int row_pruned = 0;                        // window-pruned row kernels in use
   int corr_len = 0;                          // L of the band-limited column path, 0 = fused column kernel
   DevMem twl, acorr, ghat, spart;
+  int adj_mfma = 0;                          // band-limited adjoint on the matrix cores (Toeplitz GEMM)
   int tuned_chunk = 0;                       // batch size chosen by autotune_chunk (0 = not tuned)
   int tuned_limit = 0;                       // the limit it was tuned for
   DevMem rt, mhat, t, v, y, p;
@@ -160,6 +161,7 @@ struct Plan {
     a.acorr = acorr.as<float2>();
     a.ghat = ghat.as<float2>();
     a.spart = spart.as<float>();
+    a.adj_mfma = adj_mfma;
     return a;
   }
 };

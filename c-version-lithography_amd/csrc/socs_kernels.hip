@@ -1427,7 +1427,12 @@ struct SocsImpl {
         launch_corr_any(0, a, batch, a.spart, s);
         hipLaunchKernelGGL(k_col_image_corr<N>, dim3(img_blocks(), batch), dim3(ImgCfg<N>::NT), 0, s, a, im);
       }
-      if (part != kColumnImage && im.target != nullptr && a.v != nullptr) launch_corr_any(1, a, batch, nullptr, s);
+      if (part != kColumnImage && im.target != nullptr && a.v != nullptr) {
+        if (a.adj_mfma)
+          launch_corr_adjoint_mfma(a, batch, s);
+        else
+          launch_corr_any(1, a, batch, nullptr, s);
+      }
       return;
     }
     if (part == kColumnAdjoint) return;

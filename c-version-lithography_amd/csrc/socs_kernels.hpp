@@ -41,10 +41,15 @@ struct SocsArgs {
   float2* acorr;                  // [b][wr][n] column autocorrelation c[d], d in [0, wr)
   float2* ghat;                   // [b][wr][n] column spectrum of dL/dI at m in [0, wr)
   float* spart;                   // partial power spectra of the k groups (corr_power_bytes)
+  int adj_mfma = 0;               // band-limited adjoint as a Toeplitz GEMM on the matrix cores (corr_mfma.hip)
 };
 
 // Lengths of the band-limited column path (powers of two).
 bool corr_len_supported(int l);
+// The band-limited adjoint V_k = Toeplitz(ghat) T_k on the matrix cores
+// (corr_mfma.hip): supported for W_r <= 256 and K <= 24.
+bool corr_adjoint_mfma_supported(int wr, int k);
+void launch_corr_adjoint_mfma(const SocsArgs& a, int batch, hipStream_t s);
 // Bytes of the partial power spectra of a batch (the k groups grow as the batch shrinks).
 size_t corr_power_bytes(int n, int l, int k, int batch);
 

@@ -126,6 +126,11 @@ int litho_plan_create(litho_ctx* ctx, int n, int k, int win_rows, int win_cols, 
 int litho_plan_destroy(litho_plan* plan);
 /* Device bytes held by the plan (kernels, tables and workspace). */
 size_t litho_plan_device_bytes(const litho_plan* plan);
+/* Column stage the plan runs (no reference counterpart: the reference has one
+ * FFT path).  corr_len: length L of the band-limited correlations, 0 for the
+ * length-N column transforms; adjoint_mfma: 1 when the band-limited adjoint
+ * runs as a Toeplitz GEMM on the matrix cores, 0 for the length-L transforms. */
+int litho_plan_column_path(const litho_plan* plan, int* corr_len, int* adjoint_mfma);
 
 /* Imaging::compute (imaging.cpp:60-171): aerial = sum_k w_k |E_k|^2 / sum w,
  * wafer = sigmoid(alpha (aerial - threshold)).  aerial / wafer nullable. */

@@ -27,6 +27,9 @@ PATHS = [("pruned", "default"), ("generic", "default"), ("pruned", "fused"), ("p
 def run(gpu_ctx, monkeypatch, s, mask, tgt, rows, cols):
     la = litho_amd()
     n = s.n
+    # the transform adjoint on every path: the comparison is at FP32 rounding
+    # (the matrix-core adjoint has its own test, test_gpu_mfma.py)
+    monkeypatch.setenv("LITHO_COLUMN_ADJOINT", "fft")
     for var, val in (("LITHO_ROW_PATH", rows), ("LITHO_COLUMN_PATH", cols)):
         if val in ("generic", "fused"):
             monkeypatch.setenv(var, val)
@@ -35,6 +38,7 @@ def run(gpu_ctx, monkeypatch, s, mask, tgt, rows, cols):
     plan = la.Plan.from_socs(gpu_ctx, s)
     monkeypatch.delenv("LITHO_ROW_PATH", raising=False)
     monkeypatch.delenv("LITHO_COLUMN_PATH", raising=False)
+    monkeypatch.delenv("LITHO_COLUMN_ADJOINT", raising=False)
     a, w, g = empty(n), empty(n), empty(n)
     plan.forward(dev(mask), a, w, threshold=0.225, alpha=50.0)
     pe, _ = plan.pe_gradient(dev(mask), dev(tgt), grad_mask=g, threshold=0.225, alpha=50.0)
