@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", f"--offload-arch=
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
-SOURCES = ["socs_kernels.hip", "corr_mfma.hip", "opt_kernels.hip", "precompute.hip", "levelset.hip", "tiling.hip", "noalloc.hip", "binarize.hip", "epe.hip", "plan.cpp",
+SOURCES = ["socs_kernels.hip", "corr_mfma.hip", "generic_ops.hip", "opt_kernels.hip", "precompute.hip", "levelset.hip", "tiling.hip", "noalloc.hip", "binarize.hip", "epe.hip", "plan.cpp",
            "optics.cpp", "ctm_driver.cpp", "batch.cpp", "server.cpp", "lsm.cpp", "fullchip.cpp", "io.cpp",
            "litho_capi.cpp"]
 

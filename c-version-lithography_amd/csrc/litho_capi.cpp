@@ -272,9 +272,10 @@ size_t litho_plan_device_bytes(const litho_plan* plan) {
   return plan != nullptr && plan->p != nullptr ? plan->p->device_bytes : 0;
 }
 
-int litho_plan_column_path(const litho_plan* plan, int* corr_len, int* adjoint_mfma) {
+int litho_plan_column_path(const litho_plan* plan, int* direct_dft, int* corr_len, int* adjoint_mfma) {
   return guarded([&] {
     require(plan != nullptr && plan->p != nullptr, "plan is null");
+    if (direct_dft != nullptr) *direct_dft = plan->p->generic;
     if (corr_len != nullptr) *corr_len = plan->p->corr_len;
     if (adjoint_mfma != nullptr) *adjoint_mfma = plan->p->adj_mfma;
   });

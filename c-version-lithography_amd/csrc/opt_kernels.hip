@@ -93,21 +93,38 @@ __device__ __forceinline__ float4 sub4(float4 a, float4 b) { return make_float4(
 __device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b) {
   return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
 }
+// the same on single floats: the vector kernels' element type is float4 when
+// the tile's element count is a multiple of 4 (every even N), float otherwise
+__device__ __forceinline__ double dot4(float a, float b) { return (double)a * (double)b; }
+__device__ __forceinline__ float sub4(float a, float b) { return a - b; }
+__device__ __forceinline__ float sel4(bool c, float a, float b) { return c ? a : b; }
+__device__ __forceinline__ float4 scale4(float c, float4 g) { return make_float4(c * g.x, c * g.y, c * g.z, c * g.w); }
+__device__ __forceinline__ float scale4(float c, float g) { return c * g; }
+// v += cs s + cy y per component, fmaf(cs, s, fmaf(cy, y, v))
+__device__ __forceinline__ void comb4(float4& v, float cs, float4 s, float cy, float4 y) {
+  v.x = fmaf(cs, s.x, fmaf(cy, y.x, v.x));
+  v.y = fmaf(cs, s.y, fmaf(cy, y.y, v.y));
+  v.z = fmaf(cs, s.z, fmaf(cy, y.z, v.z));
+  v.w = fmaf(cs, s.w, fmaf(cy, y.w, v.w));
+}
+__device__ __forceinline__ void comb4(float& v, float cs, float s, float cy, float y) { v = fmaf(cs, s, fmaf(cy, y, v)); }
+template <class V>
+constexpr size_t vec_width() { return sizeof(V) / sizeof(float); }
 
 // L-BFGS pair update.  For tile b: s_new = x - xp, y_new = g - gp written to
 // slot new_slot of the history; FP64 dot products per block, NV = 4 (MB + 1):
 //   [4j+0] s_j . y_new   [4j+1] y_j . y_new   [4j+2] s_j . g   [4j+3] y_j . g
 // for the active slots j < m <= MB (slot index slots[j]), then the new pair at
 // position 4 MB.  With shift set, xp <- x and gp <- g afterwards (the accepted
-// point becomes the base of the next line search).  Vectors are read as
-// float4: n is a multiple of 4 for every supported tile (N^2 with N even).
+// point becomes the base of the next line search).  Vectors are read as V:
+// float4 when n is a multiple of 4 (N^2 with N even), float otherwise.
 //
 // The two halves of a wave take the same 32 elements: lanes 0-31 (h = 0) the
 // s side (s_j and s_new dots, s written, xp shifted), lanes 32-63 (h = 1) the
 // y side.  Both halves load x, xp, g, gp at the same addresses in the same
 // instruction, so HBM reads each element once, while each lane keeps only
 // 2 (MB + 1) FP64 accumulators and has registers for more loads in flight.
-template <int MB>
+template <int MB, class V>
 __global__ void __launch_bounds__(kOptThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* __restrict__ g,
              float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy, const int* __restrict__ need,
@@ -124,32 +141,33 @@ k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* _
   const size_t hist = hist_tile_stride(n);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5;
-  // this half's history (s or y) of tile b, as float4
-  const float4* hv = reinterpret_cast<const float4*>((h ? hy : hs) + (size_t)b * hist);
-  float4* wn = reinterpret_cast<float4*>((h ? hy : hs) + (size_t)b * hist + (size_t)new_slot * hist_slot_stride(n));
+  constexpr size_t VW = vec_width<V>();
+  // this half's history (s or y) of tile b, as V
+  const V* hv = reinterpret_cast<const V*>((h ? hy : hs) + (size_t)b * hist);
+  V* wn = reinterpret_cast<V*>((h ? hy : hs) + (size_t)b * hist + (size_t)new_slot * hist_slot_stride(n));
   size_t off[MB > 0 ? MB : 1];
 #pragma unroll
-  for (int j = 0; j < MB; ++j) off[j] = (size_t)(j < m ? slots[j] : 0) * (hist_slot_stride(n) / 4);
-  const float4* x4 = reinterpret_cast<const float4*>(x + tb);
-  float4* xp4 = reinterpret_cast<float4*>(xp + tb);
-  const float4* g4 = reinterpret_cast<const float4*>(g + tb);
-  float4* gp4 = reinterpret_cast<float4*>(gp + tb);
-  float4* shift4 = h ? gp4 : xp4;
+  for (int j = 0; j < MB; ++j) off[j] = (size_t)(j < m ? slots[j] : 0) * (hist_slot_stride(n) / VW);
+  const V* x4 = reinterpret_cast<const V*>(x + tb);
+  V* xp4 = reinterpret_cast<V*>(xp + tb);
+  const V* g4 = reinterpret_cast<const V*>(g + tb);
+  V* gp4 = reinterpret_cast<V*>(gp + tb);
+  V* shift4 = h ? gp4 : xp4;
   double acc[NA];
 #pragma unroll
   for (int v = 0; v < NA; ++v) acc[v] = 0.0;
-  const size_t n4 = n / 4;
+  const size_t n4 = n / VW;
   const size_t stride = (size_t)gridDim.x * WAVES * 32;
   for (size_t i = ((size_t)blockIdx.x * WAVES + wave) * 32 + (lane & 31); i < n4; i += stride) {
-    const float4 xv = x4[i], xpv = xp4[i], gv = g4[i], gpv = gp4[i];
-    const float4 y = sub4(gv, gpv);
-    const float4 w = sel4(h != 0, y, sub4(xv, xpv));
+    const V xv = x4[i], xpv = xp4[i], gv = g4[i], gpv = gp4[i];
+    const V y = sub4(gv, gpv);
+    const V w = sel4(h != 0, y, sub4(xv, xpv));
     wn[i] = w;
     if (shift) shift4[i] = sel4(h != 0, gv, xv);
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
       if (j < m) {
-        const float4 vj = hv[off[j] + i];
+        const V vj = hv[off[j] + i];
         acc[2 * j + 0] += dot4(vj, y);
         acc[2 * j + 1] += dot4(vj, gv);
       }
@@ -179,7 +197,7 @@ k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* _
 // d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB active slots,
 // plus the FP64 partial of g . d.  coef layout per tile: [0] cg, [1+j] cs_j,
 // [1+kMaxHist+j] cy_j.
-template <int MB>
+template <int MB, class V>
 __global__ void __launch_bounds__(kOptThreads)
 k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, const float* __restrict__ hy,
                   const int* __restrict__ need, const int* __restrict__ meta, const double* __restrict__ coef,
@@ -195,31 +213,28 @@ k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, con
   const float* hy_b = hy + (size_t)b * hist;
   const float cg = (float)cb[0];
   float cs[MB > 0 ? MB : 1], cy[MB > 0 ? MB : 1];
-  const float4* sp[MB > 0 ? MB : 1];
-  const float4* yp[MB > 0 ? MB : 1];
+  const V* sp[MB > 0 ? MB : 1];
+  const V* yp[MB > 0 ? MB : 1];
 #pragma unroll
   for (int j = 0; j < MB; ++j) {
     const int sl = j < m ? slots[j] : 0;
     cs[j] = j < m ? (float)cb[1 + j] : 0.f;
     cy[j] = j < m ? (float)cb[1 + kMaxHist + j] : 0.f;
-    sp[j] = reinterpret_cast<const float4*>(hs_b + (size_t)sl * hist_slot_stride(n));
-    yp[j] = reinterpret_cast<const float4*>(hy_b + (size_t)sl * hist_slot_stride(n));
+    sp[j] = reinterpret_cast<const V*>(hs_b + (size_t)sl * hist_slot_stride(n));
+    yp[j] = reinterpret_cast<const V*>(hy_b + (size_t)sl * hist_slot_stride(n));
   }
-  const float4* g4 = reinterpret_cast<const float4*>(g + tb);
-  float4* d4 = reinterpret_cast<float4*>(d + tb);
+  const V* g4 = reinterpret_cast<const V*>(g + tb);
+  V* d4 = reinterpret_cast<V*>(d + tb);
   double acc[1] = {0.0};
-  const size_t n4 = n / 4;
+  const size_t n4 = n / vec_width<V>();
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
-    const float4 gv = g4[i];
-    float4 v = make_float4(cg * gv.x, cg * gv.y, cg * gv.z, cg * gv.w);
+    const V gv = g4[i];
+    V v = scale4(cg, gv);
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
       if (j < m) {
-        const float4 sj = sp[j][i], yj = yp[j][i];
-        v.x = fmaf(cs[j], sj.x, fmaf(cy[j], yj.x, v.x));
-        v.y = fmaf(cs[j], sj.y, fmaf(cy[j], yj.y, v.y));
-        v.z = fmaf(cs[j], sj.z, fmaf(cy[j], yj.z, v.z));
-        v.w = fmaf(cs[j], sj.w, fmaf(cy[j], yj.w, v.w));
+        const V sj = sp[j][i], yj = yp[j][i];
+        comb4(v, cs[j], sj, cy[j], yj);
       }
     }
     d4[i] = v;
@@ -379,14 +394,22 @@ using DirLaunch = void (*)(dim3, hipStream_t, const float*, const float*, const 
 template <int MB>
 void pair_launch(dim3 grid, hipStream_t s, const float* x, float* xp, const float* g, float* gp, float* hs, float* hy,
                  const int* need, const int* meta, size_t n, int shift, double* partials) {
-  hipLaunchKernelGGL(k_lbfgs_pair<MB>, grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, need, meta, n, shift,
-                     partials);
+  if (n % 4 == 0)
+    hipLaunchKernelGGL((k_lbfgs_pair<MB, float4>), grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, need, meta, n,
+                       shift, partials);
+  else   // odd N: per-float loads (slot bases are not 16-byte aligned)
+    hipLaunchKernelGGL((k_lbfgs_pair<MB, float>), grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, need, meta, n,
+                       shift, partials);
 }
 template <int MB>
 void dir_launch(dim3 grid, hipStream_t s, const float* g, const float* hs, const float* hy, const int* need,
                 const int* meta, const double* coef, size_t n, float* d, double* partials) {
-  hipLaunchKernelGGL(k_lbfgs_direction<MB>, grid, dim3(kOptThreads), 0, s, g, hs, hy, need, meta, coef, n, d,
-                     partials);
+  if (n % 4 == 0)
+    hipLaunchKernelGGL((k_lbfgs_direction<MB, float4>), grid, dim3(kOptThreads), 0, s, g, hs, hy, need, meta, coef, n,
+                       d, partials);
+  else
+    hipLaunchKernelGGL((k_lbfgs_direction<MB, float>), grid, dim3(kOptThreads), 0, s, g, hs, hy, need, meta, coef, n,
+                       d, partials);
 }
 template <int... I>
 constexpr PairLaunch pair_table_entry(int m, std::integer_sequence<int, I...>) {

@@ -23,8 +23,20 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     if (!std::isfinite(weights[i]) || weights[i] < 0.0)
       throw Error(1, "Imaging: SOCS values must be finite and nonnegative");
   if (capacity <= 0) throw Error(1, "plan: batch capacity must be positive");
-  const SocsOps* ops = socs_ops_for(n);
-  if (ops == nullptr) throw Error(4, "plan: no compiled FFT for tile size " + std::to_string(n));
+  // sizes without compiled transforms (or LITHO_SOCS_PATH = generic) run the
+  // direct-DFT pipeline, which needs the band-limited column path
+  const char* gpath = std::getenv("LITHO_SOCS_PATH");
+  const SocsOps* ops = gpath != nullptr && std::string(gpath) == "generic" ? nullptr : socs_ops_for(n);
+  const bool generic = ops == nullptr;
+  if (generic) {
+    int l = 64;
+    while (l < 2 * wr - 1) l *= 2;
+    if (n > kGenericMaxN || !corr_len_supported(l) || 2 * wr - 1 > n)
+      throw Error(4, "plan: tile size " + std::to_string(n) +
+                         " has no compiled FFT, and the direct-DFT path needs N <= " + std::to_string(kGenericMaxN) +
+                         " and 2 W_r - 1 <= min(N, 1023)");
+    ops = generic_socs_ops();
+  }
 
   Plan* p = new Plan();
   try {
@@ -104,8 +116,9 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     int L = 64;
     while (L < 2 * wr - 1) L *= 2;
     const bool corr_ok = corr_len_supported(L) && 2 * wr - 1 <= n;
-    const bool corr_off = path != nullptr && (std::string(path) == "fused" || std::string(path) == "generic");
+    const bool corr_off = !generic && path != nullptr && (std::string(path) == "fused" || std::string(path) == "generic");
     p->corr_len = corr_ok && !corr_off ? L : 0;
+    p->generic = generic ? 1 : 0;
     // the band-limited adjoint: the length-L transforms, or with
     // LITHO_COLUMN_ADJOINT = mfma a Toeplitz GEMM on the matrix cores.  The
     // GEMM (bf16 pairs, three products per term) is faster only up to

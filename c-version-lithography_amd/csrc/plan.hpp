@@ -128,6 +128,7 @@ struct Plan {
   int corr_len = 0;                          // L of the band-limited column path, 0 = fused column kernel
   DevMem twl, acorr, ghat, spart;
   int adj_mfma = 0;                          // band-limited adjoint on the matrix cores (Toeplitz GEMM)
+  int generic = 0;                           // direct-DFT pipeline (no compiled transforms for n)
   int tuned_chunk = 0;                       // batch size chosen by autotune_chunk (0 = not tuned)
   int tuned_limit = 0;                       // the limit it was tuned for
   DevMem rt, mhat, t, v, y, p;

@@ -944,32 +944,6 @@ constexpr int kCorrMaxGroups = 8;   // k groups of the power spectra (partials s
 
 bool corr_len_supported(int l) { return l == 64 || l == 128 || l == 256 || l == 512 || l == 1024; }
 
-// per-pixel resist, losses and dL/dI shared by the column image kernels
-__device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, float iv, double& pe, double& bg) {
-  const float w = 1.0f / (1.0f + expf(-im.alpha * (iv - im.thr)));
-  if (im.aerial != nullptr) im.aerial[gi] = iv;
-  if (im.wafer != nullptr) im.wafer[gi] = w;
-  float g = 0.f;
-  if (im.target != nullptr) {
-    const float tv = im.target[gi];
-    const float r = w - tv;
-    const float pw = im.pe_win != nullptr ? im.pe_win[gi] : 1.0f;
-    float drive = im.pe_weight * r * pw;
-    pe += (double)r * (double)r * (double)pw;
-    if (im.pen_win != nullptr) {
-      const float bw = im.pen_win[gi];
-      const float ex = fmaxf(w - im.pen_thr, 0.f);
-      const float bgm = tv < 0.5f ? 1.f : 0.f;
-      bg += (double)bw * (double)bgm * (double)ex * (double)ex;
-      drive += im.penalty * bgm * ex * bw;
-    }
-    g = drive * w * (1.0f - w);
-    if (im.extra_dldi != nullptr) g += im.extra_dldi[gi];
-    if (im.dldi != nullptr) im.dldi[gi] = g;
-  }
-  return g;
-}
-
 // Loads positions j + r Q, r < 4, of a column's window rows (zero beyond W_r
 // or n).  L >= 2 W_r - 1 gives W_r <= L/2 = 4 Q: positions with r >= 4 are
 // always zero, and the transforms take them as such (RegFFT HALF).
@@ -1262,7 +1236,7 @@ static void launch_corr(int stage, const SocsArgs& a, int batch, float* spart, h
 }
 
 // stage 0: K4a (power spectra and autocorrelation); stage 1: K5
-static void launch_corr_any(int stage, const SocsArgs& a, int batch, float* spart, hipStream_t s) {
+void launch_corr_any(int stage, const SocsArgs& a, int batch, float* spart, hipStream_t s) {
   switch (a.corr_len) {
     case 64: launch_corr<64>(stage, a, batch, spart, s); break;
     case 128: launch_corr<128>(stage, a, batch, spart, s); break;

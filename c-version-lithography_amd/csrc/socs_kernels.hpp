@@ -76,6 +76,32 @@ struct GradArgs {
   double* partials;               // [b][col_blocks][2]: |G_theta|^2, G_theta . dir
 };
 
+// per-pixel resist, losses and dL/dI shared by the column image kernels
+__device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, float iv, double& pe, double& bg) {
+  const float w = 1.0f / (1.0f + expf(-im.alpha * (iv - im.thr)));
+  if (im.aerial != nullptr) im.aerial[gi] = iv;
+  if (im.wafer != nullptr) im.wafer[gi] = w;
+  float g = 0.f;
+  if (im.target != nullptr) {
+    const float tv = im.target[gi];
+    const float r = w - tv;
+    const float pw = im.pe_win != nullptr ? im.pe_win[gi] : 1.0f;
+    float drive = im.pe_weight * r * pw;
+    pe += (double)r * (double)r * (double)pw;
+    if (im.pen_win != nullptr) {
+      const float bw = im.pen_win[gi];
+      const float ex = fmaxf(w - im.pen_thr, 0.f);
+      const float bgm = tv < 0.5f ? 1.f : 0.f;
+      bg += (double)bw * (double)bgm * (double)ex * (double)ex;
+      drive += im.penalty * bgm * ex * bw;
+    }
+    g = drive * w * (1.0f - w);
+    if (im.extra_dldi != nullptr) g += im.extra_dldi[gi];
+    if (im.dldi != nullptr) im.dldi[gi] = g;
+  }
+  return g;
+}
+
 enum ColumnPart { kColumnAll = 0, kColumnImage = 1, kColumnAdjoint = 2 };
 
 struct SocsOps {
@@ -91,6 +117,15 @@ struct SocsOps {
 };
 
 const SocsOps* socs_ops_for(int n);
+// Pipeline for tile sizes without compiled transforms (generic_ops.hip): the
+// same stages with direct window-restricted DFTs; needs the band-limited
+// column path (corr_len > 0) and n <= kGenericMaxN.
+const SocsOps* generic_socs_ops();
+constexpr int kGenericMaxN = 4096;
+// The length-L correlation kernels of the band-limited column path: stage 0
+// the power spectra and autocorrelation (K4a), stage 1 the adjoint window
+// rows (K5, length-L transforms).
+void launch_corr_any(int stage, const SocsArgs& a, int batch, float* spart, hipStream_t s);
 // First radix of the column factorisation of n (0 if n has a single stage).
 int first_radix(int n);
 // Stage twiddle tables of the row transforms of length n (layout of

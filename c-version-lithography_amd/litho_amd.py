@@ -166,7 +166,7 @@ SIGNATURES = {
     "litho_plan_create": (_I, [_P, _I, _I, _I, _I, _I, _I, _DP, _DP, _D, _I, C.POINTER(_P)]),
     "litho_plan_destroy": (_I, [_P]),
     "litho_plan_device_bytes": (C.c_size_t, [_P]),
-    "litho_plan_column_path": (C.c_int, [_P, _IP, _IP]),
+    "litho_plan_column_path": (C.c_int, [_P, _IP, _IP, _IP]),
     "litho_socs_forward": (_I, [_P, _I, _P, _P, _P, _D, _D]),
     "litho_pe_gradient": (_I, [_P, _I, _P, _P, _P, _P, _D, _D, _D, _D, _D, _P, _P, _P, _DP, _DP]),
     "litho_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _P, _P, _DP, _DP]),
@@ -546,10 +546,12 @@ class Plan:
         return int(self.lib.litho_plan_device_bytes(self.h))
 
     def column_path(self) -> dict:
-        """litho_plan_column_path: dict(corr_len=L or 0, adjoint="mfma" | "fft")."""
-        L, m = C.c_int(), C.c_int()
-        check(self.lib.litho_plan_column_path(self.h, C.byref(L), C.byref(m)))
-        return dict(corr_len=L.value, adjoint="mfma" if m.value else "fft")
+        """litho_plan_column_path: dict(transforms="direct_dft" | "fft", corr_len=L or 0,
+        adjoint="mfma" | "fft")."""
+        g, L, m = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.litho_plan_column_path(self.h, C.byref(g), C.byref(L), C.byref(m)))
+        return dict(transforms="direct_dft" if g.value else "fft", corr_len=L.value,
+                    adjoint="mfma" if m.value else "fft")
 
     def forward(self, mask, aerial=None, wafer=None, threshold=0.25, alpha=50.0, batch=1):
         check(self.lib.litho_socs_forward(self.h, batch, ptr(mask), ptr(aerial), ptr(wafer), threshold, alpha))

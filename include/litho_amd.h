@@ -47,7 +47,9 @@ typedef struct litho_ep_set litho_ep_set;
 /* Message of the last failing call on this thread ("" if none). */
 const char* litho_last_error(void);
 
-/* 1 if tile size n has a compiled device FFT, else 0. */
+/* 1 if tile size n has a compiled device FFT, else 0.  Plans of other sizes
+ * (any N up to 4096 whose window satisfies 2 W_r - 1 <= min(N, 1023)) run the
+ * direct-DFT pipeline instead. */
 int litho_supported_size(int n);
 
 /* Device context: one HIP device and one stream (default priority). */
@@ -126,11 +128,13 @@ int litho_plan_create(litho_ctx* ctx, int n, int k, int win_rows, int win_cols, 
 int litho_plan_destroy(litho_plan* plan);
 /* Device bytes held by the plan (kernels, tables and workspace). */
 size_t litho_plan_device_bytes(const litho_plan* plan);
-/* Column stage the plan runs (no reference counterpart: the reference has one
- * FFT path).  corr_len: length L of the band-limited correlations, 0 for the
- * length-N column transforms; adjoint_mfma: 1 when the band-limited adjoint
- * runs as a Toeplitz GEMM on the matrix cores, 0 for the length-L transforms. */
-int litho_plan_column_path(const litho_plan* plan, int* corr_len, int* adjoint_mfma);
+/* Transforms the plan runs (no reference counterpart: the reference has one
+ * FFTW path).  direct_dft: 1 for the direct-DFT pipeline of tile sizes without
+ * compiled transforms, 0 for the compiled FFTs; corr_len: length L of the
+ * band-limited column correlations, 0 for the length-N column transforms;
+ * adjoint_mfma: 1 when the band-limited adjoint runs as a Toeplitz GEMM on the
+ * matrix cores, 0 for the length-L transforms.  Pointers nullable. */
+int litho_plan_column_path(const litho_plan* plan, int* direct_dft, int* corr_len, int* adjoint_mfma);
 
 /* Imaging::compute (imaging.cpp:60-171): aerial = sum_k w_k |E_k|^2 / sum w,
  * wafer = sigmoid(alpha (aerial - threshold)).  aerial / wafer nullable. */

@@ -11,7 +11,8 @@
 //  * the device never stores the K electric fields: Gradient takes the mask
 //    the image came from instead of Imaging::get_electric_field();
 //  * LithoPrepare takes the optics as numbers (OpticsSpec) and K explicitly;
-//  * tile sizes must have a compiled device FFT (even N, litho_supported_size).
+//  * tile sizes without a compiled device FFT (litho_supported_size) run the
+//    direct-DFT pipeline, up to N = 4096.
 // Invalid arguments raise std::invalid_argument, other failures
 // std::runtime_error, with the library's message.
 #pragma once
