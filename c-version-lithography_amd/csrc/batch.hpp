@@ -22,10 +22,11 @@ struct BatchTile {
   float* out_theta = nullptr;    // device N x N, nullable
 };
 
-// Tiles per batched solve for this plan: the smallest batch whose per-tile
-// evaluation time is within 5 % of the best among 1, 2, 4, ... up to `limit`
-// (and the plan capacity).  Times one warm evaluation per candidate; the result
-// is cached in the plan.
+// Tiles per batched solve for this plan: the largest batch whose per-tile
+// evaluation time is within 3 % of the best among 1, 2, 4, ... up to `limit`
+// (and the plan capacity): fewer, fuller rounds for the same tiles.  Times
+// three warm evaluations per candidate; the result is cached in the plan.
+// run_batch tunes its plans one after another before any of them runs.
 int autotune_chunk(Plan& plan, const std::vector<const float*>& targets, int limit, const CtmOptions& opt);
 
 std::vector<CtmTileResult> run_batch(const std::vector<Plan*>& plans, const BatchTile* tiles, int num_tiles,

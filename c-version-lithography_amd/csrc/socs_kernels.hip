@@ -106,9 +106,11 @@ constexpr int col_nt_mid(int lines, int n) {
                               : ((lines * n / 8 + 63) / 64 * 64 > 1024 ? 1024 : (lines * n / 8 + 63) / 64 * 64);
 }
 
-// columns per block of the column kernels for N >= 1024 and for 640 <= N < 1024
-// (measured: 8 x 1024 threads best from 1152 up, 4 x 512 threads for
-// 640..896, 8 x 512 threads below 640 when several tiles share a launch)
+// columns per block of the column kernels for the powers of two from 1024 and
+// for 640 <= N < 1024 (measured alone: 8 x 1024 threads best from 1152 up,
+// 4 x 512 threads for 640..896, 8 x 512 threads below 640 when several tiles
+// share a launch; the sizes from 1024 that are not powers of two take the
+// 4 x 512 blocks, see ColCfg)
 #ifndef LITHO_COL_LINES_LARGE
 #define LITHO_COL_LINES_LARGE 8
 #endif
@@ -125,12 +127,20 @@ constexpr int max_radix(int n) {
   return m;
 }
 
+// The 8-column, 1024-thread blocks (about 80 KB of LDS) are kept for the
+// powers of two from 1024, which run alone (the 2048^2 tile).  The other sizes
+// from 1024 share the GPU with other streams in the batched and served paths,
+// where such a block waits for a CU with that much room free: the 1152^2
+// final column kernel took 3-6 ms per batch-32 launch beside the 640 / 896
+// streams against 0.24 ms alone, and it ran on 4-column, 512-thread blocks
+// instead.
 template <int N>
 struct ColCfg {
   static constexpr bool BIG_ODD = N >= 1024 && max_radix(N) > 16;
+  static constexpr bool WIDE = N >= 1024 && (N & (N - 1)) == 0;
   static constexpr int LINES =
-      BIG_ODD ? 4 : (N >= 1024 ? LITHO_COL_LINES_LARGE : (N >= 640 ? LITHO_COL_LINES_MID : (N >= 256 ? 8 : 16)));
-  static constexpr int NT = BIG_ODD ? 512 : (N >= 1024 ? LITHO_COL_NT_LARGE : (N >= 256 ? col_nt_mid(LINES, N) : 64));
+      BIG_ODD ? 4 : (WIDE ? LITHO_COL_LINES_LARGE : (N >= 640 ? LITHO_COL_LINES_MID : (N >= 256 ? 8 : 16)));
+  static constexpr int NT = BIG_ODD ? 512 : (WIDE ? LITHO_COL_NT_LARGE : (N >= 256 ? col_nt_mid(LINES, N) : 64));
   // swizzle shift: lanes spread over LINES lines; the four-column blocks of the
   // large odd sizes behave like the row kernels
   static constexpr int SH = BIG_ODD ? 5 : 4;
