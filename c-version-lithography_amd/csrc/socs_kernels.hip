@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 #include "fft_columns.hpp"
 #include "fft_engine.hpp"
@@ -82,6 +84,17 @@ struct RowPCfg {
   // 0.050 ms against 0.055 with 128; 640^2 batch 8: 0.040 against 0.044)
   static constexpr int K1NT = N >= 256 ? 256 : NT;
 };
+
+// band-limited image kernel with two columns per complex line
+// (k_col_image_corr2); LITHO_IMAGE_PAIRS = 0 in the environment selects the
+// one-column-per-line kernel (read once per process)
+static bool image_pairs() {
+  static const bool on = [] {
+    const char* v = std::getenv("LITHO_IMAGE_PAIRS");
+    return v == nullptr || std::string(v) != "0";
+  }();
+  return on;
+}
 
 #ifndef LITHO_COL_NT_LARGE
 #define LITHO_COL_NT_LARGE 1024
@@ -1132,6 +1145,134 @@ k_col_image_corr(SocsArgs a, ImageArgs im) {
   }
 }
 
+// K4b with two columns per complex line.  Both transforms of a column are
+// real on one side: I = IDFT_N(c) is real (c Hermitian), g is real.  So the
+// columns x0 + 2l and x0 + 2l + 1 share line l: Z = c_A + i c_B (c[0] taken
+// real) gives IDFT(Z) = I_A + i I_B, the line then holds g_A + i g_B, and
+// Y = DFT(g_A + i g_B) splits into ghat_A[m] = (Y[m] + conj Y[-m]) / 2 and
+// ghat_B[m] = (Y[m] - conj Y[-m]) / (2i).  Half the transforms and half the
+// LDS of k_col_image_corr for the same column block.
+// Used from 768 up except for the powers of two from 1024: image stage per
+// batch of 32, 0.350 against 0.387 ms at 896^2, 0.492 against 0.527 at
+// 1152^2 (batched tiles 1.3 % faster in all).  At 640^2 it is faster per
+// batch of 32 (0.215 against 0.228 ms) but the 256-Micro full-chip update
+// ran 3 % slower with it (99.1-99.8 against 101.9-102.7 updates/s), so 640
+// keeps the one-column kernel.  A lone 2048^2 tile has only 1024 column
+// pairs, two waves per SIMD: it stays latency bound (29-30 us with 4 or 8
+// columns per block against 28 us for k_col_image_corr, with 29 % fewer VALU
+// and 45 % fewer LDS instructions), so it keeps the one-column kernel too.
+template <int N>
+struct ImgPairCfg {
+  using IC = ImgCfg<N>;
+  static constexpr bool USE = IC::LINES % 2 == 0 && N >= 768 && !(N >= 1024 && (N & (N - 1)) == 0);
+  static constexpr int COLS = IC::LINES;                      // columns per block (the image kernel's blocks)
+  static constexpr int PL = COLS / 2;                         // complex lines
+  static constexpr int NT = IC::NT / 2 >= 64 ? IC::NT / 2 : 64;
+  static constexpr int SH = IC::SH;
+  static constexpr int LS = fft::padded_stride<SH>(N);
+  static constexpr int ELEMS = PL * N;
+  static constexpr int PER = (ELEMS + NT - 1) / NT;
+  static constexpr int TW = fft::staged_entries(N);
+  static constexpr size_t LDS_BYTES = (size_t)PL * LS * 8 + (size_t)(TW > 0 ? TW : 1) * 8 + (size_t)NT / 64 * 16;
+  static constexpr int BLOCKS_PER_CU = 160 * 1024 / LDS_BYTES > 0 ? (int)(160 * 1024 / LDS_BYTES) : 1;
+  static constexpr int WAVES = BLOCKS_PER_CU * NT / 256 > 8 ? 8 : (BLOCKS_PER_CU * NT / 256 > 0 ? BLOCKS_PER_CU * NT / 256 : 1);
+};
+
+template <int N>
+__global__ void __launch_bounds__(ImgPairCfg<N>::NT) __attribute__((amdgpu_waves_per_eu(ImgPairCfg<N>::WAVES, 8)))
+k_col_image_corr2(SocsArgs a, ImageArgs im) {
+  using C = ImgPairCfg<N>;
+  constexpr int TW = C::TW;
+  __shared__ float2 buf[C::PL * C::LS];
+  __shared__ double red[2 * C::NT / 64];
+  __shared__ float2 tw[TW > 0 ? TW : 1];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int cb = xcd_column_block(blockIdx.x, gridDim.x);   // XCD-contiguous column blocks
+  const int x0 = cb * C::COLS;
+  for (int e = threadIdx.x; e < TW; e += C::NT) tw[e] = a.rtw[e];
+  const size_t tile = (size_t)b * N * N;
+  const float2* ac = a.acorr + (size_t)b * a.wr * N;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    if (e < C::ELEMS) {
+      const int l = e % C::PL, d = e / C::PL;
+      const int xa = x0 + 2 * l, xb = xa + 1;
+      float2 ca = make_float2(0.f, 0.f), cbv = make_float2(0.f, 0.f);
+      bool conj_half = false;
+      int src = -1;
+      if (d < a.wr) {
+        src = d;
+      } else if (N - d < a.wr) {
+        src = N - d;
+        conj_half = true;
+      }
+      if (src >= 0) {
+        if (xa < N) ca = ac[(size_t)src * N + xa];
+        if (xb < N) cbv = ac[(size_t)src * N + xb];
+        if (src == 0) ca.y = cbv.y = 0.f;   // c[0] is real: keep I_A and I_B apart
+        if (conj_half) {
+          ca.y = -ca.y;
+          cbv.y = -cbv.y;
+        }
+      }
+      // Z = c_A + i c_B
+      buf[l * C::LS + fft::swz<C::SH>(d)] = make_float2(ca.x - cbv.y, ca.y + cbv.x);
+    }
+  }
+  __syncthreads();
+  fft::fft_lines_staged<N, C::PL, C::LS, C::NT, +1, C::SH>(buf, tw);
+  double pe = 0.0, bg = 0.0;
+#pragma unroll
+  for (int s = 0; s < C::PER; ++s) {
+    const int e = threadIdx.x + s * C::NT;
+    const int l = e % C::PL, y = e / C::PL;
+    const int xa = x0 + 2 * l, xb = xa + 1;
+    if (e < C::ELEMS) {
+      float2* q = buf + l * C::LS + fft::swz<C::SH>(y);
+      const float2 iv = *q;
+      const float ga = xa < N ? image_pixel(im, tile + (size_t)y * N + xa, iv.x, pe, bg) : 0.f;
+      const float gb = xb < N ? image_pixel(im, tile + (size_t)y * N + xb, iv.y, pe, bg) : 0.f;
+      *q = make_float2(ga, gb);
+    }
+  }
+  {
+    for (int off = 32; off > 0; off >>= 1) {
+      pe += __shfl_down(pe, off, 64);
+      bg += __shfl_down(bg, off, 64);
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+      red[2 * wave] = pe;
+      red[2 * wave + 1] = bg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && im.partials != nullptr) {
+      double sp = 0.0, sb = 0.0;
+      for (int w = 0; w < C::NT / 64; ++w) {
+        sp += red[2 * w];
+        sb += red[2 * w + 1];
+      }
+      double* out = im.partials + ((size_t)b * gridDim.x + cb) * 2;
+      out[0] = sp;
+      out[1] = sb;
+    }
+  }
+  if (im.target == nullptr || a.v == nullptr) return;
+  // the reduction's barrier orders the (g_A, g_B) writes before the transform
+  fft::fft_lines_staged<N, C::PL, C::LS, C::NT, -1, C::SH>(buf, tw);
+  float2* gh = a.ghat + (size_t)b * a.wr * N;
+  for (int e = threadIdx.x; e < C::PL * a.wr; e += C::NT) {
+    const int l = e % C::PL, m = e / C::PL;
+    const int xa = x0 + 2 * l, xb = xa + 1;
+    const float2 y = buf[l * C::LS + fft::swz<C::SH>(m)];
+    const float2 yc = cconj(buf[l * C::LS + fft::swz<C::SH>(m == 0 ? 0 : N - m)]);
+    if (xa < N) gh[(size_t)m * N + xa] = make_float2(0.5f * (y.x + yc.x), 0.5f * (y.y + yc.y));
+    if (xb < N) gh[(size_t)m * N + xb] = make_float2(0.5f * (y.y - yc.y), -0.5f * (y.x - yc.x));
+  }
+}
+
 // K5: V_k at the window rows from the band-limited ghat
 template <int L>
 __global__ void __launch_bounds__(CorrCfg<L>::NT) __attribute__((amdgpu_waves_per_eu(CorrCfg<L>::WAVES)))
@@ -1355,7 +1496,8 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
 template <int N>
 struct SocsImpl {
   static int col_blocks() { return (N + ColCfg<N>::LINES - 1) / ColCfg<N>::LINES; }
-  static int img_blocks() { return (N + ImgCfg<N>::LINES - 1) / ImgCfg<N>::LINES; }
+  static int img_cols() { return image_pairs() && ImgPairCfg<N>::USE ? ImgPairCfg<N>::COLS : ImgCfg<N>::LINES; }
+  static int img_blocks() { return (N + img_cols() - 1) / img_cols(); }
 
   static void forward_half(const SocsArgs& a, int batch, int mode, const float* base, const float* dir,
                            const float* coef, float* theta_out, float* mask_out, hipStream_t s) {
@@ -1409,7 +1551,10 @@ struct SocsImpl {
     if (a.corr_len > 0) {
       if (part != kColumnAdjoint) {
         launch_corr_any(0, a, batch, a.spart, s);
-        hipLaunchKernelGGL(k_col_image_corr<N>, dim3(img_blocks(), batch), dim3(ImgCfg<N>::NT), 0, s, a, im);
+        if (image_pairs() && ImgPairCfg<N>::USE)
+          hipLaunchKernelGGL(k_col_image_corr2<N>, dim3(img_blocks(), batch), dim3(ImgPairCfg<N>::NT), 0, s, a, im);
+        else
+          hipLaunchKernelGGL(k_col_image_corr<N>, dim3(img_blocks(), batch), dim3(ImgCfg<N>::NT), 0, s, a, im);
       }
       if (part != kColumnImage && im.target != nullptr && a.v != nullptr) {
         if (a.adj_mfma)

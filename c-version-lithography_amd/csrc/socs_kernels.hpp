@@ -77,13 +77,14 @@ struct GradArgs {
 };
 
 // per-pixel resist, losses and dL/dI shared by the column image kernels
-__device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, float iv, double& pe, double& bg) {
+// tv: the target value at gi, loaded by the caller (read only when im.target is set)
+__device__ __forceinline__ float image_pixel_t(const ImageArgs& im, size_t gi, float iv, float tv, double& pe,
+                                               double& bg) {
   const float w = 1.0f / (1.0f + expf(-im.alpha * (iv - im.thr)));
   if (im.aerial != nullptr) im.aerial[gi] = iv;
   if (im.wafer != nullptr) im.wafer[gi] = w;
   float g = 0.f;
   if (im.target != nullptr) {
-    const float tv = im.target[gi];
     const float r = w - tv;
     const float pw = im.pe_win != nullptr ? im.pe_win[gi] : 1.0f;
     float drive = im.pe_weight * r * pw;
@@ -100,6 +101,9 @@ __device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, flo
     if (im.dldi != nullptr) im.dldi[gi] = g;
   }
   return g;
+}
+__device__ __forceinline__ float image_pixel(const ImageArgs& im, size_t gi, float iv, double& pe, double& bg) {
+  return image_pixel_t(im, gi, iv, im.target != nullptr ? im.target[gi] : 0.f, pe, bg);
 }
 
 enum ColumnPart { kColumnAll = 0, kColumnImage = 1, kColumnAdjoint = 2 };
