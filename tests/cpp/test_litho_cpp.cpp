@@ -1,6 +1,6 @@
 // C++ tests of the reference call surface (include/litho_cpp.hpp) on the GPU,
 // following the reference's tests/test_ctm_optimizer.cpp (test_synthetic:
-// 64 x 64 grid, a 19 x 19 square at (23, 23), annular 0.6 / 0.9 source, NA
+// 64 x 64 and 65 x 65 grids, a 19 x 19 square at (23, 23), annular 0.6 / 0.9 source, NA
 // 1.35, 4 nm pixels, threshold 0.25, alpha 85) and the consistency checks of
 // tests/test_micro_gradient_exchange.cpp / test_full_chip_ctm.cpp.
 // Exit status 0 when every check holds; each failure prints its message.
@@ -78,8 +78,16 @@ void check_pe_gradient(const litho::ImagingCache& cache, const litho::Matrix& ta
                        : "PE gradient disagrees with finite differences");
 }
 
+void test_synthetic_grid(int n);
+
 void test_synthetic() {
-  const int n = 64;
+  // even and odd optical grids, as the reference test (65 has no compiled
+  // FFT and runs the direct-DFT pipeline)
+  for (int n : {64, 65}) test_synthetic_grid(n);
+}
+
+void test_synthetic_grid(int n) {
+  std::printf("grid %d:\n", n);
   litho::OpticsSpec optics;   // 4 nm, 193 nm, NA 1.35, annular 0.6 / 0.9
   litho::LithoPrepare prepare(n, optics, 24);
   const litho::ImagingCache& cache = prepare.cache();
