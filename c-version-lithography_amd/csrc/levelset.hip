@@ -366,20 +366,39 @@ constexpr int kChamferThreads = 256;
 constexpr int kChamferMaxCols = 4096;
 constexpr int kChamferPer = kChamferMaxCols / kChamferThreads;
 
-// Inclusive block scan (minimum) of one value per thread, Hillis-Steele in LDS.
+// Barrier that orders LDS accesses only: __syncthreads() also waits for the
+// wave's outstanding global loads and stores, which would serialise the
+// chamfer's next-row loads and its distance stores with every row.  (Every
+// global location the chamfer reads back is one the same thread wrote.)
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Exclusive block scan (minimum) of one value per thread: forward over the
+// threads before t, reverse over the threads after t.  Shuffles within each
+// wave, then the four wave totals through LDS (two barriers per row instead
+// of the 16 of an LDS Hillis-Steele scan).
 __device__ long long block_min_scan(long long v, long long* sh, bool reverse) {
-  const int t = reverse ? (kChamferThreads - 1 - (int)threadIdx.x) : (int)threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kChamferThreads; off <<= 1) {
-    const long long o = t >= off ? sh[t - off] : kInit * 4;
-    __syncthreads();
-    if (o < v) v = o;
-    sh[t] = v;
-    __syncthreads();
+  constexpr long long kBig = kInit * 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int kWaves = kChamferThreads / 64;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const long long o = reverse ? __shfl_down(v, off, 64) : __shfl_up(v, off, 64);
+    const bool ok = reverse ? lane + off < 64 : lane >= off;
+    if (ok && o < v) v = o;
   }
-  const long long r = t > 0 ? sh[t - 1] : kInit * 4;   // exclusive
-  __syncthreads();
+  if (reverse ? lane == 0 : lane == 63) sh[wave] = v;   // the wave's total
+  // exclusive within the wave: the inclusive value of the neighbouring lane
+  const long long nb = reverse ? __shfl_down(v, 1, 64) : __shfl_up(v, 1, 64);
+  long long r = (reverse ? lane < 63 : lane > 0) ? nb : kBig;
+  lds_barrier();
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w)
+    if (reverse ? w > wave : w < wave) r = min(r, sh[w]);
+  lds_barrier();
   return r;
 }
 }  // namespace
@@ -405,15 +424,25 @@ k_chamfer(const float* __restrict__ phi, int rows, int cols, float level, int in
     return (long long)ring[row % 3][j];
   };
   long long w[kChamferPer];
+  // this thread's phi values of the current row, loaded one row ahead
+  float pv[kChamferPer];
+#pragma unroll
+  for (int e = 0; e < kChamferPer; ++e) pv[e] = j0 + e < j1 ? ph[j0 + e] : 0.f;
   // forward pass, top to bottom
   for (int i = 0; i < rows; ++i) {
+    float cur[kChamferPer];
+#pragma unroll
+    for (int e = 0; e < kChamferPer; ++e) {
+      cur[e] = pv[e];
+      pv[e] = (i + 1 < rows && j0 + e < j1) ? ph[(size_t)(i + 1) * cols + j0 + e] : 0.f;
+    }
     long long run = kInit * 4;
 #pragma unroll
     for (int e = 0; e < kChamferPer; ++e) {
       const int j = j0 + e;
       w[e] = kInit * 4;
       if (j < j1) {
-        const float v = ph[(size_t)i * cols + j];
+        const float v = cur[e];
         const bool inside = inclusive ? v >= level : v > level;
         const bool src = which == 0 ? !inside : inside;
         long long t0 = 0;
@@ -442,17 +471,28 @@ k_chamfer(const float* __restrict__ phi, int rows, int cols, float level, int in
         tmp[(size_t)i * cols + j] = val;
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
-  // backward pass, bottom to top (the ring now carries backward rows)
+  // backward pass, bottom to top (the ring now carries backward rows); the
+  // forward distances of the next row up are loaded one row ahead (this
+  // thread wrote them itself in the forward pass)
+  uint32_t tv[kChamferPer];
+#pragma unroll
+  for (int e = 0; e < kChamferPer; ++e) tv[e] = j0 + e < j1 ? tmp[(size_t)(rows - 1) * cols + j0 + e] : 0u;
   for (int i = rows - 1; i >= 0; --i) {
+    uint32_t tcur[kChamferPer];
+#pragma unroll
+    for (int e = 0; e < kChamferPer; ++e) {
+      tcur[e] = tv[e];
+      tv[e] = (i > 0 && j0 + e < j1) ? tmp[(size_t)(i - 1) * cols + j0 + e] : 0u;
+    }
     long long run = kInit * 4;
 #pragma unroll
     for (int e = 0; e < kChamferPer; ++e) {
       const int j = j0 + e;
       w[e] = kInit * 4;
       if (j < j1) {
-        long long t0 = (long long)tmp[(size_t)i * cols + j];
+        long long t0 = (long long)tcur[e];
         t0 = min(t0, ringv(i + 2, j + 1) + kLong);
         t0 = min(t0, ringv(i + 2, j - 1) + kLong);
         t0 = min(t0, ringv(i + 1, j + 2) + kLong);
@@ -476,7 +516,7 @@ k_chamfer(const float* __restrict__ phi, int rows, int cols, float level, int in
         out[(size_t)i * cols + j] = (float)min(val, kDistMax) * (1.0f / 65536.0f);
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
