@@ -127,6 +127,17 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
+// Barrier that orders the block's LDS accesses only.  __syncthreads() also
+// waits for the wave's outstanding global loads and stores, which pins a
+// kernel's prefetch of the next line (and its stores of the last) to the first
+// exchange of every transform.  For the transforms whose callers never read
+// back, within the kernel, global memory that another thread wrote.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // multiply by e^{DIR * 2 pi i m / R} with compile-time constants
 template <int M, int R, int DIR>
 __device__ __forceinline__ float2 twiddle_ct(float2 a) {

@@ -8,7 +8,11 @@
 // between exchange through LDS.  Every exchange writes one of two buffers,
 // alternating, so it needs a single barrier: a buffer is written again only
 // after the barrier of the exchange in between, which every thread reaches
-// after its reads of that buffer.
+// after its reads of that buffer.  The barriers order LDS only (lds_barrier):
+// the callers (the power spectra, the autocorrelation and the adjoint of the
+// band-limited column path) read only global memory that earlier kernels
+// wrote, so the next line's loads and the last line's stores stay in flight
+// across the exchanges.
 //
 // Lanes run across lines (lane = line + LINES * jj), so the global loads and
 // stores of a wave cover LINES consecutive columns of a row.  gfx950 serves a
@@ -112,7 +116,7 @@ struct RegFFT {
 #pragma unroll
       for (int r = 0; r < 8; ++r) dst[r] = v[r];
     }
-    __syncthreads();
+    lds_barrier();
     middle<1, DIR>(v, lds, tw, line, j, ex);
     {
       constexpr int q = PL.count - 1;
@@ -172,7 +176,7 @@ struct RegFFT {
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[so + r * P + ((r * P) >> SH)] = v[t * R + r];
       }
-      __syncthreads();
+      lds_barrier();
       middle<q + 1, DIR>(v, lds, tw, line, j, ex);
     }
   }
