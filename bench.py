@@ -9,6 +9,11 @@ One step = one full CTM L-BFGS solve; a grad-eval is one objective
 evaluation (line-search trials included).  Inputs are resident in HBM before
 the timed region.  FP32 fields, FP64 loss / norm / L-BFGS scalars.
 
+After the timed solves, a short served-tile run (Poisson requests of 384 /
+640 / 1152 / 2176 px tiles at 70 % of capacity) reports the tile p50 / p99
+latency of continuous batching against the plain one-request-at-a-time path
+(`served_tiles`; `--no-served` skips it).
+
 The CPU baseline is the FP64 numpy/scipy oracle (the reference's full-FFT
 dataflow, 3K+1 N x N FFTs per grad-eval) timed on bounded samples on this
 host: the 2048^2 workload on all cores (its rate is `value`), and the
@@ -164,6 +169,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-served", action="store_true",
+                    help="skip the served-tile latency measurement (tile p99 vs plain)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -272,9 +279,21 @@ def main():
                                            round(measured_traffic(EVAL_TRAFFIC_FILE_ALL, "grad-eval") /
                                                  (st[4] * 1e-3) / HBM_PEAK, 4))},
     }
+    plan.close()
+    if not args.no_served:
+        # the metric's second half, tile p99 latency vs plain: Poisson requests
+        # of 384 / 640 / 1152 / 2176 px tiles (20 GD iterations each) at 70 % of
+        # the server's measured capacity, continuous batching against one
+        # request at a time (bench_tiles.py served, 120 requests)
+        bt = _load("litho_bench_tiles", ROOT / "c-version-lithography_amd" / "bench_tiles.py")
+        sv = bt.served_measure(requests=120, iterations=20, load=0.7)
+        line["served_tiles"] = {k: sv[k] for k in ("capacity_tiles_per_s", "offered_tiles_per_s", "served", "plain",
+                                                   "requests", "sizes", "iterations",
+                                                   "identical_masks_alone_vs_mixed")}
+        print(f"served: p50 {sv['served']['p50_ms']} ms, p99 {sv['served']['p99_ms']} ms vs plain p99 "
+              f"{sv['plain']['p99_ms']} ms", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(dk, target, ctx, tg)
-    plan.close()
     ctx.close()
     print(json.dumps(line), flush=True)
 

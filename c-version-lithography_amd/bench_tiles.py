@@ -131,7 +131,14 @@ def batched(args):
 
 
 def served(args):
+    print(json.dumps(served_measure(args.requests, args.iterations, args.load)), flush=True)
+
+
+def served_measure(requests=160, iterations=20, load=0.7):
+    """The served-tile measurement (see the module docstring); returns the JSON dict."""
     import torch
+    from types import SimpleNamespace
+    args = SimpleNamespace(requests=requests, iterations=iterations, load=load)
     sizes = [256 + 128, 512 + 128, 1024 + 128, 2048 + 128]
     caps = [16, 16, 8, 4]
     ctxs = [la.Context(0) for _ in sizes]     # one worker and stream per tile size
@@ -211,7 +218,7 @@ def served(args):
         p.close()
     for ctx in ctxs:
         ctx.close()
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def fullchip(args):
