@@ -81,10 +81,11 @@ def corr_adjoint_bytes(n, k, w):
 
 
 def pair_kernel_bytes(n, m):
-    """Compulsory HBM bytes of one L-BFGS pair-kernel launch: x, xp, g, gp and
-    the 2m history vectors read, s, y and the shifted xp, gp written (N^2
-    floats each)."""
-    return (2 * m + 8) * 4 * n * n
+    """Compulsory HBM bytes of one L-BFGS pair-kernel launch over a chained
+    history of m pairs: the m + 2 points (the chain's m + 1 iterates and the
+    trial) read from the iterate slots and from the gradient slots, nothing
+    written (N^2 floats each)."""
+    return (2 * m + 4) * 4 * n * n
 
 
 HISTORY = 10

@@ -68,7 +68,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     # C++ caller of the C ABI, used for profiling runs without Python
     btmp = BENCH.with_suffix(".tmp")
     cmd = [HIPCC, "-O2", "-std=c++17", "-x", "hip", str(CSRC / "litho_bench_main.cpp"), "-o", str(btmp),
-           f"--offload-arch={ARCH}", "-L", str(PKG), "-llitho_amd", f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+           f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-command-line-argument", "-L", str(PKG), "-llitho_amd", f"-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath,/opt/rocm/lib"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"litho_bench link failed:\n{p.stdout}\n{p.stderr}")
@@ -76,7 +78,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     # C++ tests of the reference call surface (include/litho_cpp.hpp)
     ttmp = CPP_TEST.with_suffix(".tmp")
     cmd = [HIPCC, "-O2", "-std=c++17", "-x", "hip", str(PKG.parent / "tests" / "cpp" / "test_litho_cpp.cpp"), "-o",
-           str(ttmp), f"--offload-arch={ARCH}", "-I", str(PKG.parent / "include"), "-L", str(PKG), "-llitho_amd",
+           str(ttmp), f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-command-line-argument", "-I", str(PKG.parent / "include"), "-L", str(PKG), "-llitho_amd",
            f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:

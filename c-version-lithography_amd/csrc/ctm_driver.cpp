@@ -127,14 +127,14 @@ struct CtmPool::Impl {
       if (!running[b]) return b;
     return -1;
   }
-  // Copies the job's target into slot b and its initial theta into theta_dst[b].
+  // Copies the job's target into slot b and its initial theta to theta_dst (n2 floats).
   void load_slot(int b, const CtmJob& j, float* theta_dst) {
     if (j.target == nullptr || j.out_mask == nullptr) throw Error(1, "CTM: target and output mask are required");
     copy_tile(tgt.as<float>() + b * n2, j.target, n2, s);
     if (j.theta0 != nullptr)
-      copy_tile(theta_dst + b * n2, j.theta0, n2, s);
+      copy_tile(theta_dst, j.theta0, n2, s);
     else
-      launch_init_theta(j.target, theta_dst + b * n2, n2, s);
+      launch_init_theta(j.target, theta_dst, n2, s);
     job[b] = j;
     res[b] = CtmTileResult{};
     running[b] = 1;
@@ -190,7 +190,7 @@ struct GdPool final : CtmPool::Impl {
   bool admit(const CtmJob& j) override {
     const int b = find_free();
     if (b < 0) return false;
-    load_slot(b, j, x.as<float>());
+    load_slot(b, j, x.as<float>() + b * n2);
     LITHO_HIP(hipMemsetAsync(g.as<float>() + b * n2, 0, n2 * sizeof(float), s));   // x + 0 * g must be x
     min_pe[b] = std::numeric_limits<double>::max();
     first[b] = 1;
@@ -260,9 +260,12 @@ struct GdPool final : CtmPool::Impl {
 
 // ------------------------------------------------------------------ L-BFGS
 struct LbfgsPool final : CtmPool::Impl {
-  DevMem x, xp, g, gp, d, hs, hy, pair_part, dir_part, vec_red;
-  // device-resident history of every slot: [m, spare, slots] ints, S'Y and
-  // Y'Y (kMaxHist^2 doubles each) and the direction coefficients
+  // X, G: the point slots of every tile (iterates and their gradients,
+  // [S][P][N^2], opt_kernels.hpp); d: the direction of every tile
+  DevMem X, G, d, pair_part, dir_part, vec_red;
+  // device-resident history of every slot: meta ints (pairs, base, trial,
+  // pair points), S'Y and Y'Y (kMaxHist^2 doubles each) and the direction
+  // coefficients
   DevMem d_meta, d_sy, d_yy, d_coef;
   HostPinned hdg;
   std::vector<LbfgsTile> st;
@@ -276,17 +279,14 @@ struct LbfgsPool final : CtmPool::Impl {
   bool pair_timed = false;
   const LineSearchParams lp;
   const int m_max;
+  const int P;   // point slots per tile
 
   LbfgsPool(Plan& p, int slots, const CtmOptions& o)
       : Impl(p, slots, o), hdg((size_t)slots * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent), st(slots), pending_dg0(slots, 0),
-        m_max(o.history) {
-    alloc_slots(x, S, n2);
-    alloc_slots(xp, S, n2);
-    alloc_slots(g, S, n2);
-    alloc_slots(gp, S, n2);
+        m_max(o.history), P(lbfgs_points(o.history)) {
+    alloc_slots(X, S * P, n2);
+    alloc_slots(G, S * P, n2);
     alloc_slots(d, S, n2);
-    hs.alloc((size_t)S * hist_tile_stride(n2) * sizeof(float));
-    hy.alloc((size_t)S * hist_tile_stride(n2) * sizeof(float));
     pair_part.alloc((size_t)S * kOptBlocks * kPairDots * sizeof(double));
     dir_part.alloc((size_t)S * kOptBlocks * sizeof(double));
     vec_red.alloc((size_t)S * kPairDots * sizeof(double));
@@ -297,16 +297,20 @@ struct LbfgsPool final : CtmPool::Impl {
     LITHO_HIP(hipMemsetAsync(d_meta.ptr, 0, d_meta.bytes, s));
     LITHO_HIP(hipMemsetAsync(d_sy.ptr, 0, d_sy.bytes, s));
     LITHO_HIP(hipMemsetAsync(d_yy.ptr, 0, d_yy.bytes, s));
-    // every evaluation reads the same buffers: theta = xp + step * d for the
-    // active slots, gradient to g, g . d for the line search
-    // (the masks of the trials are not kept: the result is rebuilt from theta)
-    EvalRequest rq = base_request(nullptr, g.as<float>());
-    rq.base = xp.as<float>();
+    // every evaluation: theta = X[base] + step * d for the active slots into
+    // X[trial], its gradient into G[trial], g . d for the line search; base
+    // and trial are read per tile from meta when the kernels run (the masks
+    // of the trials are not kept: the result is rebuilt from theta)
+    EvalRequest rq = base_request(nullptr, G.as<float>());
+    rq.base = X.as<float>();
     rq.dir = d.as<float>();
     rq.coef = pm.d_step();
-    rq.theta_out = x.as<float>();
-    rq.ga.theta = x.as<float>();
+    rq.theta_out = X.as<float>();
+    rq.ga.theta = X.as<float>();
     rq.ga.dir = d.as<float>();
+    rq.ring = d_meta.as<int>() + kMetaBase;   // [base, trial] at kMetaBase, kMetaBase + 1
+    rq.ring_stride = kMetaStride;
+    rq.ring_points = P;
     graph = std::make_unique<EvalGraph>(plan, S, rq);
     LITHO_HIP(hipEventCreate(&pair_ev[0]));
     LITHO_HIP(hipEventCreate(&pair_ev[1]));
@@ -319,16 +323,16 @@ struct LbfgsPool final : CtmPool::Impl {
   bool admit(const CtmJob& j) override {
     const int b = find_free();
     if (b < 0) return false;
-    load_slot(b, j, xp.as<float>());
-    LITHO_HIP(hipMemsetAsync(d.as<float>() + b * n2, 0, n2 * sizeof(float), s));   // first trial: x = xp + 0 * d
-    LITHO_HIP(hipMemsetAsync(d_meta.as<int>() + b * kMetaStride, 0, 2 * sizeof(int), s));   // m = 0, spare slot 0
+    load_slot(b, j, X.as<float>() + (size_t)b * P * n2);   // point 0, the base
+    LITHO_HIP(hipMemsetAsync(d.as<float>() + b * n2, 0, n2 * sizeof(float), s));   // first trial: theta0 + 0 * d
+    launch_lbfgs_reset(d_meta.as<int>(), b, s);   // no pairs, base point 0, trial point 1
     st[b] = LbfgsTile{};
     pending_dg0[b] = 0;
     return true;
   }
 
-  // xp / gp must already hold the accepted point and its gradient (the pair
-  // kernel shifts them; the first line search copies them)
+  // the base point must already hold the accepted iterate and its gradient
+  // (the history update moves the trial there; the first line search swaps)
   void begin_line_search(int b, double dg0) {
     LbfgsTile& t = st[b];
     t.f0 = t.f;
@@ -400,15 +404,15 @@ struct LbfgsPool final : CtmPool::Impl {
     // timed: launches over a full history (the k_lbfgs_pair<m> instantiation)
     pair_timed = plan.ctx->time_pair && mb_pair == m_max;
     if (pair_timed) LITHO_HIP(hipEventRecord(pair_ev[0], s));
-    launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(), hy.as<float>(),
-                      pm.d_need(), d_meta.as<int>(), n2, S, mb_pair, true, pair_part.as<double>(), s);
+    launch_lbfgs_pair(X.as<float>(), G.as<float>(), P, pm.d_need(), d_meta.as<int>(), n2, S, mb_pair,
+                      pair_part.as<double>(), s);
     if (pair_timed) LITHO_HIP(hipEventRecord(pair_ev[1], s));
     launch_reduce_fixed(pair_part.as<double>(), kOptBlocks, 4 * (mb_pair + 1), vec_red.as<double>(), kPairDots, S,
                         s);
     launch_lbfgs_update(S, pm.d_need(), vec_red.as<double>(), mb_pair, m_max, d_meta.as<int>(), d_sy.as<double>(),
                         d_yy.as<double>(), d_coef.as<double>(), s);
-    launch_lbfgs_direction(g.as<float>(), hs.as<float>(), hy.as<float>(), pm.d_need(), d_meta.as<int>(),
-                           d_coef.as<double>(), n2, d.as<float>(), S, mb_dir, dir_part.as<double>(), s);
+    launch_lbfgs_direction(X.as<float>(), G.as<float>(), P, pm.d_need(), d_meta.as<int>(), d_coef.as<double>(), n2,
+                           d.as<float>(), S, mb_dir, dir_part.as<double>(), s);
     // g . d straight into mapped host memory (read after the evaluation's wait)
     launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, hdg.dev_as<double>(), 1, S, s);
   }
@@ -537,34 +541,28 @@ struct LbfgsPool final : CtmPool::Impl {
       t.a = ls_quad_interp(t.lo_a, t.hi_a, t.lo_f, t.hi_f, t.lo_g);
     }
     if (started.empty()) return;
-    // slots whose first evaluation just ended: d = -g and the first line search
+    // slots whose first evaluation just ended: its point becomes the base,
+    // d = -g and the first line search
     for (int b = 0; b < S; ++b) {
       flags[b] = 0;
       stepv[b] = 0.f;
     }
-    for (int b : started) {
-      flags[b] = 1;
-      stepv[b] = -1.f;
-    }
+    for (int b : started) flags[b] = 1;
     pm.upload_active_step(s);
-    launch_scale(g.as<float>(), d.as<float>(), pm.d_step(), pm.d_active(), n2, S, s);
-    for (int b : started) {
-      copy_tile(xp.as<float>() + b * n2, x.as<float>() + b * n2, n2, s);
-      copy_tile(gp.as<float>() + b * n2, g.as<float>() + b * n2, n2, s);
-      begin_line_search(b, -st[b].gnorm * st[b].gnorm);
-    }
+    launch_lbfgs_start(G.as<float>(), P, pm.d_active(), d_meta.as<int>(), n2, S, d.as<float>(), s);
+    for (int b : started) begin_line_search(b, -st[b].gnorm * st[b].gnorm);
     spin_wait(s);   // the staged flags / steps are rewritten next round
   }
 
   void step() override {
     if (busy() == 0) return;
     evaluate(prepare_directions());
-    // write out the slots that finished: x, or xp after a failed line search
+    // write out the slots that finished: the last trial, or the base after a
+    // failed line search
     for (int b = 0; b < S; ++b) {
       if (!running[b] || st[b].phase != LbfgsTile::DONE) continue;
-      const float* src = (st[b].final_is_xp ? xp.as<float>() : x.as<float>()) + b * n2;
-      if (job[b].out_theta != nullptr) copy_tile(job[b].out_theta, src, n2, s);
-      launch_mask_from_theta(src, job[b].out_mask, n2, s);
+      launch_lbfgs_point_out(X.as<float>(), P, d_meta.as<int>(), b, st[b].final_is_xp ? kMetaBase : kMetaTrial, n2,
+                             job[b].out_theta, job[b].out_mask, s);
       res[b].iterations = std::max(st[b].k, 1);
       res[b].final_pe = st[b].f;
       res[b].final_grad_norm = st[b].gnorm;
@@ -596,7 +594,7 @@ std::vector<std::pair<long long, CtmTileResult>> CtmPool::take_finished() {
 }
 size_t CtmPool::slot_bytes(const Plan& plan, const CtmOptions& opt) {
   const size_t n2 = plan.tile_elems();
-  return (opt.method == 0 ? 5 * n2 : 6 * n2 + 2 * hist_tile_stride(n2)) * sizeof(float);
+  return (opt.method == 0 ? 5 * n2 : (2 * (size_t)lbfgs_points(opt.history) + 2) * n2) * sizeof(float);
 }
 
 std::vector<CtmTileResult> run_ctm(Plan& plan, int batch, const float* targets, const float* theta0,

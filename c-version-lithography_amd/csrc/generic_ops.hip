@@ -53,13 +53,15 @@ __global__ void __launch_bounds__(kGT) g_row_fwd(SocsArgs a, int mode, const flo
   const int n = a.n, y = blockIdx.x, b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
   const size_t row = (size_t)b * n * n + (size_t)y * n;
+  const size_t pt = point_offset(a, b, 1) + (size_t)y * n;                      // trial theta row
+  const size_t pb = mode == 1 ? point_offset(a, b, 0) + (size_t)y * n : row;     // origin row (or the mask)
   const float cf = dir != nullptr ? coef[b] : 0.f;
   for (int x = threadIdx.x; x < n; x += kGT) {
     const size_t g = row + x;
-    float v = base[g], m;
+    float v = base[pb + x], m;
     if (mode == 1) {
       if (dir != nullptr) v = v + cf * dir[g];
-      if (theta_out != nullptr) theta_out[g] = v;
+      if (theta_out != nullptr) theta_out[pt + x] = v;
       m = 0.5f * (1.0f + cosf(v));
     } else {
       m = v;
@@ -270,6 +272,7 @@ __global__ void __launch_bounds__(kGT) g_final_col_inv(SocsArgs a, GradArgs ga) 
   const int n = a.n, b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
   const size_t tile = (size_t)b * n * n;
+  const size_t pt = point_offset(a, b, 1);   // trial theta and its gradient
   const float2* p = a.p + (size_t)b * a.wr * n;
   const int* rect = ga.rect != nullptr ? ga.rect + 4 * b : nullptr;
   const int c = threadIdx.x % kGFinCols, lane = threadIdx.x / kGFinCols;
@@ -293,13 +296,13 @@ __global__ void __launch_bounds__(kGT) g_final_col_inv(SocsArgs a, GradArgs ga) 
         s += (double)pv.x * (double)w.x + (double)pv.y * (double)w.y;   // Re(p conj(w))
         idx = step_mod(idx, y, n);
       }
-      const size_t gi = tile + (size_t)y * n + x;
+      const size_t lg = (size_t)y * n + x, gi = tile + lg;
       const float gmv = ga.scale * (float)s;
       if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gmv;
       if (ga.theta != nullptr) {
-        float gt = -0.5f * gmv * sinf(ga.theta[gi]);
+        float gt = -0.5f * gmv * sinf(ga.theta[pt + lg]);
         if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
-        if (ga.grad_theta != nullptr) ga.grad_theta[gi] = gt;
+        if (ga.grad_theta != nullptr) ga.grad_theta[pt + lg] = gt;
         nrm += (double)gt * (double)gt;
         if (ga.dir != nullptr) dot += (double)gt * (double)ga.dir[gi];
       }

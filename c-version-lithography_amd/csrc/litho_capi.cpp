@@ -538,20 +538,22 @@ int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, doubl
     require(n > 0 && n % 4 == 0, "n must be a positive multiple of 4");
     require(m >= 1 && m <= litho::kMaxHist, "history m out of range");
     hipStream_t s = c.stream;
-    litho::DevMem x, xp, g, gp, hs, hy, need, meta, part;
-    for (litho::DevMem* v : {&x, &xp, &g, &gp}) {
-      v->alloc(n * sizeof(float));
+    const int P = litho::lbfgs_points(m);
+    litho::DevMem X, G, need, meta, part;
+    for (litho::DevMem* v : {&X, &G}) {
+      v->alloc((size_t)P * n * sizeof(float));
       LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
     }
-    for (litho::DevMem* v : {&hs, &hy}) {
-      v->alloc(litho::hist_tile_stride(n) * sizeof(float));
-      LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
-    }
-    // one tile with a full history of m pairs in slots 0 .. m-1, the spare slot m
-    std::vector<int> hm(litho::kMetaStride, 0);
+    // one tile with a full history of m chained pairs (points 0 .. m, pair j
+    // from point j to j + 1), base point m, trial point m + 1
+    std::vector<int> hm(litho::kMetaStride, -1);
     hm[0] = m;
-    hm[1] = m;
-    for (int j = 0; j < m; ++j) hm[2 + j] = j;
+    hm[litho::kMetaBase] = m;
+    hm[litho::kMetaTrial] = m + 1;
+    for (int j = 0; j < m; ++j) {
+      hm[litho::kMetaStart + j] = j;
+      hm[litho::kMetaEnd + j] = j + 1;
+    }
     const int one = 1;
     need.alloc(sizeof(int));
     meta.alloc(hm.size() * sizeof(int));
@@ -559,8 +561,8 @@ int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, doubl
     LITHO_HIP(hipMemcpyAsync(need.ptr, &one, sizeof(int), hipMemcpyHostToDevice, s));
     LITHO_HIP(hipMemcpyAsync(meta.ptr, hm.data(), meta.bytes, hipMemcpyHostToDevice, s));
     auto launch = [&] {
-      litho::launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(),
-                               hy.as<float>(), need.as<int>(), meta.as<int>(), n, 1, m, true, part.as<double>(), s);
+      litho::launch_lbfgs_pair(X.as<float>(), G.as<float>(), P, need.as<int>(), meta.as<int>(), n, 1, m,
+                               part.as<double>(), s);
     };
     launch();   // untimed: loads the code object
     hipEvent_t e0, e1;
@@ -589,12 +591,9 @@ int litho_lbfgs_direction(litho_ctx* ctx, size_t n, int m, int num_points, const
     hipStream_t s = c.stream;
     LITHO_HIP(hipSetDevice(c.device));
     // the buffers of one L-BFGS slot (LbfgsPool in ctm_driver.cpp)
-    litho::DevMem x, xp, g, gp, hs, hy, need, meta, sy, yy, coef, pair_part, dir_part, vec_red, gd;
-    for (litho::DevMem* v : {&x, &xp, &g, &gp}) v->alloc(n * sizeof(float));
-    for (litho::DevMem* v : {&hs, &hy}) {
-      v->alloc(litho::hist_tile_stride(n) * sizeof(float));
-      LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
-    }
+    const int P = litho::lbfgs_points(m);
+    litho::DevMem X, G, need, meta, sy, yy, coef, pair_part, dir_part, vec_red, gd;
+    for (litho::DevMem* v : {&X, &G}) v->alloc((size_t)P * n * sizeof(float));
     need.alloc(sizeof(int));
     meta.alloc(litho::kMetaStride * sizeof(int));
     sy.alloc(litho::kMaxHist * litho::kMaxHist * sizeof(double));
@@ -605,22 +604,27 @@ int litho_lbfgs_direction(litho_ctx* ctx, size_t n, int m, int num_points, const
     vec_red.alloc(litho::kPairDots * sizeof(double));
     gd.alloc(sizeof(double));
     for (litho::DevMem* v : {&meta, &sy, &yy, &coef}) LITHO_HIP(hipMemsetAsync(v->ptr, 0, v->bytes, s));
+    litho::launch_lbfgs_reset(meta.as<int>(), 0, s);
     const int one = 1;
     LITHO_HIP(hipMemcpyAsync(need.ptr, &one, sizeof(int), hipMemcpyHostToDevice, s));
-    auto copy = [&](litho::DevMem& dst, const float* src) {
-      LITHO_HIP(hipMemcpyAsync(dst.ptr, src, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    // point p of the slot <- iterate j
+    auto put = [&](int p, int j) {
+      LITHO_HIP(hipMemcpyAsync(X.as<float>() + (size_t)p * n, x_points + (size_t)j * n, n * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+      LITHO_HIP(hipMemcpyAsync(G.as<float>() + (size_t)p * n, g_points + (size_t)j * n, n * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
     };
+    put(0, 0);   // the base
     int mb = 0;
+    std::vector<int> hm(litho::kMetaStride);
     for (int j = 1; j < num_points; ++j) {
-      // pair j: s = x_j - x_{j-1}, y = g_j - g_{j-1}; the dots of the history
-      // with g_j give the direction coefficients of the newest point
-      copy(x, x_points + (size_t)j * n);
-      copy(xp, x_points + (size_t)(j - 1) * n);
-      copy(g, g_points + (size_t)j * n);
-      copy(gp, g_points + (size_t)(j - 1) * n);
+      // iterate j as the accepted trial: pair (x_j - x_{j-1}, g_j - g_{j-1});
+      // the dots of the history with g_j give the coefficients of its direction
+      LITHO_HIP(hipMemcpyAsync(hm.data(), meta.ptr, meta.bytes, hipMemcpyDeviceToHost, s));
+      LITHO_HIP(hipStreamSynchronize(s));
+      put(hm[litho::kMetaTrial], j);
       mb = std::min(j - 1, m);
-      litho::launch_lbfgs_pair(x.as<float>(), xp.as<float>(), g.as<float>(), gp.as<float>(), hs.as<float>(),
-                               hy.as<float>(), need.as<int>(), meta.as<int>(), n, 1, mb, false,
+      litho::launch_lbfgs_pair(X.as<float>(), G.as<float>(), P, need.as<int>(), meta.as<int>(), n, 1, mb,
                                pair_part.as<double>(), s);
       litho::launch_reduce_fixed(pair_part.as<double>(), litho::kOptBlocks, 4 * (mb + 1), vec_red.as<double>(),
                                  litho::kPairDots, 1, s);
@@ -634,17 +638,16 @@ int litho_lbfgs_direction(litho_ctx* ctx, size_t n, int m, int num_points, const
       mb = -1;
     }
     const int mb_dir = std::min(mb + 1, m);
-    litho::launch_lbfgs_direction(g_points + (size_t)(num_points - 1) * n, hs.as<float>(), hy.as<float>(),
-                                  need.as<int>(), meta.as<int>(), coef.as<double>(), n, d_out, 1, mb_dir,
-                                  dir_part.as<double>(), s);
+    litho::launch_lbfgs_direction(X.as<float>(), G.as<float>(), P, need.as<int>(), meta.as<int>(),
+                                  coef.as<double>(), n, d_out, 1, mb_dir, dir_part.as<double>(), s);
     litho::launch_reduce_fixed(dir_part.as<double>(), litho::kOptBlocks, 1, gd.as<double>(), 1, 1, s);
     double gdv = 0.0;
-    int hm = 0;
+    int kept = 0;
     LITHO_HIP(hipMemcpyAsync(&gdv, gd.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
-    LITHO_HIP(hipMemcpyAsync(&hm, meta.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    LITHO_HIP(hipMemcpyAsync(&kept, meta.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     LITHO_HIP(hipStreamSynchronize(s));
     if (gd_out != nullptr) *gd_out = gdv;
-    if (kept_out != nullptr) *kept_out = hm;
+    if (kept_out != nullptr) *kept_out = kept;
   });
 }
 

@@ -111,82 +111,141 @@ __device__ __forceinline__ void comb4(float& v, float cs, float s, float cy, flo
 template <class V>
 constexpr size_t vec_width() { return sizeof(V) / sizeof(float); }
 
-// L-BFGS pair update.  For tile b: s_new = x - xp, y_new = g - gp written to
-// slot new_slot of the history; FP64 dot products per block, NV = 4 (MB + 1):
+// Whether tile b's held pairs form one chain (start of pair j = end of pair
+// j - 1): then each point is loaded once.  A rejected pair between two held
+// ones breaks the chain; those tiles load both points of every pair (same
+// values, same arithmetic).
+__device__ __forceinline__ bool pairs_chained(const int* mt, int m) {
+  bool chain = true;
+  for (int j = 1; j < m; ++j) chain = chain && mt[kMetaStart + j] == mt[kMetaEnd + j - 1];
+  return chain;
+}
+// Point p of a tile's slots (p < 0: the base, a valid stand-in for the pairs
+// beyond m, whose loads are issued but not used).
+template <class V>
+__device__ __forceinline__ const V* point_ptr(const V* slots, const int* mt, int p, size_t n4) {
+  return slots + (size_t)(p < 0 ? mt[kMetaBase] : p) * n4;
+}
+// Load at a byte offset from a wave-uniform base: the offset stays 32 bits (a
+// point holds at most kGenericMaxN^2 floats), so every load of a sweep shares
+// one offset register and takes its base from scalar registers.
+template <class V>
+__device__ __forceinline__ V ldv(const V* base, unsigned byte_off) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// L-BFGS pair dots.  For tile b the new pair is s = x - xp, y = g - gp with x,
+// g the trial point and xp, gp the base; the held pairs are s_j = X[end_j] -
+// X[start_j], y_j = G[end_j] - G[start_j] (FP32 differences of the stored
+// points, the values the history would hold).  FP64 dot products per block,
+// NV = 4 (MB + 1):
 //   [4j+0] s_j . y_new   [4j+1] y_j . y_new   [4j+2] s_j . g   [4j+3] y_j . g
-// for the active slots j < m <= MB (slot index slots[j]), then the new pair at
-// position 4 MB.  With shift set, xp <- x and gp <- g afterwards (the accepted
-// point becomes the base of the next line search).  Vectors are read as V:
+// for j < m <= MB, then the new pair at position 4 MB.  Vectors are read as V:
 // float4 when n is a multiple of 4 (N^2 with N even), float otherwise.
 //
-// The two halves of a wave take the same 32 elements: lanes 0-31 (h = 0) the
-// s side (s_j and s_new dots, s written, xp shifted), lanes 32-63 (h = 1) the
-// y side.  Both halves load x, xp, g, gp at the same addresses in the same
-// instruction, so HBM reads each element once, while each lane keeps only
-// 2 (MB + 1) FP64 accumulators and has registers for more loads in flight.
+// Element groups of 32 (group q of block k: elements (4k + q) * 32 + l, then
+// strided over the grid) are taken by two waves: waves 0 and 1 the s side
+// (points of X) of groups 0-1 and 2-3, waves 2 and 3 the y side (points of
+// G) of the same groups, lanes 0-31 the first group of a wave and lanes
+// 32-63 the second.  So each lane keeps 2 (MB + 1) FP64 accumulators, the
+// point bases of a wave are uniform, and both sides of a group load x, xp, g,
+// gp at the same addresses (the second load hits in cache).  All the point
+// loads of an element are issued before its arithmetic; pairs beyond m load
+// the base and are not accumulated.  Nothing is written: the trial point
+// becomes the base by its index alone.
+template <int MB, class V, bool CHAIN>
+__device__ __forceinline__ void pair_sweep(const V* __restrict__ hv, const V* __restrict__ x4,
+                                           const V* __restrict__ xp4, const V* __restrict__ g4,
+                                           const V* __restrict__ gp4, const int* mt, int m, int h, unsigned bo0,
+                                           unsigned bstride, unsigned bend, size_t n4, double (&acc)[2 * (MB + 1)]) {
+  if constexpr (CHAIN) {
+    // chain points c_0 = start of pair 0, c_{j+1} = end of pair j
+    const V* pc[MB + 1];
+    pc[0] = point_ptr(hv, mt, m > 0 ? mt[kMetaStart] : -1, n4);
+#pragma unroll
+    for (int j = 0; j < MB; ++j) pc[j + 1] = point_ptr(hv, mt, j < m ? mt[kMetaEnd + j] : -1, n4);
+    for (unsigned bo = bo0; bo < bend; bo += bstride) {
+      const V xv = ldv(x4, bo), xpv = ldv(xp4, bo), gv = ldv(g4, bo), gpv = ldv(gp4, bo);
+      V cv[MB + 1];
+#pragma unroll
+      for (int j = 0; j <= MB; ++j) cv[j] = ldv(pc[j], bo);
+      const V y = sub4(gv, gpv);
+      const V w = h ? y : sub4(xv, xpv);
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j < m) {
+          const V vj = sub4(cv[j + 1], cv[j]);
+          acc[2 * j + 0] += dot4(vj, y);
+          acc[2 * j + 1] += dot4(vj, gv);
+        }
+      }
+      acc[2 * MB + 0] += dot4(w, y);
+      acc[2 * MB + 1] += dot4(w, gv);
+    }
+  } else {
+    for (unsigned bo = bo0; bo < bend; bo += bstride) {
+      const V xv = ldv(x4, bo), xpv = ldv(xp4, bo), gv = ldv(g4, bo), gpv = ldv(gp4, bo);
+      const V y = sub4(gv, gpv);
+      const V w = h ? y : sub4(xv, xpv);
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j < m) {
+          const V vj = sub4(ldv(point_ptr(hv, mt, mt[kMetaEnd + j], n4), bo),
+                            ldv(point_ptr(hv, mt, mt[kMetaStart + j], n4), bo));
+          acc[2 * j + 0] += dot4(vj, y);
+          acc[2 * j + 1] += dot4(vj, gv);
+        }
+      }
+      acc[2 * MB + 0] += dot4(w, y);
+      acc[2 * MB + 1] += dot4(w, gv);
+    }
+  }
+}
+
 template <int MB, class V>
-__global__ void __launch_bounds__(kOptThreads) __attribute__((amdgpu_waves_per_eu(4)))
-k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* __restrict__ g,
-             float* __restrict__ gp, float* __restrict__ hs, float* __restrict__ hy, const int* __restrict__ need,
-             const int* __restrict__ meta, size_t n, int shift, double* __restrict__ partials) {
+__global__ void __launch_bounds__(kOptThreads) __attribute__((amdgpu_waves_per_eu(3)))
+k_lbfgs_pair(const float* __restrict__ X, const float* __restrict__ G, int points, const int* __restrict__ need,
+             const int* __restrict__ meta, size_t n, double* __restrict__ partials) {
   constexpr int NV = 4 * (MB + 1);
   constexpr int NA = 2 * (MB + 1);
   constexpr int WAVES = kOptThreads / 64;
+  static_assert(WAVES == 4, "two element groups per wave, two waves per side");
   const int b = blockIdx.y;
   if (need[b] == 0) return;
-  const int m = meta[b * kMetaStride + 0];
-  const int new_slot = meta[b * kMetaStride + 1];
-  const int* slots = meta + b * kMetaStride + 2;
-  const size_t tb = (size_t)b * n;
-  const size_t hist = hist_tile_stride(n);
+  const int* mt = meta + b * kMetaStride;
+  const int m = mt[0];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int h = lane >> 5;
+  const int h = wave >> 1;                                   // 0: s side, 1: y side
+  const int grp = 2 * (wave & 1) + (lane >> 5);              // element group of this lane
   constexpr size_t VW = vec_width<V>();
-  // this half's history (s or y) of tile b, as V
-  const V* hv = reinterpret_cast<const V*>((h ? hy : hs) + (size_t)b * hist);
-  V* wn = reinterpret_cast<V*>((h ? hy : hs) + (size_t)b * hist + (size_t)new_slot * hist_slot_stride(n));
-  size_t off[MB > 0 ? MB : 1];
-#pragma unroll
-  for (int j = 0; j < MB; ++j) off[j] = (size_t)(j < m ? slots[j] : 0) * (hist_slot_stride(n) / VW);
-  const V* x4 = reinterpret_cast<const V*>(x + tb);
-  V* xp4 = reinterpret_cast<V*>(xp + tb);
-  const V* g4 = reinterpret_cast<const V*>(g + tb);
-  V* gp4 = reinterpret_cast<V*>(gp + tb);
-  V* shift4 = h ? gp4 : xp4;
+  const size_t n4 = n / VW;
+  const V* X4 = reinterpret_cast<const V*>(X + (size_t)b * points * n);
+  const V* G4 = reinterpret_cast<const V*>(G + (size_t)b * points * n);
+  const size_t trial = (size_t)mt[kMetaTrial] * n4, base = (size_t)mt[kMetaBase] * n4;
   double acc[NA];
 #pragma unroll
   for (int v = 0; v < NA; ++v) acc[v] = 0.0;
-  const size_t n4 = n / VW;
-  const size_t stride = (size_t)gridDim.x * WAVES * 32;
-  for (size_t i = ((size_t)blockIdx.x * WAVES + wave) * 32 + (lane & 31); i < n4; i += stride) {
-    const V xv = x4[i], xpv = xp4[i], gv = g4[i], gpv = gp4[i];
-    const V y = sub4(gv, gpv);
-    const V w = sel4(h != 0, y, sub4(xv, xpv));
-    wn[i] = w;
-    if (shift) shift4[i] = sel4(h != 0, gv, xv);
-#pragma unroll
-    for (int j = 0; j < MB; ++j) {
-      if (j < m) {
-        const V vj = hv[off[j] + i];
-        acc[2 * j + 0] += dot4(vj, y);
-        acc[2 * j + 1] += dot4(vj, gv);
-      }
-    }
-    acc[2 * MB + 0] += dot4(w, y);
-    acc[2 * MB + 1] += dot4(w, gv);
-  }
-  // fixed-order reduction: each half over its 32 lanes, then the waves in order
+  const unsigned bo0 = (unsigned)((((size_t)blockIdx.x * WAVES + grp) * 32 + (lane & 31)) * sizeof(V));
+  const unsigned bstride = (unsigned)((size_t)gridDim.x * WAVES * 32 * sizeof(V));
+  const unsigned bend = (unsigned)(n4 * sizeof(V));
+  const V* hv = h ? G4 : X4;                                 // wave-uniform
+  if (pairs_chained(mt, m))
+    pair_sweep<MB, V, true>(hv, X4 + trial, X4 + base, G4 + trial, G4 + base, mt, m, h, bo0, bstride, bend, n4, acc);
+  else
+    pair_sweep<MB, V, false>(hv, X4 + trial, X4 + base, G4 + trial, G4 + base, mt, m, h, bo0, bstride, bend, n4,
+                             acc);
+  // fixed-order reduction: each group over its 32 lanes, then the groups in order
   __shared__ double red[WAVES][2][NA];
 #pragma unroll
   for (int v = 0; v < NA; ++v) {
     double t = acc[v];
     for (int o = 16; o > 0; o >>= 1) t += __shfl_down(t, o, 32);
-    if ((lane & 31) == 0) red[wave][h][v] = t;
+    if ((lane & 31) == 0) red[grp][h][v] = t;
   }
   __syncthreads();
   double* out = partials + ((size_t)b * gridDim.x + blockIdx.x) * NV;
   for (int v = threadIdx.x; v < NV; v += blockDim.x) {
-    // v = 4 j + 2 t + hv: half hv, accumulator 2 j + t (j = MB: the new pair)
+    // v = 4 j + 2 t + hv: side hv, accumulator 2 j + t (j = MB: the new pair)
     const int jj = v >> 2, t = (v >> 1) & 1, hh = v & 1;
     double sum = 0.0;
     for (int w2 = 0; w2 < WAVES; ++w2) sum += red[w2][hh][2 * jj + t];
@@ -194,52 +253,94 @@ k_lbfgs_pair(const float* __restrict__ x, float* __restrict__ xp, const float* _
   }
 }
 
-// d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB active slots,
-// plus the FP64 partial of g . d.  coef layout per tile: [0] cg, [1+j] cs_j,
-// [1+kMaxHist+j] cy_j.
-template <int MB, class V>
-__global__ void __launch_bounds__(kOptThreads)
-k_lbfgs_direction(const float* __restrict__ g, const float* __restrict__ hs, const float* __restrict__ hy,
-                  const int* __restrict__ need, const int* __restrict__ meta, const double* __restrict__ coef,
-                  size_t n, float* __restrict__ d, double* __restrict__ partials) {
-  const int b = blockIdx.y;
-  if (need[b] == 0) return;
-  const int m = meta[b * kMetaStride + 0];
-  const int* slots = meta + b * kMetaStride + 2;
-  const double* cb = coef + (size_t)b * kCoefStride;
-  const size_t tb = (size_t)b * n;
-  const size_t hist = hist_tile_stride(n);
-  const float* hs_b = hs + (size_t)b * hist;
-  const float* hy_b = hy + (size_t)b * hist;
-  const float cg = (float)cb[0];
-  float cs[MB > 0 ? MB : 1], cy[MB > 0 ? MB : 1];
-  const V* sp[MB > 0 ? MB : 1];
-  const V* yp[MB > 0 ? MB : 1];
-#pragma unroll
-  for (int j = 0; j < MB; ++j) {
-    const int sl = j < m ? slots[j] : 0;
-    cs[j] = j < m ? (float)cb[1 + j] : 0.f;
-    cy[j] = j < m ? (float)cb[1 + kMaxHist + j] : 0.f;
-    sp[j] = reinterpret_cast<const V*>(hs_b + (size_t)sl * hist_slot_stride(n));
-    yp[j] = reinterpret_cast<const V*>(hy_b + (size_t)sl * hist_slot_stride(n));
-  }
-  const V* g4 = reinterpret_cast<const V*>(g + tb);
-  V* d4 = reinterpret_cast<V*>(d + tb);
-  double acc[1] = {0.0};
-  const size_t n4 = n / vec_width<V>();
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
-    const V gv = g4[i];
-    V v = scale4(cg, gv);
+// d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB held pairs,
+// g = G[base], plus the FP64 partial of g . d.  coef layout per tile: [0] cg,
+// [1+j] cs_j, [1+kMaxHist+j] cy_j.  All the point loads of an element are
+// issued before its arithmetic (pairs beyond m load the base, unused).
+template <int MB, class V, bool CHAIN>
+__device__ __forceinline__ double direction_sweep(const V* __restrict__ X4, const V* __restrict__ G4,
+                                                  const V* __restrict__ gb, const int* mt, int m, float cg,
+                                                  const float (&cs)[MB > 0 ? MB : 1],
+                                                  const float (&cy)[MB > 0 ? MB : 1], V* __restrict__ d4,
+                                                  unsigned bo0, unsigned bstride, unsigned bend, size_t n4) {
+  double acc = 0.0;
+  if constexpr (CHAIN) {
+    // chain points c_0 = start of pair 0, c_{j+1} = end of pair j
+    const V* cx[MB + 1];
+    const V* cgp[MB + 1];
+    const int p0 = m > 0 ? mt[kMetaStart] : -1;
+    cx[0] = point_ptr(X4, mt, p0, n4);
+    cgp[0] = point_ptr(G4, mt, p0, n4);
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
-      if (j < m) {
-        const V sj = sp[j][i], yj = yp[j][i];
-        comb4(v, cs[j], sj, cy[j], yj);
-      }
+      const int p = j < m ? mt[kMetaEnd + j] : -1;
+      cx[j + 1] = point_ptr(X4, mt, p, n4);
+      cgp[j + 1] = point_ptr(G4, mt, p, n4);
     }
-    d4[i] = v;
-    acc[0] += dot4(gv, v);
+    for (unsigned bo = bo0; bo < bend; bo += bstride) {
+      const V gv = ldv(gb, bo);
+      V xv[MB + 1], gvv[MB + 1];
+#pragma unroll
+      for (int j = 0; j <= MB; ++j) {
+        xv[j] = ldv(cx[j], bo);
+        gvv[j] = ldv(cgp[j], bo);
+      }
+      V v = scale4(cg, gv);
+#pragma unroll
+      for (int j = 0; j < MB; ++j)
+        if (j < m) comb4(v, cs[j], sub4(xv[j + 1], xv[j]), cy[j], sub4(gvv[j + 1], gvv[j]));
+      *reinterpret_cast<V*>(reinterpret_cast<char*>(d4) + bo) = v;
+      acc += dot4(gv, v);
+    }
+  } else {
+    for (unsigned bo = bo0; bo < bend; bo += bstride) {
+      const V gv = ldv(gb, bo);
+      V v = scale4(cg, gv);
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j < m) {
+          const int sp = mt[kMetaStart + j], ep = mt[kMetaEnd + j];
+          const V sj = sub4(ldv(point_ptr(X4, mt, ep, n4), bo), ldv(point_ptr(X4, mt, sp, n4), bo));
+          const V yj = sub4(ldv(point_ptr(G4, mt, ep, n4), bo), ldv(point_ptr(G4, mt, sp, n4), bo));
+          comb4(v, cs[j], sj, cy[j], yj);
+        }
+      }
+      *reinterpret_cast<V*>(reinterpret_cast<char*>(d4) + bo) = v;
+      acc += dot4(gv, v);
+    }
   }
+  return acc;
+}
+
+template <int MB, class V>
+__global__ void __launch_bounds__(kOptThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_lbfgs_direction(const float* __restrict__ X, const float* __restrict__ G, int points, const int* __restrict__ need,
+                  const int* __restrict__ meta, const double* __restrict__ coef, size_t n, float* __restrict__ d,
+                  double* __restrict__ partials) {
+  const int b = blockIdx.y;
+  if (need[b] == 0) return;
+  const int* mt = meta + b * kMetaStride;
+  const int m = mt[0];
+  const double* cb = coef + (size_t)b * kCoefStride;
+  const size_t n4 = n / vec_width<V>();
+  const V* X4 = reinterpret_cast<const V*>(X + (size_t)b * points * n);
+  const V* G4 = reinterpret_cast<const V*>(G + (size_t)b * points * n);
+  const float cg = (float)cb[0];
+  float cs[MB > 0 ? MB : 1], cy[MB > 0 ? MB : 1];
+#pragma unroll
+  for (int j = 0; j < MB; ++j) {
+    cs[j] = j < m ? (float)cb[1 + j] : 0.f;
+    cy[j] = j < m ? (float)cb[1 + kMaxHist + j] : 0.f;
+  }
+  const V* gb = G4 + (size_t)mt[kMetaBase] * n4;
+  V* d4 = reinterpret_cast<V*>(d + (size_t)b * n);
+  const unsigned bo0 = (unsigned)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) * sizeof(V));
+  const unsigned bstride = (unsigned)((size_t)gridDim.x * blockDim.x * sizeof(V));
+  const unsigned bend = (unsigned)(n4 * sizeof(V));
+  double acc[1];
+  acc[0] = pairs_chained(mt, m)
+               ? direction_sweep<MB, V, true>(X4, G4, gb, mt, m, cg, cs, cy, d4, bo0, bstride, bend, n4)
+               : direction_sweep<MB, V, false>(X4, G4, gb, mt, m, cg, cs, cy, d4, bo0, bstride, bend, n4);
   block_reduce_store<1>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x));
 }
 
@@ -303,26 +404,43 @@ k_lbfgs_update(const int* __restrict__ need, const double* __restrict__ dots, in
   __syncthreads();
   if (tid != 0) return;
   int* mt_g = meta + b * kMetaStride;
-  if (accept) {
-    const int freed = drop ? mt[2] : -1;
-    int slots[H];
+  // the held pairs after the update (chronological), the trial point as the
+  // new base, and the next trial: the smallest point that neither a held pair
+  // nor the base uses (at most 2 m_max + 1 points are in use)
+  const int base = mt[kMetaBase], trial = mt[kMetaTrial];
+  int st2[H], en2[H];
 #pragma unroll
-    for (int i = 0; i < H; ++i) slots[i] = (i < m2 - 1) ? mt[2 + i + drop] : (i == m2 - 1 ? mt[1] : -1);
-    int spare = freed;
-    if (spare < 0) {   // the smallest slot index not in the list
-      spare = 0;
-      for (bool used = true; used;) {
-        used = false;
-#pragma unroll
-        for (int i = 0; i < H; ++i) used = used || slots[i] == spare;
-        if (used) ++spare;
+  for (int i = 0; i < H; ++i) {
+    int a0 = -1, e0 = -1;
+    if (accept) {
+      if (i < m2 - 1) {
+        a0 = mt[kMetaStart + i + drop];
+        e0 = mt[kMetaEnd + i + drop];
+      } else if (i == m2 - 1) {
+        a0 = base;
+        e0 = trial;
       }
+    } else if (i < m2) {
+      a0 = mt[kMetaStart + i];
+      e0 = mt[kMetaEnd + i];
     }
+    st2[i] = a0;
+    en2[i] = e0;
+  }
+  int next = 0;
+  for (bool used = true; used;) {
+    used = next == trial;
 #pragma unroll
-    for (int i = 0; i < H; ++i)
-      if (i < m2) mt_g[2 + i] = slots[i];
-    mt_g[0] = m2;
-    mt_g[1] = spare;
+    for (int i = 0; i < H; ++i) used = used || st2[i] == next || en2[i] == next;
+    if (used) ++next;
+  }
+  mt_g[0] = m2;
+  mt_g[kMetaBase] = trial;
+  mt_g[kMetaTrial] = next;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    mt_g[kMetaStart + i] = st2[i];
+    mt_g[kMetaEnd + i] = en2[i];
   }
   double* cf = coef + (size_t)b * kCoefStride;
   for (int i = 0; i < kCoefStride; ++i) cf[i] = 0.0;
@@ -386,30 +504,30 @@ void launch_lbfgs_update(int batch, const int* need, const double* dots, int mb,
 
 namespace {
 
-using PairLaunch = void (*)(dim3, hipStream_t, const float*, float*, const float*, float*, float*, float*, const int*,
-                            const int*, size_t, int, double*);
-using DirLaunch = void (*)(dim3, hipStream_t, const float*, const float*, const float*, const int*, const int*,
-                           const double*, size_t, float*, double*);
+using PairLaunch = void (*)(dim3, hipStream_t, const float*, const float*, int, const int*, const int*, size_t,
+                            double*);
+using DirLaunch = void (*)(dim3, hipStream_t, const float*, const float*, int, const int*, const int*, const double*,
+                           size_t, float*, double*);
 
 template <int MB>
-void pair_launch(dim3 grid, hipStream_t s, const float* x, float* xp, const float* g, float* gp, float* hs, float* hy,
-                 const int* need, const int* meta, size_t n, int shift, double* partials) {
+void pair_launch(dim3 grid, hipStream_t s, const float* X, const float* G, int points, const int* need,
+                 const int* meta, size_t n, double* partials) {
   if (n % 4 == 0)
-    hipLaunchKernelGGL((k_lbfgs_pair<MB, float4>), grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, need, meta, n,
-                       shift, partials);
-  else   // odd N: per-float loads (slot bases are not 16-byte aligned)
-    hipLaunchKernelGGL((k_lbfgs_pair<MB, float>), grid, dim3(kOptThreads), 0, s, x, xp, g, gp, hs, hy, need, meta, n,
-                       shift, partials);
+    hipLaunchKernelGGL((k_lbfgs_pair<MB, float4>), grid, dim3(kOptThreads), 0, s, X, G, points, need, meta, n,
+                       partials);
+  else   // odd N: per-float loads (point bases are not 16-byte aligned)
+    hipLaunchKernelGGL((k_lbfgs_pair<MB, float>), grid, dim3(kOptThreads), 0, s, X, G, points, need, meta, n,
+                       partials);
 }
 template <int MB>
-void dir_launch(dim3 grid, hipStream_t s, const float* g, const float* hs, const float* hy, const int* need,
+void dir_launch(dim3 grid, hipStream_t s, const float* X, const float* G, int points, const int* need,
                 const int* meta, const double* coef, size_t n, float* d, double* partials) {
   if (n % 4 == 0)
-    hipLaunchKernelGGL((k_lbfgs_direction<MB, float4>), grid, dim3(kOptThreads), 0, s, g, hs, hy, need, meta, coef, n,
-                       d, partials);
+    hipLaunchKernelGGL((k_lbfgs_direction<MB, float4>), grid, dim3(kOptThreads), 0, s, X, G, points, need, meta,
+                       coef, n, d, partials);
   else
-    hipLaunchKernelGGL((k_lbfgs_direction<MB, float>), grid, dim3(kOptThreads), 0, s, g, hs, hy, need, meta, coef, n,
-                       d, partials);
+    hipLaunchKernelGGL((k_lbfgs_direction<MB, float>), grid, dim3(kOptThreads), 0, s, X, G, points, need, meta,
+                       coef, n, d, partials);
 }
 template <int... I>
 constexpr PairLaunch pair_table_entry(int m, std::integer_sequence<int, I...>) {
@@ -424,17 +542,74 @@ constexpr DirLaunch dir_table_entry(int m, std::integer_sequence<int, I...>) {
 
 }  // namespace
 
-void launch_lbfgs_pair(const float* x, float* xp, const float* g, float* gp, float* hs, float* hy, const int* need,
-                       const int* meta, size_t n, int batch, int mb, bool shift, double* partials, hipStream_t s) {
-  pair_table_entry(mb, std::make_integer_sequence<int, kMaxHist + 1>{})(
-      dim3(kOptBlocks, batch), s, x, xp, g, gp, hs, hy, need, meta, n, shift ? 1 : 0, partials);
+void launch_lbfgs_pair(const float* X, const float* G, int points, const int* need, const int* meta, size_t n,
+                       int batch, int mb, double* partials, hipStream_t s) {
+  pair_table_entry(mb, std::make_integer_sequence<int, kMaxHist + 1>{})(dim3(kOptBlocks, batch), s, X, G, points,
+                                                                        need, meta, n, partials);
 }
 
-void launch_lbfgs_direction(const float* g, const float* hs, const float* hy, const int* need, const int* meta,
+void launch_lbfgs_direction(const float* X, const float* G, int points, const int* need, const int* meta,
                             const double* coef, size_t n, float* d, int batch, int mb, double* partials,
                             hipStream_t s) {
-  dir_table_entry(mb, std::make_integer_sequence<int, kMaxHist + 1>{})(dim3(kOptBlocks, batch), s, g, hs, hy, need,
-                                                                       meta, coef, n, d, partials);
+  dir_table_entry(mb, std::make_integer_sequence<int, kMaxHist + 1>{})(dim3(kOptBlocks, batch), s, X, G, points,
+                                                                       need, meta, coef, n, d, partials);
+}
+
+__global__ void k_lbfgs_reset(int* __restrict__ meta, int b) {
+  if (threadIdx.x != 0) return;
+  int* mt = meta + b * kMetaStride;
+  mt[0] = 0;
+  mt[kMetaBase] = 0;
+  mt[kMetaTrial] = 1;
+}
+
+void launch_lbfgs_reset(int* meta, int b, hipStream_t s) {
+  hipLaunchKernelGGL(k_lbfgs_reset, dim3(1), dim3(64), 0, s, meta, b);
+}
+
+__global__ void k_lbfgs_start_meta(const int* __restrict__ flags, int* __restrict__ meta, int batch) {
+  const int b = threadIdx.x;
+  if (b >= batch || flags[b] == 0) return;
+  int* mt = meta + b * kMetaStride;
+  const int base = mt[kMetaBase];
+  mt[kMetaBase] = mt[kMetaTrial];
+  mt[kMetaTrial] = base;
+}
+
+// d = -G[base] (the first direction, -1 * g as CTM_Optimizer's steepest start)
+__global__ void k_lbfgs_steepest(const float* __restrict__ G, int points, const int* __restrict__ flags,
+                                 const int* __restrict__ meta, size_t n, float* __restrict__ d) {
+  const int b = blockIdx.y;
+  if (flags[b] == 0) return;
+  const float* gb = G + ((size_t)b * points + meta[b * kMetaStride + kMetaBase]) * n;
+  float* db = d + (size_t)b * n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    db[i] = -1.0f * gb[i];
+}
+
+void launch_lbfgs_start(const float* G, int points, const int* flags, int* meta, size_t n, int batch, float* d,
+                        hipStream_t s) {
+  const int nt = batch <= 64 ? 64 : (batch + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_lbfgs_start_meta, dim3(1), dim3(nt), 0, s, flags, meta, batch);
+  hipLaunchKernelGGL(k_lbfgs_steepest, dim3(kOptBlocks, batch), dim3(kOptThreads), 0, s, G, points, flags,
+                     (const int*)meta, n, d);
+}
+
+__global__ void k_lbfgs_point_out(const float* __restrict__ X, int points, const int* __restrict__ meta, int b,
+                                  int which, size_t n, float* __restrict__ out_theta, float* __restrict__ out_mask) {
+  const float* p = X + ((size_t)b * points + meta[b * kMetaStride + which]) * n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float t = p[i];
+    if (out_theta != nullptr) out_theta[i] = t;
+    out_mask[i] = 0.5f * (1.0f + cosf(t));
+  }
+}
+
+void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, int which, size_t n,
+                            float* out_theta, float* out_mask, hipStream_t s) {
+  const size_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_lbfgs_point_out, dim3(blocks < 4096 ? (blocks > 0 ? blocks : 1) : 4096), dim3(256), 0, s, X,
+                     points, meta, b, which, n, out_theta, out_mask);
 }
 
 // One block per (v, tile): thread t sums blocks t, t + 256, ... in order, then a

@@ -14,12 +14,22 @@ constexpr int kOptThreads = 256;
 constexpr int kOptBlocks = LITHO_OPT_BLOCKS;  // grid-stride blocks per tile of the L-BFGS vector kernels
 constexpr int kMaxHist = 12;             // largest supported L-BFGS history m
 constexpr int kPairDots = 4 * (kMaxHist + 1);
-constexpr int kMetaStride = 2 + kMaxHist;   // [m, new_slot, slots[kMaxHist]]
 constexpr int kCoefStride = 1 + 2 * kMaxHist;
-// Floats between consecutive L-BFGS history slots of one tile, and per tile
-// (kMaxHist + 1 slots: m pairs plus the spare that receives the next one).
-__host__ __device__ constexpr size_t hist_slot_stride(size_t n) { return n; }
-__host__ __device__ constexpr size_t hist_tile_stride(size_t n) { return (size_t)(kMaxHist + 1) * hist_slot_stride(n); }
+// L-BFGS state of a tile: its accepted iterates and their gradients as
+// points, not as differences.  Tile b owns lbfgs_points(m_max) point slots in
+// X and in G (point p of tile b at (b * points + p) * n floats).  Its
+// kMetaStride ints of meta (chronological):
+//   [0] m, the pairs held
+//   [1] base point: the accepted iterate, origin of the line search
+//   [2] trial point: the line search writes its trials (and their gradients) there
+//   [3 + j], [3 + kMaxHist + j]: start and end point of pair j
+// Pair j is s_j = X[end] - X[start], y_j = G[end] - G[start], formed in FP32 by
+// the kernels that use it.  Consecutive pairs share a point unless a pair was
+// rejected in between, so the kept pairs, the base and the trial use at most
+// 2 m_max + 2 points.
+constexpr int kMetaStride = 3 + 2 * kMaxHist;
+constexpr int kMetaBase = 1, kMetaTrial = 2, kMetaStart = 3, kMetaEnd = 3 + kMaxHist;
+__host__ __device__ constexpr int lbfgs_points(int m_max) { return 2 * m_max + 2; }
 
 void launch_init_theta(const float* pattern, float* theta, size_t count, hipStream_t s);
 void launch_scale(const float* x, float* y, const float* coef, const int* active, size_t n, int batch,
@@ -27,23 +37,38 @@ void launch_scale(const float* x, float* y, const float* coef, const int* active
 void launch_axpy(const float* x, float* y, const float* coef, const int* active, size_t n, int batch,
                  hipStream_t s);
 void launch_mask_from_theta(const float* theta, float* mask, size_t count, hipStream_t s);
-// L-BFGS vector kernels over the tiles b with need[b] != 0.  meta holds the
-// device-resident history of each tile ([m, spare slot, slots[kMaxHist]] per
-// kMetaStride ints, chronological); mb >= m is the instantiation (the pair
-// kernel writes the new pair's dots at position 4 mb).  When shift is set the
-// pair kernel also stores xp = x and gp = g after forming the new pair.
-// Vectors are read as float4: n must be a multiple of 4.
-void launch_lbfgs_pair(const float* x, float* xp, const float* g, float* gp, float* hs, float* hy, const int* need,
-                       const int* meta, size_t n, int batch, int mb, bool shift, double* partials, hipStream_t s);
+// L-BFGS kernels over the tiles b with need[b] != 0 (points: point slots per
+// tile, meta as above); mb >= m is the instantiation.  The pair kernel forms
+// the new pair (base -> trial) and its FP64 dot partials with the m held
+// pairs and the trial's gradient, layout per tile and block (NV = 4 (mb + 1)):
+//   [4j+0] s_j . y_new   [4j+1] y_j . y_new   [4j+2] s_j . g   [4j+3] y_j . g
+// for j < m, then the new pair at position 4 mb.  It writes no vectors.
+void launch_lbfgs_pair(const float* X, const float* G, int points, const int* need, const int* meta, size_t n,
+                       int batch, int mb, double* partials, hipStream_t s);
 // History update and compact-form coefficients from the pair kernel's reduced
 // dots (dots[b * kPairDots + ...], layout of the pair kernel at mb): accepts
 // the new pair when s.y > eps y.y, drops the oldest pair beyond m_max, keeps
 // the m x m matrices S'Y (upper triangle) and Y'Y in sy / yy
-// ([b][kMaxHist][kMaxHist] doubles) and writes coef ([b][kCoefStride]).
+// ([b][kMaxHist][kMaxHist] doubles) and writes coef ([b][kCoefStride]).  The
+// trial becomes the base (the accepted iterate, whether or not its pair is
+// kept) and the next trial a point that no kept pair uses.
 void launch_lbfgs_update(int batch, const int* need, const double* dots, int mb, int m_max, int* meta, double* sy,
                          double* yy, double* coef, hipStream_t s);
-void launch_lbfgs_direction(const float* g, const float* hs, const float* hy, const int* need, const int* meta,
+// d = cg G[base] + sum_j cs_j s_j + cy_j y_j over the m <= mb held pairs, and
+// the FP64 partials of G[base] . d ([b][kOptBlocks]).
+void launch_lbfgs_direction(const float* X, const float* G, int points, const int* need, const int* meta,
                             const double* coef, size_t n, float* d, int batch, int mb, double* partials, hipStream_t s);
+// A new tile b: m = 0, base point 0 (which the caller fills with theta0), trial point 1.
+void launch_lbfgs_reset(int* meta, int b, hipStream_t s);
+// Tiles with flags[b] != 0 after their first evaluation (theta0 and its
+// gradient in the trial point): the trial becomes the base, the old base the
+// trial, and d = -G[base] (steepest descent, the first direction).
+void launch_lbfgs_start(const float* G, int points, const int* flags, int* meta, size_t n, int batch, float* d,
+                        hipStream_t s);
+// Point `which` (kMetaBase or kMetaTrial) of tile b: copied to out_theta
+// (optional) and rendered to out_mask = (1 + cos theta) / 2.
+void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, int which, size_t n,
+                            float* out_theta, float* out_mask, hipStream_t s);
 // out[b * out_stride + v] = sum over blk of partials[(b * nblk + blk) * nv + v] for
 // v < nv, summed in a fixed tree order (deterministic, independent of scheduling).
 void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,

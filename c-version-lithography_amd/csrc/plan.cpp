@@ -173,6 +173,9 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   hipStream_t s = plan.ctx->stream;
   SocsArgs a = plan.args();
   a.active = rq.active;
+  a.ring = rq.ring;
+  a.ring_stride = rq.ring_stride;
+  a.ring_points = rq.ring_points;
   plan.ops->forward_half(a, batch, rq.mode, rq.base, rq.dir, rq.coef, rq.theta_out, rq.mask_out, s);
   ImageArgs im = rq.im;
   im.partials = plan.img_partials.as<double>();
@@ -252,6 +255,9 @@ void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, dou
   double acc[5] = {0, 0, 0, 0, 0};
   SocsArgs a = plan.args();
   a.active = rq.active;
+  a.ring = rq.ring;
+  a.ring_stride = rq.ring_stride;
+  a.ring_points = rq.ring_points;
   ImageArgs im = rq.im;
   im.partials = plan.img_partials.as<double>();
   GradArgs ga = rq.ga;

@@ -186,6 +186,10 @@ struct EvalRequest {
   bool adjoint = true;
   const int* active = nullptr;       // device [batch] flags, null = all tiles
   double* red_out = nullptr;         // device-visible [batch][4] reductions (PE, penalty, |g|^2, g.d); null = plan.red
+  // L-BFGS point slots (SocsArgs::ring): base / theta_out / ga.theta /
+  // ga.grad_theta are then the point arrays, indexed per tile by ring
+  const int* ring = nullptr;
+  int ring_stride = 0, ring_points = 0;
 };
 
 void run_eval(Plan& plan, int batch, const EvalRequest& rq);

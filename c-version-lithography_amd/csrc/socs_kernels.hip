@@ -228,6 +228,7 @@ k_row_fwd(SocsArgs a, int mode, const float* base, const float* __restrict__ dir
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
   const size_t tile = (size_t)b * N * N;
+  const size_t pb = point_offset(a, b, 0), pt = point_offset(a, b, 1);   // theta origin and trial
   const int row0 = blockIdx.x * 2 * PAIRS;
   const float cf = (dir != nullptr) ? coef[b] : 0.f;
   for (int e = threadIdx.x; e < PAIRS * N; e += C::NT) {
@@ -238,11 +239,11 @@ k_row_fwd(SocsArgs a, int mode, const float* base, const float* __restrict__ dir
       const int y = row0 + 2 * l + h;
       float m = 0.f;
       if (y < N) {
-        const size_t g = tile + (size_t)y * N + x;
-        float v = base[g];
+        const size_t lg = (size_t)y * N + x, g = tile + lg;
+        float v = base[(mode == 1 ? pb : tile) + lg];
         if (mode == 1) {
           if (dir != nullptr) v = v + cf * dir[g];
-          if (theta_out != nullptr) theta_out[g] = v;
+          if (theta_out != nullptr) theta_out[pt + lg] = v;
           m = 0.5f * (1.0f + cosf(v));
         } else {
           m = v;
@@ -570,6 +571,8 @@ k_row_fwd_pruned(SocsArgs a, int mode, const float* base, const float* __restric
   const int b = blockIdx.y;
   if (a.active != nullptr && a.active[b] == 0) return;
   const size_t tile = (size_t)b * N * N;
+  const size_t pt = point_offset(a, b, 1);                            // trial theta
+  const size_t pb = mode == 1 ? point_offset(a, b, 0) : tile;         // theta origin (or the mask)
   const int row0 = blockIdx.x * 2 * C::LINES;
   const float cf = (dir != nullptr) ? coef[b] : 0.f;
   const bool use_dir = mode == 1 && dir != nullptr;
@@ -578,13 +581,13 @@ k_row_fwd_pruned(SocsArgs a, int mode, const float* base, const float* __restric
       [&](int l, int x) {   // base and direction of the two rows, loads only
         float4 raw = make_float4(0.f, 0.f, 0.f, 0.f);
         const int y = row0 + 2 * l;
-        const size_t g = tile + (size_t)y * N + x;
+        const size_t lg = (size_t)y * N + x, g = tile + lg;
         if (y < N) {
-          raw.x = base[g];
+          raw.x = base[pb + lg];
           if (use_dir) raw.z = dir[g];
         }
         if (y + 1 < N) {
-          raw.y = base[g + N];
+          raw.y = base[pb + lg + N];
           if (use_dir) raw.w = dir[g + N];
         }
         return raw;
@@ -596,11 +599,11 @@ k_row_fwd_pruned(SocsArgs a, int mode, const float* base, const float* __restric
           const int y = row0 + 2 * l + h;
           float m = 0.f;
           if (y < N) {
-            const size_t g = tile + (size_t)y * N + x;
+            const size_t lg = (size_t)y * N + x, g = tile + lg;
             float v = h == 0 ? raw.x : raw.y;
             if (mode == 1) {
               if (dir != nullptr) v = v + cf * (h == 0 ? raw.z : raw.w);
-              if (theta_out != nullptr) theta_out[g] = v;
+              if (theta_out != nullptr) theta_out[pt + lg] = v;
               m = 0.5f * (1.0f + cosf(v));
             } else {
               m = v;
@@ -755,6 +758,7 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
   if (a.active != nullptr && a.active[b] == 0) return;
   const int x0 = blockIdx.x * C::LINES;
   const size_t tile = (size_t)b * N * N;
+  const size_t pt = point_offset(a, b, 1);   // trial theta and its gradient
 #pragma unroll
   for (int s = 0; s < C::PER; ++s) {
     const int e = threadIdx.x + s * C::NT;
@@ -772,13 +776,13 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
     const int c = e % C::LINES, y = e / C::LINES;
     const int x = x0 + c;
     if (e < C::ELEMS && x < N) {
-      const size_t gi = tile + (size_t)y * N + x;
+      const size_t lg = (size_t)y * N + x, gi = tile + lg;
       const float gm = ga.scale * buf[c * C::LS + fft::swz<C::SH>(y)].x;
       if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gm;
       if (ga.theta != nullptr) {
-        float gt = -0.5f * gm * sinf(ga.theta[gi]);
+        float gt = -0.5f * gm * sinf(ga.theta[pt + lg]);
         if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
-        if (ga.grad_theta != nullptr) ga.grad_theta[gi] = gt;
+        if (ga.grad_theta != nullptr) ga.grad_theta[pt + lg] = gt;
         nrm += (double)gt * (double)gt;
         if (ga.dir != nullptr) dot += (double)gt * (double)ga.dir[gi];
       }
@@ -1416,6 +1420,7 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
   const int cb = xcd_column_block(blockIdx.x, gridDim.x);   // XCD-contiguous column blocks
   const int x0 = cb * C::LINES;
   const size_t tile = (size_t)b * N * N;
+  const size_t pt = point_offset(a, b, 1);   // trial theta and its gradient
   // theta at this thread's output elements, all loads in flight at once
   // (inside the epilogue, sinf's branches left one round trip per element)
   float th[CF::PERM][CF::RM];
@@ -1426,8 +1431,8 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
     const bool ok = (CF::BM % C::NT == 0 || q < CF::BM) && x < N;
 #pragma unroll
     for (int r = 0; r < CF::RM; ++r) {
-      const size_t gi = tile + (size_t)(i + r * CF::TM) * N + x;
-      th[s][r] = ok && ga.theta != nullptr ? ga.theta[gi] : 0.f;
+      const size_t lg = (size_t)(i + r * CF::TM) * N + x;
+      th[s][r] = ok && ga.theta != nullptr ? ga.theta[pt + lg] : 0.f;
     }
   }
   CF::fill_mid_tables(mid, a.tw);
@@ -1445,13 +1450,13 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
 #pragma unroll
     for (int r = 0; r < CF::RM; ++r) {
       const int y = i + r * CF::TM;
-      const size_t gi = tile + (size_t)y * N + x;
+      const size_t lg = (size_t)y * N + x, gi = tile + lg;
       const float gm = ga.scale * v[s][r].x;
       if (ga.grad_mask != nullptr) ga.grad_mask[gi] = gm;
       if (ga.theta != nullptr) {
         float gt = -0.5f * gm * sinf(th[s][r]);
         if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
-        if (ga.grad_theta != nullptr) ga.grad_theta[gi] = gt;
+        if (ga.grad_theta != nullptr) ga.grad_theta[pt + lg] = gt;
         nrm += (double)gt * (double)gt;
         th[s][r] = gt;
       }

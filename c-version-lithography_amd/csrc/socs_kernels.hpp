@@ -42,7 +42,22 @@ struct SocsArgs {
   float2* ghat;                   // [b][wr][n] column spectrum of dL/dI at m in [0, wr)
   float* spart;                   // partial power spectra of the k groups (corr_power_bytes)
   int adj_mfma = 0;               // band-limited adjoint as a Toeplitz GEMM on the matrix cores (corr_mfma.hip)
+  // Point slots of the L-BFGS tiles (opt_kernels.hpp).  When set, K1 reads
+  // the line-search origin theta from point ring[b * ring_stride] of tile b
+  // and writes the trial theta to point ring[b * ring_stride + 1]; the final
+  // column kernel reads that trial theta and writes its gradient to the same
+  // point of the gradient slots.  Tile b's points start at point
+  // b * ring_points.  Null: the theta buffers are tile-major like the others.
+  const int* ring = nullptr;
+  int ring_stride = 0, ring_points = 0;
 };
+
+// Element offset of tile b's origin (which = 0) or trial (which = 1) theta.
+__device__ __forceinline__ size_t point_offset(const SocsArgs& a, int b, int which) {
+  const size_t n2 = (size_t)a.n * a.n;
+  if (a.ring == nullptr) return (size_t)b * n2;
+  return ((size_t)b * a.ring_points + a.ring[b * a.ring_stride + which]) * n2;
+}
 
 // Lengths of the band-limited column path (powers of two).
 bool corr_len_supported(int l);
