@@ -261,10 +261,9 @@ int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, cons
 
 /* Timing of the L-BFGS pair kernel alone (no reference counterpart; the
  * reference's LBFGS++ forms s, y and its dot products on the host): one tile
- * of n floats (a multiple of 4) with a full history of m pairs, the
- * accepted-step variant that also shifts xp <- x and gp <- g (24 vectors read,
- * 4 written).  *ms = mean over `repeats` back-to-back launches, by HIP events
- * on the context stream. */
+ * of n floats (a multiple of 4) with a full history of m chained pairs over
+ * its point slots (2 m + 4 vectors read, none written).  *ms = mean over
+ * `repeats` back-to-back launches, by HIP events on the context stream. */
 int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, double* ms);
 
 /* The L-BFGS history and inverse-Hessian product of CTM_Optimizer::optimize_lbfgs
