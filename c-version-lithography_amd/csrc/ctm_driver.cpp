@@ -206,8 +206,9 @@ struct GdPool final : CtmPool::Impl {
       coef[b] = (running[b] && !first[b]) ? (float)(-opt.learning_rate) : 0.f;
     }
     pm.upload_active_step(s);
+    arm_reductions(plan, flags, S);
     graph->launch();
-    const double* red = wait_reductions(plan);
+    const double* red = wait_reductions(plan, flags, S);
     std::vector<int> done;
     bool update = false;
     for (int b = 0; b < S; ++b) {
@@ -433,8 +434,9 @@ struct LbfgsPool final : CtmPool::Impl {
     if (!any) return;   // a new direction always comes with its first trial
     pm.upload_active_step(s);
     if (directions) launch_directions();
+    arm_reductions(plan, flags, S);
     graph->launch();
-    const double* red = wait_reductions(plan);
+    const double* red = wait_reductions(plan, flags, S);
     if (directions && pair_timed) {   // both events completed before the evaluation did
       float ms = 0.f;
       LITHO_HIP(hipEventElapsedTime(&ms, pair_ev[0], pair_ev[1]));

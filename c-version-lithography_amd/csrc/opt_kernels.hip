@@ -615,8 +615,10 @@ void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, 
 // One block per (v, tile): thread t sums blocks t, t + 256, ... in order, then a
 // fixed shuffle tree and a fixed sum over the four waves.
 __global__ void __launch_bounds__(256)
-k_reduce_fixed(const double* __restrict__ partials, int nblk, int nv, double* __restrict__ out, int out_stride) {
+k_reduce_fixed(const double* __restrict__ partials, int nblk, int nv, double* __restrict__ out, int out_stride,
+               const int* __restrict__ active) {
   const int v = blockIdx.x, b = blockIdx.y;
+  if (active != nullptr && active[b] == 0) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < nblk; i += 256) s += partials[((size_t)b * nblk + i) * nv + v];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
@@ -627,8 +629,8 @@ k_reduce_fixed(const double* __restrict__ partials, int nblk, int nv, double* __
 }
 
 void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_fixed, dim3(nv, batch), dim3(256), 0, s, partials, nblk, nv, out, out_stride);
+                         hipStream_t s, const int* active) {
+  hipLaunchKernelGGL(k_reduce_fixed, dim3(nv, batch), dim3(256), 0, s, partials, nblk, nv, out, out_stride, active);
 }
 
 }  // namespace litho

@@ -71,7 +71,8 @@ void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, 
                             float* out_theta, float* out_mask, hipStream_t s);
 // out[b * out_stride + v] = sum over blk of partials[(b * nblk + blk) * nv + v] for
 // v < nv, summed in a fixed tree order (deterministic, independent of scheduling).
+// With active set, only the tiles with active[b] != 0 are written.
 void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,
-                         hipStream_t s);
+                         hipStream_t s, const int* active = nullptr);
 
 }  // namespace litho

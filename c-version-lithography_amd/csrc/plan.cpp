@@ -1,6 +1,9 @@
 // Plan construction and the batched evaluation sequence.
 #include "plan.hpp"
 
+#include <atomic>
+#include <cstdint>
+
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -183,12 +186,14 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   if (!rq.adjoint) a2.v = nullptr;
   plan.ops->image_adjoint(a2, im, batch, s, kColumnAll);
   double* red = rq.red_out != nullptr ? rq.red_out : plan.red.as<double>();
-  reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, red, 4, batch, s);
+  // only the evaluated tiles' reductions are written: the host may wait on
+  // them by value (wait_reductions) while other tiles' entries are rearmed
+  reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, red, 4, batch, s, rq.active);
   if (rq.adjoint && rq.im.target != nullptr) {
     GradArgs ga = rq.ga;
     ga.partials = plan.grad_partials.as<double>();
     plan.ops->adjoint_tail(a, ga, batch, s);
-    reduce_pairs(plan.grad_partials.as<double>(), plan.col_blocks, red + 2, 4, batch, s);
+    reduce_pairs(plan.grad_partials.as<double>(), plan.col_blocks, red + 2, 4, batch, s, rq.active);
   }
   LITHO_HIP(hipGetLastError());
 }
@@ -300,8 +305,38 @@ void spin_wait(hipStream_t s) {
   LITHO_HIP(e);
 }
 
-const double* wait_reductions(Plan& plan) {
-  spin_wait(plan.ctx->stream);
+namespace {
+// A signalling-NaN pattern that no reduction produces (FP64 arithmetic yields
+// quiet NaNs), marking a reduction as not yet written.
+constexpr uint64_t kUnwritten = 0x7FF4C0FFEE0DD00DULL;
+}  // namespace
+
+void arm_reductions(Plan& plan, const int* flags, int batch) {
+  uint64_t* red = static_cast<uint64_t*>(plan.host_red->ptr);
+  for (int b = 0; b < batch; ++b)
+    if (flags[b] != 0)
+      for (int k = 0; k < 4; ++k) __atomic_store_n(&red[4 * b + k], kUnwritten, __ATOMIC_RELAXED);
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+}
+
+const double* wait_reductions(Plan& plan, const int* flags, int batch) {
+  const uint64_t* red = static_cast<const uint64_t*>(plan.host_red->ptr);
+  hipStream_t s = plan.ctx->stream;
+  for (int b = 0; b < batch; ++b) {
+    if (flags[b] == 0) continue;
+    for (int k = 0; k < 4; ++k) {
+      for (unsigned spins = 1; __atomic_load_n(&red[4 * b + k], __ATOMIC_ACQUIRE) == kUnwritten; ++spins) {
+        if ((spins & 4095u) != 0) continue;
+        // a failed or finished stream with the value still unwritten ends the wait
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipErrorNotReady) continue;
+        LITHO_HIP(e);
+        if (__atomic_load_n(&red[4 * b + k], __ATOMIC_ACQUIRE) == kUnwritten)
+          throw Error(3, "evaluation finished without writing its reductions");
+      }
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
   return plan.host_red->as<double>();
 }
 

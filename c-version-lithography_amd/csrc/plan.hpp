@@ -226,8 +226,14 @@ void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, dou
 const double* fetch_reductions(Plan& plan, int batch);
 // Waits for every command on the stream by polling it.
 void spin_wait(hipStream_t s);
-// The reductions of an evaluation whose request wrote them to plan.host_red
-// (red_out = its device address): waits for the stream, no copy.
-const double* wait_reductions(Plan& plan);
+// Evaluations whose request writes its reductions to plan.host_red (red_out =
+// its device address): arm_reductions marks the entries of the tiles with
+// flags[b] != 0 unwritten before the evaluation is launched, and
+// wait_reductions spins on those host-memory values until the device has
+// written them all (no stream call on the way: a stream poll answers
+// ~20 us later than the values arrive).  The evaluation writes only its
+// active tiles' entries.
+void arm_reductions(Plan& plan, const int* flags, int batch);
+const double* wait_reductions(Plan& plan, const int* flags, int batch);
 
 }  // namespace litho

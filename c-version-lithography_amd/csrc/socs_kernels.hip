@@ -1610,8 +1610,9 @@ struct SocsImpl {
   }
 };
 
-void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s) {
-  launch_reduce_fixed(partials, nblk, 2, out, out_stride, batch, s);
+void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s,
+                  const int* active) {
+  launch_reduce_fixed(partials, nblk, 2, out, out_stride, batch, s, active);
 }
 
 template <int N>

@@ -153,6 +153,7 @@ int first_radix(int n);
 std::vector<float2> row_stage_twiddles(int n);
 // Pass twiddle tables of the register-resident length-L transforms (fft_regs.hpp).
 std::vector<float2> corr_twiddles(int l);
-void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s);
+void reduce_pairs(const double* partials, int nblk, double* out, int out_stride, int batch, hipStream_t s,
+                  const int* active = nullptr);
 
 }  // namespace litho
