@@ -1,5 +1,5 @@
-// Register-resident line transforms of power-of-two length L (64 ... 1024)
-// for the band-limited column path.  Thread j of a line (0 <= j < L/8) holds
+// Register-resident line transforms of power-of-two length L (64 ... 2048)
+// for the band-limited column path and its image kernel.  Thread j of a line (0 <= j < L/8) holds
 // the eight elements j + r L/8 (r = 0..7) in registers both on entry and on
 // exit: the first and last Stockham passes are radix 8, so the first reads
 // its butterfly straight from the registers (which the caller fills from
@@ -38,7 +38,7 @@ struct RegPlan {
   int tab;         // table entries
 };
 
-// 64 = 8*8, 128 = 8*2*8, 256 = 8*4*8, 512 = 8*8*8, 1024 = 8*4*4*8
+// 64 = 8*8, 128 = 8*2*8, 256 = 8*4*8, 512 = 8*8*8, 1024 = 8*4*4*8, 2048 = 8*4*8*8
 constexpr RegPlan reg_plan(int l) {
   RegPlan p{};
   int mid[2] = {0, 0}, nm = 0;
@@ -46,6 +46,7 @@ constexpr RegPlan reg_plan(int l) {
   else if (l == 256) mid[nm++] = 4;
   else if (l == 512) mid[nm++] = 8;
   else if (l == 1024) { mid[nm++] = 4; mid[nm++] = 4; }
+  else if (l == 2048) { mid[nm++] = 4; mid[nm++] = 8; }
   else if (l != 64) { p.count = -1; return p; }
   p.R[p.count++] = 8;
   for (int i = 0; i < nm; ++i) p.R[p.count++] = mid[i];
@@ -75,7 +76,7 @@ constexpr int reg_line_stride(int l, int lines) {
 template <int L, int LINES>
 struct RegFFT {
   static constexpr RegPlan PL = reg_plan(L);
-  static_assert(PL.count >= 2, "register transforms cover L = 64 ... 1024");
+  static_assert(PL.count >= 2, "register transforms cover L = 64 ... 2048");
   static constexpr int Q = L / 8;               // threads per line
   static constexpr int NT = LINES * Q;
   static constexpr int SH = 3;

@@ -138,6 +138,12 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
       const std::vector<float2> twl = corr_twiddles(L);
       p->twl.alloc(twl.size() * sizeof(float2));
       LITHO_HIP(hipMemcpy(p->twl.ptr, twl.data(), p->twl.bytes, hipMemcpyHostToDevice));
+      // the same tables at length n for the register-resident image kernel (powers of two)
+      const std::vector<float2> twi = p->generic ? std::vector<float2>{} : corr_twiddles(n);
+      if (!twi.empty()) {
+        p->twi.alloc(twi.size() * sizeof(float2));
+        LITHO_HIP(hipMemcpy(p->twi.ptr, twi.data(), p->twi.bytes, hipMemcpyHostToDevice));
+      }
     }
 
     const size_t c = (size_t)capacity, N = (size_t)n;
@@ -161,7 +167,7 @@ Plan* make_plan(Context* ctx, int n, int k, int wr, int wc, int y0, int x0, cons
     p->host_coef = new HostPinned(c * sizeof(float));
     p->device_bytes = p->h.bytes + p->tw.bytes + p->rtw.bytes + p->row_of_u.bytes + p->col_of_u.bytes +
                       p->row_freq.bytes + p->col_freq.bytes + p->rt.bytes +
-                      p->mhat.bytes + p->t.bytes + p->v.bytes + p->y.bytes + p->p.bytes + p->twl.bytes +
+                      p->mhat.bytes + p->t.bytes + p->v.bytes + p->y.bytes + p->p.bytes + p->twl.bytes + p->twi.bytes +
                       p->acorr.bytes + p->ghat.bytes + p->spart.bytes + p->img_partials.bytes +
                       p->grad_partials.bytes + p->red.bytes + p->coef.bytes;
   } catch (...) {

@@ -127,6 +127,7 @@ struct Plan {
   int row_pruned = 0;                        // window-pruned row kernels in use
   int corr_len = 0;                          // L of the band-limited column path, 0 = fused column kernel
   DevMem twl, acorr, ghat, spart;
+  DevMem twi;                                // pass twiddles of the length-n register transforms (powers of two)
   int adj_mfma = 0;                          // band-limited adjoint on the matrix cores (Toeplitz GEMM)
   int generic = 0;                           // direct-DFT pipeline (no compiled transforms for n)
   int tuned_chunk = 0;                       // batch size chosen by autotune_chunk (0 = not tuned)
@@ -162,6 +163,7 @@ struct Plan {
     a.acorr = acorr.as<float2>();
     a.ghat = ghat.as<float2>();
     a.spart = spart.as<float>();
+    a.twi = twi.bytes > 0 ? twi.as<float2>() : nullptr;
     a.adj_mfma = adj_mfma;
     return a;
   }

@@ -95,6 +95,17 @@ static bool image_pairs() {
   }();
   return on;
 }
+// The register-resident image kernel (k_col_image_reg) where it measured
+// faster, 1024^2 tiles; LITHO_IMAGE_REG = 0 in the environment selects the
+// LDS-resident kernel there too (read once per process)
+static bool image_reg() {
+  static const bool on = [] {
+    const char* v = std::getenv("LITHO_IMAGE_REG");
+    return v == nullptr || std::string(v) != "0";
+  }();
+  return on;
+}
+
 
 #ifndef LITHO_COL_NT_LARGE
 #define LITHO_COL_NT_LARGE 1024
@@ -1277,6 +1288,118 @@ k_col_image_corr2(SocsArgs a, ImageArgs im) {
   }
 }
 
+// K4b on register-resident transforms: thread j of a column holds the eight
+// elements j + r N/8 through both transforms (fft_regs.hpp), so the image,
+// the resist and dL/dI are formed in registers between them and only the
+// transforms' inner passes exchange through LDS (two single-barrier
+// exchanges per transform at 1024 = 8*4*4*8, against the LDS-resident
+// kernel's three passes of two barriers each).  LINES columns per block.
+// Measured image stage (power spectra, autocorrelation, image kernel) of
+// one 1024^2 tile, K = 24: 0.028 ms against 0.036 with k_col_image_corr; 4
+// and 2 columns per block the same.  At 2048^2 (8*4*8*8, extended to that
+// length for this test) one tile took 28.4 us against 27.0 and a batch of
+// 8 23.9 against 26.7 us per tile; both kernels there leave the VALU ~13 %
+// busy with their waves waiting on memory, LDS and barriers at the 1-2
+// blocks per CU their LDS allows, so 2048 keeps k_col_image_corr.  Loading
+// the next column group during the current one's transforms spilled and
+// took 45-55 us.
+template <int N, int LINES>
+struct ImgRegCfg {
+  using F = fft::RegFFT<N, LINES>;
+  static constexpr int NT = F::NT;
+  static constexpr int Q = F::Q;
+};
+template <int N>
+constexpr bool img_reg_ok() { return N == 1024; }
+constexpr int kImgRegLines = 4;
+
+// c[d] and the target of column x at d, y = j + r N/8 for r < 8.  A group r
+// that can hold window entries (a block-uniform test) loads for every lane,
+// row 0 standing in where the entry is zero, so that all its loads are in
+// flight together.
+template <int N, int Q>
+__device__ __forceinline__ void img_reg_load(const SocsArgs& a, const ImageArgs& im, int b, int x, int j,
+                                             float2 (&v)[8], float (&tv)[8]) {
+  const int wr = a.wr;
+  const float2* ac = a.acorr + (size_t)b * wr * N + x;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    v[r] = make_float2(0.f, 0.f);
+    if (r * Q < wr || N - ((r + 1) * Q - 1) < wr) {
+      const int d = j + r * Q;
+      const bool lo = d < wr, hi = !lo && N - d < wr;
+      const float2 c = ac[(size_t)(lo ? d : (hi ? N - d : 0)) * N];
+      v[r] = lo ? c : (hi ? cconj(c) : make_float2(0.f, 0.f));
+    }
+  }
+  const size_t col = (size_t)b * N * N + x;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) tv[r] = im.target != nullptr ? im.target[col + (size_t)(j + r * Q) * N] : 0.f;
+}
+
+template <int N, int LINES>
+__global__ void __launch_bounds__((ImgRegCfg<N, LINES>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_image_reg(SocsArgs a, ImageArgs im) {
+  using C = ImgRegCfg<N, LINES>;
+  using F = typename C::F;
+  static_assert(N % LINES == 0, "whole column blocks");
+  __shared__ float2 lds[2 * F::BUF];
+  __shared__ double red[2 * C::NT / 64];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int line = threadIdx.x % LINES, j = threadIdx.x / LINES;
+  const int cb = xcd_column_block(blockIdx.x, gridDim.x);   // XCD-contiguous column blocks
+  const int x = cb * LINES + line;
+  float2 tw[F::TWN];
+  F::load_twiddles(a.twi, tw, j);
+  const size_t tile = (size_t)b * N * N;
+  const int wr = a.wr;
+  float2 v[8];
+  float tv[8];
+  img_reg_load<N, C::Q>(a, im, b, x, j, v, tv);
+  int ex = 0;
+  F::template transform<+1>(v, lds, tw, line, j, ex);
+  // the image at rows j + r N/8: resist, losses, dL/dI (real input of the forward transform)
+  double pe = 0.0, bg = 0.0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int y = j + r * C::Q;
+    const float gv = image_pixel_t(im, tile + (size_t)y * N + x, v[r].x, tv[r], pe, bg);
+    v[r] = make_float2(gv, 0.f);
+  }
+  {
+    for (int off = 32; off > 0; off >>= 1) {
+      pe += __shfl_down(pe, off, 64);
+      bg += __shfl_down(bg, off, 64);
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+      red[2 * wave] = pe;
+      red[2 * wave + 1] = bg;
+    }
+    fft::lds_barrier();
+    if (threadIdx.x == 0 && im.partials != nullptr) {
+      double sp = 0.0, sb = 0.0;
+      for (int w = 0; w < C::NT / 64; ++w) {
+        sp += red[2 * w];
+        sb += red[2 * w + 1];
+      }
+      double* out = im.partials + ((size_t)b * gridDim.x + cb) * 2;
+      out[0] = sp;
+      out[1] = sb;
+    }
+  }
+  if (im.target == nullptr || a.v == nullptr) return;
+  F::template transform<-1>(v, lds, tw, line, j, ex);
+  // ghat[m] = DFT_N(g)[m] on the window rows m < W_r
+  float2* gh = a.ghat + (size_t)b * wr * N + x;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int m = j + r * C::Q;
+    if (m < wr) gh[(size_t)m * N] = v[r];
+  }
+}
+
 // K5: V_k at the window rows from the band-limited ghat
 template <int L>
 __global__ void __launch_bounds__(CorrCfg<L>::NT) __attribute__((amdgpu_waves_per_eu(CorrCfg<L>::WAVES)))
@@ -1501,7 +1624,10 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
 template <int N>
 struct SocsImpl {
   static int col_blocks() { return (N + ColCfg<N>::LINES - 1) / ColCfg<N>::LINES; }
-  static int img_cols() { return image_pairs() && ImgPairCfg<N>::USE ? ImgPairCfg<N>::COLS : ImgCfg<N>::LINES; }
+  static int img_cols() {
+    if (img_reg_ok<N>() && image_reg()) return kImgRegLines;
+    return image_pairs() && ImgPairCfg<N>::USE ? ImgPairCfg<N>::COLS : ImgCfg<N>::LINES;
+  }
   static int img_blocks() { return (N + img_cols() - 1) / img_cols(); }
 
   static void forward_half(const SocsArgs& a, int batch, int mode, const float* base, const float* dir,
@@ -1556,10 +1682,20 @@ struct SocsImpl {
     if (a.corr_len > 0) {
       if (part != kColumnAdjoint) {
         launch_corr_any(0, a, batch, a.spart, s);
-        if (image_pairs() && ImgPairCfg<N>::USE)
-          hipLaunchKernelGGL(k_col_image_corr2<N>, dim3(img_blocks(), batch), dim3(ImgPairCfg<N>::NT), 0, s, a, im);
-        else
-          hipLaunchKernelGGL(k_col_image_corr<N>, dim3(img_blocks(), batch), dim3(ImgCfg<N>::NT), 0, s, a, im);
+        bool image_done = false;
+        if constexpr (img_reg_ok<N>()) {
+          if (image_reg()) {
+            hipLaunchKernelGGL((k_col_image_reg<N, kImgRegLines>), dim3(img_blocks(), batch),
+                               dim3(ImgRegCfg<N, kImgRegLines>::NT), 0, s, a, im);
+            image_done = true;
+          }
+        }
+        if (!image_done) {
+          if (image_pairs() && ImgPairCfg<N>::USE)
+            hipLaunchKernelGGL(k_col_image_corr2<N>, dim3(img_blocks(), batch), dim3(ImgPairCfg<N>::NT), 0, s, a, im);
+          else
+            hipLaunchKernelGGL(k_col_image_corr<N>, dim3(img_blocks(), batch), dim3(ImgCfg<N>::NT), 0, s, a, im);
+        }
       }
       if (part != kColumnImage && im.target != nullptr && a.v != nullptr) {
         if (a.adj_mfma)

@@ -41,6 +41,7 @@ struct SocsArgs {
   float2* acorr;                  // [b][wr][n] column autocorrelation c[d], d in [0, wr)
   float2* ghat;                   // [b][wr][n] column spectrum of dL/dI at m in [0, wr)
   float* spart;                   // partial power spectra of the k groups (corr_power_bytes)
+  const float2* twi;              // pass twiddle tables of the length-N register transforms (corr_twiddles(n)), or null
   int adj_mfma = 0;               // band-limited adjoint as a Toeplitz GEMM on the matrix cores (corr_mfma.hip)
   // Point slots of the L-BFGS tiles (opt_kernels.hpp).  When set, K1 reads
   // the line-search origin theta from point ring[b * ring_stride] of tile b

@@ -46,7 +46,7 @@ def run(gpu_ctx, monkeypatch, s, mask, tgt, rows, cols):
     return host(a)[0], host(g)[0], pe[0]
 
 
-@pytest.mark.parametrize("n,k", [(384, 6), (640, 6), (1152, 6), (2048, 4)])
+@pytest.mark.parametrize("n,k", [(384, 6), (640, 6), (1024, 6), (1152, 6), (2048, 4)])
 def test_pruned_transforms_equal_unpruned(gpu_ctx, monkeypatch, n, k):
     s = window_kernels(n, k)
     tgt = targets().manhattan_target(n, 3)

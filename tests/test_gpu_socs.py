@@ -108,7 +108,8 @@ def make_case(kind, n, k=24, seed=0):
 
 
 CASES = [("synthetic", 20, 2), ("synthetic", 24, 2), ("socs", 64, 8), ("socs", 128, 24), ("socs", 512, 24),
-         ("window", 384, 6), ("window", 640, 6), ("window", 896, 6), ("window", 1152, 6), ("window", 2176, 4),
+         ("window", 384, 6), ("window", 640, 6), ("window", 896, 6), ("window", 1024, 6), ("window", 1152, 6),
+         ("window", 2176, 4),
          ("wide", 640, 4)]
 
 
