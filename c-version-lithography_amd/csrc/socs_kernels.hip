@@ -95,9 +95,10 @@ static bool image_pairs() {
   }();
   return on;
 }
-// The register-resident image kernel (k_col_image_reg) where it measured
-// faster, 1024^2 tiles; LITHO_IMAGE_REG = 0 in the environment selects the
-// LDS-resident kernel there too (read once per process)
+// The register-resident image kernels (k_col_image_reg at 1024^2,
+// k_col_image_reg2 at 2048^2) where they measured faster; LITHO_IMAGE_REG = 0
+// in the environment selects the LDS-resident kernels there too (read once
+// per process)
 static bool image_reg() {
   static const bool on = [] {
     const char* v = std::getenv("LITHO_IMAGE_REG");
@@ -105,7 +106,6 @@ static bool image_reg() {
   }();
   return on;
 }
-
 
 #ifndef LITHO_COL_NT_LARGE
 #define LITHO_COL_NT_LARGE 1024
@@ -1172,10 +1172,10 @@ k_col_image_corr(SocsArgs a, ImageArgs im) {
 // 1152^2 (batched tiles 1.3 % faster in all).  At 640^2 it is faster per
 // batch of 32 (0.215 against 0.228 ms) but the 256-Micro full-chip update
 // ran 3 % slower with it (99.1-99.8 against 101.9-102.7 updates/s), so 640
-// keeps the one-column kernel.  A lone 2048^2 tile has only 1024 column
-// pairs, two waves per SIMD: it stays latency bound (29-30 us with 4 or 8
-// columns per block against 28 us for k_col_image_corr, with 29 % fewer VALU
-// and 45 % fewer LDS instructions), so it keeps the one-column kernel too.
+// keeps the one-column kernel.  At 2048^2 this LDS-resident pairing stayed
+// latency bound (29-30 us with 4 or 8 columns per block against 28 us for
+// k_col_image_corr); the same pairing on register-resident transforms
+// (k_col_image_reg2) is the one that runs there.
 template <int N>
 struct ImgPairCfg {
   using IC = ImgCfg<N>;
@@ -1291,27 +1291,28 @@ k_col_image_corr2(SocsArgs a, ImageArgs im) {
 // K4b on register-resident transforms: thread j of a column holds the eight
 // elements j + r N/8 through both transforms (fft_regs.hpp), so the image,
 // the resist and dL/dI are formed in registers between them and only the
-// transforms' inner passes exchange through LDS (two single-barrier
-// exchanges per transform at 1024 = 8*4*4*8, against the LDS-resident
-// kernel's three passes of two barriers each).  LINES columns per block.
-// Measured image stage (power spectra, autocorrelation, image kernel) of
-// one 1024^2 tile, K = 24: 0.028 ms against 0.036 with k_col_image_corr; 4
-// and 2 columns per block the same.  At 2048^2 (8*4*8*8, extended to that
-// length for this test) one tile took 28.4 us against 27.0 and a batch of
-// 8 23.9 against 26.7 us per tile; both kernels there leave the VALU ~13 %
-// busy with their waves waiting on memory, LDS and barriers at the 1-2
-// blocks per CU their LDS allows, so 2048 keeps k_col_image_corr.  Loading
-// the next column group during the current one's transforms spilled and
-// took 45-55 us.
+// transforms' inner passes exchange through LDS (single-barrier exchanges:
+// two per transform at 1024 = 8*4*4*8, three at 2048 = 8*4*8*8, against the
+// LDS-resident kernel's three passes of two barriers each).  LINES lines
+// per block.  Measured image kernel (kernel trace, K = 24, one tile):
+// * 1024^2: k_col_image_reg<1024, 4> 9.1 us, the paired k_col_image_reg2
+//   10.3, k_col_image_corr 17.3;
+// * 2048^2: k_col_image_reg2<2048, 2> 21.5 us (1 / 4 pairs per block: 24.1 /
+//   20.2; batch of 8: 16.0 us per tile against 16.6 / 18.9), the unpaired
+//   k_col_image_reg 28.4, k_col_image_corr 26.5 (batch of 8: 26.7 per
+//   tile).  Loading the next column group during the current one's
+//   transforms spilled registers and took 45-55 us.
 template <int N, int LINES>
 struct ImgRegCfg {
   using F = fft::RegFFT<N, LINES>;
   static constexpr int NT = F::NT;
   static constexpr int Q = F::Q;
 };
+// 1: k_col_image_reg (4 columns per block), 2: k_col_image_reg2 (2 column
+// pairs per block), 0: the LDS-resident kernels
 template <int N>
-constexpr bool img_reg_ok() { return N == 1024; }
-constexpr int kImgRegLines = 4;
+constexpr int img_reg_kind() { return N == 1024 ? 1 : (N == 2048 ? 2 : 0); }
+constexpr int kImgRegLines = 4, kImgRegPairLines = 2;
 
 // c[d] and the target of column x at d, y = j + r N/8 for r < 8.  A group r
 // that can hold window entries (a block-uniform test) loads for every lane,
@@ -1397,6 +1398,112 @@ k_col_image_reg(SocsArgs a, ImageArgs im) {
   for (int r = 0; r < 8; ++r) {
     const int m = j + r * C::Q;
     if (m < wr) gh[(size_t)m * N] = v[r];
+  }
+}
+
+// k_col_image_reg with two real columns per complex line (the pairing of
+// k_col_image_corr2): line l of a block holds Z = c_A + i c_B for columns
+// xa = x0 + 2l and xb = xa + 1 (c[0] taken real), so its inverse is
+// I_A + i I_B and its forward transform of g_A + i g_B splits into
+// ghat_A[m] = (Y[m] + conj Y[-m]) / 2, ghat_B[m] = (Y[m] - conj Y[-m]) / (2i).
+// Y[-m] sits with another thread of the line (Q - j, or thread 0), so the
+// forward transform's outputs go through LDS once more.  LINES complex lines
+// (2 LINES columns) per block.
+template <int N, int LINES>
+__global__ void __launch_bounds__((ImgRegCfg<N, LINES>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_image_reg2(SocsArgs a, ImageArgs im) {
+  using C = ImgRegCfg<N, LINES>;
+  using F = typename C::F;
+  static_assert(N % (2 * LINES) == 0, "whole column blocks");
+  constexpr int Q = C::Q;
+  __shared__ float2 lds[2 * F::BUF];
+  __shared__ double red[2 * C::NT / 64];
+  const int b = blockIdx.y;
+  if (a.active != nullptr && a.active[b] == 0) return;
+  const int line = threadIdx.x % LINES, j = threadIdx.x / LINES;
+  const int cb = xcd_column_block(blockIdx.x, gridDim.x);   // XCD-contiguous column blocks
+  const int xa = cb * 2 * LINES + 2 * line, xb = xa + 1;
+  float2 tw[F::TWN];
+  F::load_twiddles(a.twi, tw, j);
+  const size_t tile = (size_t)b * N * N;
+  const int wr = a.wr;
+  // Z[d] = c_A[d] + i c_B[d] at d = j + r Q (loads as img_reg_load, both columns)
+  float2 v[8];
+  const float2* ac = a.acorr + (size_t)b * wr * N;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    v[r] = make_float2(0.f, 0.f);
+    if (r * Q < wr || N - ((r + 1) * Q - 1) < wr) {
+      const int d = j + r * Q;
+      const bool lo = d < wr, hi = !lo && N - d < wr;
+      const size_t src = (size_t)(lo ? d : (hi ? N - d : 0)) * N;
+      float2 ca = ac[src + xa], cbv = ac[src + xb];
+      if (d == 0) ca.y = cbv.y = 0.f;   // c[0] is real: keep I_A and I_B apart
+      if (hi) {
+        ca.y = -ca.y;
+        cbv.y = -cbv.y;
+      }
+      if (lo || hi) v[r] = make_float2(ca.x - cbv.y, ca.y + cbv.x);
+    }
+  }
+  float ta[8], tb[8];
+  const size_t col = tile + xa;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const size_t g = col + (size_t)(j + r * Q) * N;
+    ta[r] = im.target != nullptr ? im.target[g] : 0.f;
+    tb[r] = im.target != nullptr ? im.target[g + 1] : 0.f;
+  }
+  int ex = 0;
+  F::template transform<+1>(v, lds, tw, line, j, ex);
+  double pe = 0.0, bg = 0.0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const size_t g = col + (size_t)(j + r * Q) * N;
+    const float ga = image_pixel_t(im, g, v[r].x, ta[r], pe, bg);
+    const float gb = image_pixel_t(im, g + 1, v[r].y, tb[r], pe, bg);
+    v[r] = make_float2(ga, gb);
+  }
+  {
+    for (int off = 32; off > 0; off >>= 1) {
+      pe += __shfl_down(pe, off, 64);
+      bg += __shfl_down(bg, off, 64);
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+      red[2 * wave] = pe;
+      red[2 * wave + 1] = bg;
+    }
+    fft::lds_barrier();
+    if (threadIdx.x == 0 && im.partials != nullptr) {
+      double sp = 0.0, sb = 0.0;
+      for (int w = 0; w < C::NT / 64; ++w) {
+        sp += red[2 * w];
+        sb += red[2 * w + 1];
+      }
+      double* out = im.partials + ((size_t)b * gridDim.x + cb) * 2;
+      out[0] = sp;
+      out[1] = sb;
+    }
+  }
+  if (im.target == nullptr || a.v == nullptr) return;
+  F::template transform<-1>(v, lds, tw, line, j, ex);
+  // Y at positions j + r Q into the next exchange buffer, then each window row
+  // m < W_r reads its Y[-m] back
+  float2* yb = lds + (ex & 1) * F::BUF + line * F::LS;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) yb[fft::swz<F::SH>(j + r * Q)] = v[r];
+  fft::lds_barrier();
+  float2* gh = a.ghat + (size_t)b * wr * N;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int m = j + r * Q;
+    if (r * Q < wr && m < wr) {
+      const float2 y = v[r];
+      const float2 yc = cconj(yb[fft::swz<F::SH>(m == 0 ? 0 : N - m)]);
+      gh[(size_t)m * N + xa] = make_float2(0.5f * (y.x + yc.x), 0.5f * (y.y + yc.y));
+      gh[(size_t)m * N + xb] = make_float2(0.5f * (y.y - yc.y), -0.5f * (y.x - yc.x));
+    }
   }
 }
 
@@ -1625,7 +1732,8 @@ template <int N>
 struct SocsImpl {
   static int col_blocks() { return (N + ColCfg<N>::LINES - 1) / ColCfg<N>::LINES; }
   static int img_cols() {
-    if (img_reg_ok<N>() && image_reg()) return kImgRegLines;
+    if (img_reg_kind<N>() == 1 && image_reg()) return kImgRegLines;
+    if (img_reg_kind<N>() == 2 && image_reg()) return 2 * kImgRegPairLines;
     return image_pairs() && ImgPairCfg<N>::USE ? ImgPairCfg<N>::COLS : ImgCfg<N>::LINES;
   }
   static int img_blocks() { return (N + img_cols() - 1) / img_cols(); }
@@ -1683,10 +1791,16 @@ struct SocsImpl {
       if (part != kColumnAdjoint) {
         launch_corr_any(0, a, batch, a.spart, s);
         bool image_done = false;
-        if constexpr (img_reg_ok<N>()) {
+        if constexpr (img_reg_kind<N>() == 1) {
           if (image_reg()) {
             hipLaunchKernelGGL((k_col_image_reg<N, kImgRegLines>), dim3(img_blocks(), batch),
                                dim3(ImgRegCfg<N, kImgRegLines>::NT), 0, s, a, im);
+            image_done = true;
+          }
+        } else if constexpr (img_reg_kind<N>() == 2) {
+          if (image_reg()) {
+            hipLaunchKernelGGL((k_col_image_reg2<N, kImgRegPairLines>), dim3(img_blocks(), batch),
+                               dim3(ImgRegCfg<N, kImgRegPairLines>::NT), 0, s, a, im);
             image_done = true;
           }
         }
