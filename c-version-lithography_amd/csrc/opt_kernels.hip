@@ -402,98 +402,117 @@ k_lbfgs_update(const int* __restrict__ need, const double* __restrict__ dots, in
     yy_b[e] = c;
   }
   __syncthreads();
-  if (tid != 0) return;
-  int* mt_g = meta + b * kMetaStride;
-  // the held pairs after the update (chronological), the trial point as the
-  // new base, and the next trial: the smallest point that neither a held pair
-  // nor the base uses (at most 2 m_max + 1 points are in use)
-  const int base = mt[kMetaBase], trial = mt[kMetaTrial];
-  int st2[H], en2[H];
+  if (tid == 0) {
+    int* mt_g = meta + b * kMetaStride;
+    // the held pairs after the update (chronological), the trial point as the
+    // new base, and the next trial: the smallest point that neither a held
+    // pair nor the base uses (at most 2 m_max + 1 points are in use)
+    const int base = mt[kMetaBase], trial = mt[kMetaTrial];
+    int st2[H], en2[H];
 #pragma unroll
-  for (int i = 0; i < H; ++i) {
-    int a0 = -1, e0 = -1;
-    if (accept) {
-      if (i < m2 - 1) {
-        a0 = mt[kMetaStart + i + drop];
-        e0 = mt[kMetaEnd + i + drop];
-      } else if (i == m2 - 1) {
-        a0 = base;
-        e0 = trial;
+    for (int i = 0; i < H; ++i) {
+      int a0 = -1, e0 = -1;
+      if (accept) {
+        if (i < m2 - 1) {
+          a0 = mt[kMetaStart + i + drop];
+          e0 = mt[kMetaEnd + i + drop];
+        } else if (i == m2 - 1) {
+          a0 = base;
+          e0 = trial;
+        }
+      } else if (i < m2) {
+        a0 = mt[kMetaStart + i];
+        e0 = mt[kMetaEnd + i];
       }
-    } else if (i < m2) {
-      a0 = mt[kMetaStart + i];
-      e0 = mt[kMetaEnd + i];
+      st2[i] = a0;
+      en2[i] = e0;
     }
-    st2[i] = a0;
-    en2[i] = e0;
-  }
-  int next = 0;
-  for (bool used = true; used;) {
-    used = next == trial;
+    int next = 0;
+    for (bool used = true; used;) {
+      used = next == trial;
 #pragma unroll
-    for (int i = 0; i < H; ++i) used = used || st2[i] == next || en2[i] == next;
-    if (used) ++next;
-  }
-  mt_g[0] = m2;
-  mt_g[kMetaBase] = trial;
-  mt_g[kMetaTrial] = next;
+      for (int i = 0; i < H; ++i) used = used || st2[i] == next || en2[i] == next;
+      if (used) ++next;
+    }
+    mt_g[0] = m2;
+    mt_g[kMetaBase] = trial;
+    mt_g[kMetaTrial] = next;
 #pragma unroll
-  for (int i = 0; i < H; ++i) {
-    mt_g[kMetaStart + i] = st2[i];
-    mt_g[kMetaEnd + i] = en2[i];
+    for (int i = 0; i < H; ++i) {
+      mt_g[kMetaStart + i] = st2[i];
+      mt_g[kMetaEnd + i] = en2[i];
+    }
   }
+  // The compact-form coefficients, lane l for index l: the triangular solves
+  // run over the lanes in the order of the serial recurrences (each lane
+  // holds its row of S'Y and Y'Y and its column of S'Y, every w_i and u_i is
+  // broadcast as soon as it is known), so every sum adds the same terms in
+  // the same order as a one-lane solve.
   double* cf = coef + (size_t)b * kCoefStride;
-  for (int i = 0; i < kCoefStride; ++i) cf[i] = 0.0;
+  const int l = tid;
+  const bool lane = l < H;
   if (m2 == 0) {
-    cf[0] = -1.0;
+    if (l == 0) cf[0] = -1.0;
+    if (lane) {
+      cf[1 + l] = 0.0;
+      cf[1 + kMaxHist + l] = 0.0;
+    }
     return;
   }
-  double p[H], q[H], w[H], u[H];
+  double srow[H], scol[H], yrow[H];
 #pragma unroll
   for (int j = 0; j < H; ++j) {
-    const int src = accept ? (j < m2 - 1 ? j + drop : mb) : j;
-    p[j] = j < m2 ? v[4 * src + 2] : 0.0;
-    q[j] = j < m2 ? v[4 * src + 3] : 0.0;
-    w[j] = 0.0;
-    u[j] = 0.0;
+    srow[j] = lane ? s1[l * H + j] : 1.0;
+    scol[j] = lane ? s1[j * H + l] : 0.0;
+    yrow[j] = lane ? y1[l * H + j] : 0.0;
   }
+  const double diag = lane ? s1[l * H + l] : 1.0;
+  const int src = accept ? (l < m2 - 1 ? l + drop : mb) : l;
+  const double pl = lane && l < m2 ? v[4 * src + 2] : 0.0;
+  const double ql = lane && l < m2 ? v[4 * src + 3] : 0.0;
   const double gamma = s1[(m2 - 1) * H + m2 - 1] / y1[(m2 - 1) * H + m2 - 1];
+  // w = R^{-1} p, i descending: lane i forms w_i
+  double w[H], wl = 0.0;
 #pragma unroll
   for (int i = H - 1; i >= 0; --i) {
+    w[i] = 0.0;
     if (i < m2) {
-      double acc = p[i];
+      double acc = pl;
 #pragma unroll
       for (int j = i + 1; j < H; ++j)
-        if (j < m2) acc -= s1[i * H + j] * w[j];
-      w[i] = acc / s1[i * H + i];
+        if (j < m2) acc -= srow[j] * w[j];
+      const double wi = __shfl(acc / srow[i], i, 64);
+      w[i] = wi;
+      wl = l == i ? wi : wl;
     }
   }
+  // r_l = (D + gamma Y'Y) w - gamma q at row l
+  double rl = 0.0;
+  if (lane && l < m2) {
+    double acc = diag * wl - gamma * ql;
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      if (j < m2) acc += gamma * yrow[j] * w[j];
+    rl = acc;
+  }
+  // u = R^{-T} r, i ascending: lane i forms u_i
+  double u[H], ul = 0.0;
 #pragma unroll
   for (int i = 0; i < H; ++i) {
+    u[i] = 0.0;
     if (i < m2) {
-      double acc = s1[i * H + i] * w[i] - gamma * q[i];
+      double acc = rl;
 #pragma unroll
-      for (int j = 0; j < H; ++j)
-        if (j < m2) acc += gamma * y1[i * H + j] * w[j];
-      u[i] = acc;   // r_i; replaced by u_i in increasing i below
+      for (int j = 0; j < i; ++j) acc -= scol[j] * u[j];
+      const double ui = __shfl(acc / diag, i, 64);
+      u[i] = ui;
+      ul = l == i ? ui : ul;
     }
   }
-#pragma unroll
-  for (int i = 0; i < H; ++i) {
-    if (i < m2) {
-      double acc = u[i];
-#pragma unroll
-      for (int j = 0; j < i; ++j) acc -= s1[j * H + i] * u[j];
-      u[i] = acc / s1[i * H + i];
-    }
-  }
-  cf[0] = -gamma;
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    if (j < m2) {
-      cf[1 + j] = -u[j];
-      cf[1 + kMaxHist + j] = gamma * w[j];
-    }
+  if (l == 0) cf[0] = -gamma;
+  if (lane) {
+    cf[1 + l] = l < m2 ? -ul : 0.0;
+    cf[1 + kMaxHist + l] = l < m2 ? gamma * wl : 0.0;
   }
 }
 
