@@ -132,25 +132,30 @@ __device__ __forceinline__ float ls_normal_term(const P& phi, int y, int x, floa
 // The one-sided ends -3 g0 + 4 g1 - g2 and 3 g[n-1] - 4 g[n-2] + g[n-3] are
 // regrouped as combinations of neighbour differences, which keeps them exact
 // up to one rounding of the small result.
-template <class G>
+// INTERIOR: the caller guarantees 0 < j < n - 1 (no edge tests).
+template <bool INTERIOR = false, class G>
 __device__ __forceinline__ float grad1u(const G& g, int j, int n) {
-  if (j == 0) return 3.f * (g(1) - g(0)) + (g(1) - g(2));
-  if (j == n - 1) return 3.f * (g(n - 1) - g(n - 2)) - (g(n - 2) - g(n - 3));
+  if constexpr (!INTERIOR) {
+    if (j == 0) return 3.f * (g(1) - g(0)) + (g(1) - g(2));
+    if (j == n - 1) return 3.f * (g(n - 1) - g(n - 2)) - (g(n - 2) - g(n - 3));
+  }
   return g(j + 1) - g(j - 1);
 }
 
 // evolve_kappa at (y, x) (level_set_utils.cpp:257-278).  The one-sided
 // differences at the tile edges keep every access inside the tile and
 // within 3 pixels of (y, x).  inv_dx2 = 1 / (2 dx), inv_dy2 = 1 / (2 dy).
-template <class P>
+// INTERIOR: (y, x) lies at least 2 pixels inside the tile, so every
+// difference is the central one (the same arithmetic without edge tests).
+template <bool INTERIOR = false, class P>
 __device__ __forceinline__ float ls_kappa_term(const P& phi, int rows, int cols, int y, int x, float inv_dx2,
                                                float inv_dy2, float b) {
-  auto px_at = [&](int yy, int xx) { return grad1u([&](int k) { return phi(yy, k); }, xx, cols); };
-  auto py_at = [&](int yy, int xx) { return grad1u([&](int k) { return phi(k, xx); }, yy, rows); };
+  auto px_at = [&](int yy, int xx) { return grad1u<INTERIOR>([&](int k) { return phi(yy, k); }, xx, cols); };
+  auto py_at = [&](int yy, int xx) { return grad1u<INTERIOR>([&](int k) { return phi(k, xx); }, yy, rows); };
   const float px = px_at(y, x) * inv_dx2, py = py_at(y, x) * inv_dy2;
-  const float pxy = grad1u([&](int k) { return px_at(k, x); }, y, rows) * (inv_dx2 * inv_dy2);
-  const float pxx = grad1u([&](int k) { return px_at(y, k); }, x, cols) * (inv_dx2 * inv_dx2);
-  const float pyy = grad1u([&](int k) { return py_at(k, x); }, y, rows) * (inv_dy2 * inv_dy2);
+  const float pxy = grad1u<INTERIOR>([&](int k) { return px_at(k, x); }, y, rows) * (inv_dx2 * inv_dy2);
+  const float pxx = grad1u<INTERIOR>([&](int k) { return px_at(y, k); }, x, cols) * (inv_dx2 * inv_dx2);
+  const float pyy = grad1u<INTERIOR>([&](int k) { return py_at(k, x); }, y, rows) * (inv_dy2 * inv_dy2);
   const float g2 = px * px + py * py;
   const float num = pxx * py * py - 2.f * px * py * pxy + pyy * px * px;
   return b * (g2 > 2.220446049250313e-16f ? num / g2 : 0.f);
@@ -230,11 +235,19 @@ void launch_lsm_dt(const float* grad, const int* rects, int batch, int n, double
 constexpr int kStageTX = 32, kStageTY = 8;
 constexpr int kStageW = kStageTX + 6, kStageH = kStageTY + 6;
 
+// The FP32 factors of the stencils, formed once on the host in FP64 exactly
+// as the terms kernel forms them per thread.
+struct StageScales {
+  float inv_dx, inv_dy;     // 1 / dx, 1 / dy (normal term)
+  float inv_dx2, inv_dy2;   // 1 / (2 dx), 1 / (2 dy) (curvature term)
+  float b;                  // curvature weight
+};
+
 template <int STAGE>
 __global__ void __launch_bounds__(kStageTX * kStageTY)
 k_lsm_stage(const float* __restrict__ phi_in, const float* base, const float* __restrict__ grad,
-            const int* __restrict__ rects, const double* __restrict__ dt, int n, double dx, double dy, double b,
-            float* out, float* __restrict__ mask_out) {
+            const int* __restrict__ rects, const double* __restrict__ dt, int n, StageScales sc, float* out,
+            float* __restrict__ mask_out) {
   // out may alias base (an in-place update): both are touched at the thread's
   // own pixel only; phi_in is never written by this launch
   __shared__ float sp[kStageH * kStageW];
@@ -252,11 +265,15 @@ k_lsm_stage(const float* __restrict__ phi_in, const float* base, const float* __
   const size_t p = (size_t)y * n + x;
   const LdsPhi<kStageW> phi{sp, y0, x0};
   const int* r = rects + 4 * t;
+  // blocks whose pixels all lie 2 or more inside the tile take the
+  // curvature term without edge tests (block-uniform)
+  const bool interior = x0 >= 2 && y0 >= 2 && x0 + kStageTX + 2 <= n && y0 + kStageTY + 2 <= n;
   float vf;
   if (in_rect(r, y, x)) {
     const float vn = grad[t * plane + p];
-    const float normal = ls_kappa_term(phi, n, n, y, x, (float)(1.0 / (2.0 * dx)), (float)(1.0 / (2.0 * dy)), (float)b) -
-                         ls_normal_term(phi, y, x, vn, (float)(1.0 / dx), (float)(1.0 / dy));
+    const float kap = interior ? ls_kappa_term<true>(phi, n, n, y, x, sc.inv_dx2, sc.inv_dy2, sc.b)
+                               : ls_kappa_term<false>(phi, n, n, y, x, sc.inv_dx2, sc.inv_dy2, sc.b);
+    const float normal = kap - ls_normal_term(phi, y, x, vn, sc.inv_dx, sc.inv_dy);
     const float cur = phi(y, x);
     const float step = (float)dt[t];
     if (STAGE == 1)
@@ -274,12 +291,14 @@ void launch_lsm_stage(int stage, const float* phi_in, const float* base, const f
                       const double* dt, int batch, int n, double dx, double dy, double b, float* out, float* mask_out,
                       hipStream_t s) {
   const dim3 grid((n + kStageTX - 1) / kStageTX, (n + kStageTY - 1) / kStageTY, batch);
+  const StageScales sc{(float)(1.0 / dx), (float)(1.0 / dy), (float)(1.0 / (2.0 * dx)), (float)(1.0 / (2.0 * dy)),
+                       (float)b};
   if (stage == 1)
-    hipLaunchKernelGGL(k_lsm_stage<1>, grid, dim3(kStageTX * kStageTY), 0, s, phi_in, base, grad, rects, dt, n, dx, dy,
-                       b, out, mask_out);
+    hipLaunchKernelGGL(k_lsm_stage<1>, grid, dim3(kStageTX * kStageTY), 0, s, phi_in, base, grad, rects, dt, n, sc,
+                       out, mask_out);
   else
-    hipLaunchKernelGGL(k_lsm_stage<2>, grid, dim3(kStageTX * kStageTY), 0, s, phi_in, base, grad, rects, dt, n, dx, dy,
-                       b, out, mask_out);
+    hipLaunchKernelGGL(k_lsm_stage<2>, grid, dim3(kStageTX * kStageTY), 0, s, phi_in, base, grad, rects, dt, n, sc,
+                       out, mask_out);
 }
 
 // ---------------------------------------------------------------- render
