@@ -256,7 +256,10 @@ k_lbfgs_pair(const float* __restrict__ X, const float* __restrict__ G, int point
 // d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB held pairs,
 // g = G[base], plus the FP64 partial of g . d.  coef layout per tile: [0] cg,
 // [1+j] cs_j, [1+kMaxHist+j] cy_j.  All the point loads of an element are
-// issued before its arithmetic (pairs beyond m load the base, unused).
+// issued before its arithmetic (pairs beyond m load the base, unused).  The
+// sweep runs from the highest addresses down: the pair kernel has just read
+// the same points upwards, so the lines it read last, still in the
+// last-level cache, are read first.
 template <int MB, class V, bool CHAIN>
 __device__ __forceinline__ double direction_sweep(const V* __restrict__ X4, const V* __restrict__ G4,
                                                   const V* __restrict__ gb, const int* mt, int m, float cg,
@@ -277,7 +280,8 @@ __device__ __forceinline__ double direction_sweep(const V* __restrict__ X4, cons
       cx[j + 1] = point_ptr(X4, mt, p, n4);
       cgp[j + 1] = point_ptr(G4, mt, p, n4);
     }
-    for (unsigned bo = bo0; bo < bend; bo += bstride) {
+    for (unsigned kk = bo0 < bend ? (bend - bo0 + bstride - 1) / bstride : 0; kk-- > 0;) {
+      const unsigned bo = bo0 + kk * bstride;
       const V gv = ldv(gb, bo);
       V xv[MB + 1], gvv[MB + 1];
 #pragma unroll
@@ -293,7 +297,8 @@ __device__ __forceinline__ double direction_sweep(const V* __restrict__ X4, cons
       acc += dot4(gv, v);
     }
   } else {
-    for (unsigned bo = bo0; bo < bend; bo += bstride) {
+    for (unsigned kk = bo0 < bend ? (bend - bo0 + bstride - 1) / bstride : 0; kk-- > 0;) {
+      const unsigned bo = bo0 + kk * bstride;
       const V gv = ldv(gb, bo);
       V v = scale4(cg, gv);
 #pragma unroll
