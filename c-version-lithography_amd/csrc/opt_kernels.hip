@@ -652,6 +652,30 @@ k_reduce_fixed(const double* __restrict__ partials, int nblk, int nv, double* __
   if (threadIdx.x == 0) out[(size_t)b * out_stride + v] = ((w[0] + w[1]) + w[2]) + w[3];
 }
 
+// The two reductions that end an evaluation in one launch: values v < 2 from
+// the image partials (PE, penalty), v >= 2 from the gradient partials
+// (|g|^2, g . d), each summed as k_reduce_fixed sums it.
+__global__ void __launch_bounds__(256)
+k_reduce_eval(const double* __restrict__ img, int img_blk, const double* __restrict__ grad, int grad_blk,
+              double* __restrict__ out, const int* __restrict__ active) {
+  const int v = blockIdx.x, b = blockIdx.y;
+  if (active != nullptr && active[b] == 0) return;
+  const double* part = v < 2 ? img : grad;
+  const int nblk = v < 2 ? img_blk : grad_blk, vv = v & 1;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) s += part[((size_t)b * nblk + i) * 2 + vv];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[(size_t)b * 4 + v] = ((w[0] + w[1]) + w[2]) + w[3];
+}
+
+void launch_reduce_eval(const double* img, int img_blk, const double* grad, int grad_blk, double* out, int batch,
+                        hipStream_t s, const int* active) {
+  hipLaunchKernelGGL(k_reduce_eval, dim3(4, batch), dim3(256), 0, s, img, img_blk, grad, grad_blk, out, active);
+}
+
 void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,
                          hipStream_t s, const int* active) {
   hipLaunchKernelGGL(k_reduce_fixed, dim3(nv, batch), dim3(256), 0, s, partials, nblk, nv, out, out_stride, active);

@@ -74,5 +74,10 @@ void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, 
 // With active set, only the tiles with active[b] != 0 are written.
 void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,
                          hipStream_t s, const int* active = nullptr);
+// An evaluation's four reductions, out[b * 4 + v]: v = 0, 1 from the image
+// partials ([b][img_blk][2]), v = 2, 3 from the gradient partials
+// ([b][grad_blk][2]), each in the order of launch_reduce_fixed.
+void launch_reduce_eval(const double* img, int img_blk, const double* grad, int grad_blk, double* out, int batch,
+                        hipStream_t s, const int* active = nullptr);
 
 }  // namespace litho

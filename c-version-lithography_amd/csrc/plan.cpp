@@ -193,13 +193,17 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   plan.ops->image_adjoint(a2, im, batch, s, kColumnAll);
   double* red = rq.red_out != nullptr ? rq.red_out : plan.red.as<double>();
   // only the evaluated tiles' reductions are written: the host may wait on
-  // them by value (wait_reductions) while other tiles' entries are rearmed
-  reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, red, 4, batch, s, rq.active);
+  // them by value (wait_reductions) while other tiles' entries are rearmed.
+  // With the adjoint, the image partials are reduced together with the
+  // gradient's at the end (one launch, off the adjoint's path).
   if (rq.adjoint && rq.im.target != nullptr) {
     GradArgs ga = rq.ga;
     ga.partials = plan.grad_partials.as<double>();
     plan.ops->adjoint_tail(a, ga, batch, s);
-    reduce_pairs(plan.grad_partials.as<double>(), plan.col_blocks, red + 2, 4, batch, s, rq.active);
+    launch_reduce_eval(plan.img_partials.as<double>(), plan.img_blocks, plan.grad_partials.as<double>(),
+                       plan.col_blocks, red, batch, s, rq.active);
+  } else {
+    reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, red, 4, batch, s, rq.active);
   }
   LITHO_HIP(hipGetLastError());
 }
