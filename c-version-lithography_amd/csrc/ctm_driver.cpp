@@ -290,6 +290,7 @@ struct LbfgsPool final : CtmPool::Impl {
     alloc_slots(d, S, n2);
     pair_part.alloc((size_t)S * kOptBlocks * kPairDots * sizeof(double));
     dir_part.alloc((size_t)S * kOptBlocks * sizeof(double));
+    LITHO_HIP(hipMemsetAsync(dir_part.ptr, 0, dir_part.bytes, s));   // reduced by every evaluation
     vec_red.alloc((size_t)S * kPairDots * sizeof(double));
     d_meta.alloc((size_t)S * kMetaStride * sizeof(int));
     d_sy.alloc((size_t)S * kMaxHist * kMaxHist * sizeof(double));
@@ -312,6 +313,11 @@ struct LbfgsPool final : CtmPool::Impl {
     rq.ring = d_meta.as<int>() + kMetaBase;   // [base, trial] at kMetaBase, kMetaBase + 1
     rq.ring_stride = kMetaStride;
     rq.ring_points = P;
+    // the direction's g . d partials, reduced with the evaluation's losses
+    // into mapped host memory (meaningful for the slots with a new direction)
+    rq.extra_partials = dir_part.as<double>();
+    rq.extra_blocks = kOptBlocks;
+    rq.extra_out = hdg.dev_as<double>();
     graph = std::make_unique<EvalGraph>(plan, S, rq);
     LITHO_HIP(hipEventCreate(&pair_ev[0]));
     LITHO_HIP(hipEventCreate(&pair_ev[1]));
@@ -414,8 +420,8 @@ struct LbfgsPool final : CtmPool::Impl {
                         d_yy.as<double>(), d_coef.as<double>(), s);
     launch_lbfgs_direction(X.as<float>(), G.as<float>(), P, pm.d_need(), d_meta.as<int>(), d_coef.as<double>(), n2,
                            d.as<float>(), S, mb_dir, dir_part.as<double>(), s);
-    // g . d straight into mapped host memory (read after the evaluation's wait)
-    launch_reduce_fixed(dir_part.as<double>(), kOptBlocks, 1, hdg.dev_as<double>(), 1, S, s);
+    // g . d is reduced into mapped host memory by the evaluation's final
+    // reduction (off the path to the evaluation)
   }
 
   // (B) one batched evaluation of every slot with a pending trial
@@ -434,9 +440,14 @@ struct LbfgsPool final : CtmPool::Impl {
     if (!any) return;   // a new direction always comes with its first trial
     pm.upload_active_step(s);
     if (directions) launch_directions();
+    // the evaluation writes the losses and g . d of every evaluated slot
+    // (g . d is used where a new direction is pending); waiting for all of
+    // them keeps a late write out of the next round's armed entries
     arm_reductions(plan, flags, S);
+    arm_values(hdg.as<double>(), flags, S);
     graph->launch();
     const double* red = wait_reductions(plan, flags, S);
+    wait_values(hdg.as<double>(), flags, S, s);
     if (directions && pair_timed) {   // both events completed before the evaluation did
       float ms = 0.f;
       LITHO_HIP(hipEventElapsedTime(&ms, pair_ev[0], pair_ev[1]));

@@ -655,25 +655,37 @@ k_reduce_fixed(const double* __restrict__ partials, int nblk, int nv, double* __
 // The two reductions that end an evaluation in one launch: values v < 2 from
 // the image partials (PE, penalty), v >= 2 from the gradient partials
 // (|g|^2, g . d), each summed as k_reduce_fixed sums it.
+// (v = 4, with extra set: one more value per tile from extra's partials
+// ([b][extra_blk], one value each) into extra_out[b].)
 __global__ void __launch_bounds__(256)
 k_reduce_eval(const double* __restrict__ img, int img_blk, const double* __restrict__ grad, int grad_blk,
-              double* __restrict__ out, const int* __restrict__ active) {
+              double* __restrict__ out, const int* __restrict__ active, const double* __restrict__ extra,
+              int extra_blk, double* __restrict__ extra_out) {
   const int v = blockIdx.x, b = blockIdx.y;
   if (active != nullptr && active[b] == 0) return;
-  const double* part = v < 2 ? img : grad;
-  const int nblk = v < 2 ? img_blk : grad_blk, vv = v & 1;
+  const double* part = v < 2 ? img : (v < 4 ? grad : extra);
+  const int nblk = v < 2 ? img_blk : (v < 4 ? grad_blk : extra_blk);
+  const int nv = v < 4 ? 2 : 1, vv = v < 4 ? (v & 1) : 0;
   double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += 256) s += part[((size_t)b * nblk + i) * 2 + vv];
+  for (int i = threadIdx.x; i < nblk; i += 256) s += part[((size_t)b * nblk + i) * nv + vv];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   __shared__ double w[4];
   if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[(size_t)b * 4 + v] = ((w[0] + w[1]) + w[2]) + w[3];
+  if (threadIdx.x == 0) {
+    const double r = ((w[0] + w[1]) + w[2]) + w[3];
+    if (v < 4)
+      out[(size_t)b * 4 + v] = r;
+    else
+      extra_out[b] = r;
+  }
 }
 
 void launch_reduce_eval(const double* img, int img_blk, const double* grad, int grad_blk, double* out, int batch,
-                        hipStream_t s, const int* active) {
-  hipLaunchKernelGGL(k_reduce_eval, dim3(4, batch), dim3(256), 0, s, img, img_blk, grad, grad_blk, out, active);
+                        hipStream_t s, const int* active, const double* extra, int extra_blk, double* extra_out) {
+  const int nv = extra != nullptr && extra_out != nullptr ? 5 : 4;
+  hipLaunchKernelGGL(k_reduce_eval, dim3(nv, batch), dim3(256), 0, s, img, img_blk, grad, grad_blk, out, active, extra,
+                     extra_blk, extra_out);
 }
 
 void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, int out_stride, int batch,

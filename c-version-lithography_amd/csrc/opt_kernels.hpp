@@ -76,8 +76,11 @@ void launch_reduce_fixed(const double* partials, int nblk, int nv, double* out, 
                          hipStream_t s, const int* active = nullptr);
 // An evaluation's four reductions, out[b * 4 + v]: v = 0, 1 from the image
 // partials ([b][img_blk][2]), v = 2, 3 from the gradient partials
-// ([b][grad_blk][2]), each in the order of launch_reduce_fixed.
+// ([b][grad_blk][2]), each in the order of launch_reduce_fixed.  With extra
+// set, also extra_out[b] = the sum of extra's partials ([b][extra_blk], one
+// value each) in the same order.
 void launch_reduce_eval(const double* img, int img_blk, const double* grad, int grad_blk, double* out, int batch,
-                        hipStream_t s, const int* active = nullptr);
+                        hipStream_t s, const int* active = nullptr, const double* extra = nullptr,
+                        int extra_blk = 0, double* extra_out = nullptr);
 
 }  // namespace litho

@@ -201,7 +201,7 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
     ga.partials = plan.grad_partials.as<double>();
     plan.ops->adjoint_tail(a, ga, batch, s);
     launch_reduce_eval(plan.img_partials.as<double>(), plan.img_blocks, plan.grad_partials.as<double>(),
-                       plan.col_blocks, red, batch, s, rq.active);
+                       plan.col_blocks, red, batch, s, rq.active, rq.extra_partials, rq.extra_blocks, rq.extra_out);
   } else {
     reduce_pairs(plan.img_partials.as<double>(), plan.img_blocks, red, 4, batch, s, rq.active);
   }
@@ -320,6 +320,29 @@ namespace {
 // quiet NaNs), marking a reduction as not yet written.
 constexpr uint64_t kUnwritten = 0x7FF4C0FFEE0DD00DULL;
 }  // namespace
+
+void arm_values(double* host_values, const int* flags, int count) {
+  uint64_t* v = reinterpret_cast<uint64_t*>(host_values);
+  for (int i = 0; i < count; ++i)
+    if (flags == nullptr || flags[i] != 0) __atomic_store_n(&v[i], kUnwritten, __ATOMIC_RELAXED);
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+}
+
+void wait_values(const double* host_values, const int* flags, int count, hipStream_t s) {
+  const uint64_t* v = reinterpret_cast<const uint64_t*>(host_values);
+  for (int i = 0; i < count; ++i) {
+    if (flags != nullptr && flags[i] == 0) continue;
+    for (unsigned spins = 1; __atomic_load_n(&v[i], __ATOMIC_ACQUIRE) == kUnwritten; ++spins) {
+      if ((spins & 4095u) != 0) continue;
+      const hipError_t e = hipStreamQuery(s);
+      if (e == hipErrorNotReady) continue;
+      LITHO_HIP(e);
+      if (__atomic_load_n(&v[i], __ATOMIC_ACQUIRE) == kUnwritten)
+        throw Error(3, "evaluation finished without writing its reductions");
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+}
 
 void arm_reductions(Plan& plan, const int* flags, int batch) {
   uint64_t* red = static_cast<uint64_t*>(plan.host_red->ptr);

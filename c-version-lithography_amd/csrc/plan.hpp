@@ -192,6 +192,12 @@ struct EvalRequest {
   // ga.grad_theta are then the point arrays, indexed per tile by ring
   const int* ring = nullptr;
   int ring_stride = 0, ring_points = 0;
+  // optional: partials of one more value per tile ([batch][extra_blocks]),
+  // reduced with the evaluation's own into extra_out[b] (the L-BFGS
+  // direction's g . d, which the host reads with the evaluation's results)
+  const double* extra_partials = nullptr;
+  int extra_blocks = 0;
+  double* extra_out = nullptr;
 };
 
 void run_eval(Plan& plan, int batch, const EvalRequest& rq);
@@ -236,6 +242,10 @@ void spin_wait(hipStream_t s);
 // ~20 us later than the values arrive).  The evaluation writes only its
 // active tiles' entries.
 void arm_reductions(Plan& plan, const int* flags, int batch);
+// The same for other device-written values in mapped host memory: count
+// doubles, those with flags[i] != 0 (all with flags null).
+void arm_values(double* host_values, const int* flags, int count);
+void wait_values(const double* host_values, const int* flags, int count, hipStream_t s);
 const double* wait_reductions(Plan& plan, const int* flags, int batch);
 
 }  // namespace litho
