@@ -238,6 +238,9 @@ def main():
         raise RuntimeError("no pair-kernel launch was timed in the solves")
     pair_ms = pair_total_ms / pair_launches
     pair_bytes = pair_kernel_bytes(N, HISTORY)
+    # the yardstick: a plain kernel with the pair kernel's access pattern
+    # (2m + 4 = 24 vectors read, none written, the same grid), HIP events
+    pattern_ms = ctx.profile_stream(N * N, 2 * HISTORY + 4, 0, repeats=20)
     print(f"stages (ms): forward {st[0]:.3f} column image {st[1]:.3f} column adjoint {st[2]:.3f} "
           f"adjoint-tail {st[3]:.3f} eval {st[4]:.3f}; {PAIR_KERNEL} {pair_ms:.4f} (mean of {pair_launches} "
           f"launches in the timed solves)", file=sys.stderr, flush=True)
@@ -256,7 +259,12 @@ def main():
                      "achieved": round(pair_bytes / (pair_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(pair_bytes / (pair_ms * 1e-3) / HBM_PEAK, 4),
                      "traffic": measured_traffic(PAIR_TRAFFIC_FILE, PAIR_KERNEL), "algorithmic_bytes": pair_bytes,
-                     "launch_ms": round(pair_ms, 4), "launches_timed": pair_launches},
+                     "launch_ms": round(pair_ms, 4), "launches_timed": pair_launches,
+                     # the rate a plain kernel with the same access pattern
+                     # reaches on this GPU (litho_profile_stream), and the
+                     # in-solve launches' fraction of it
+                     "pattern_GBps": round(pair_bytes / (pattern_ms * 1e-3) / 1e9, 1),
+                     "frac_of_pattern": round(pattern_ms / pair_ms, 4)},
         # The dominant kernel of a grad-eval, bound by FP32 vector (VALU) FFT
         # arithmetic, priced against MI355X's published FP32 peak (packed FP32
         # is not used, so the issue ceiling is half of it); its compulsory HBM

@@ -9,9 +9,9 @@
 //   litho_bench save  N K KFILE             precompute the SOCS kernels and save them
 //   litho_bench copy  MIB                   device-to-device copy of MIB MiB (counter calibration)
 //   litho_bench pair  N M REPEATS           the L-BFGS pair kernel alone (litho_profile_lbfgs_pair)
-//   litho_bench stream N R W                a plain kernel that reads R and writes W vectors of N^2
-//                                           floats with the L-BFGS kernels' grid: the rate that
-//                                           access pattern reaches (R, W: 24 0, 22 1, 24 4, 21 1)
+//   litho_bench stream N R W                litho_profile_stream: a plain kernel that reads R and writes
+//                                           W vectors of N^2 floats with the L-BFGS kernels' grid, the
+//                                           rate that access pattern reaches (R, W: 24 0, 22 1, 24 4, 21 1)
 // With KFILE the kernels are read from a file written by `save` (no precompute).
 #include <hip/hip_runtime.h>
 
@@ -49,82 +49,6 @@ std::vector<float> lattice_target(int n) {
   return t;
 }
 
-// Streams R vectors in and W vectors out (float4), 1024 blocks x 256 threads
-// grid-striding like the L-BFGS vector kernels; every load of an element is
-// issued before the sum that consumes it.
-constexpr int kStreamMax = 32;
-struct StreamPtrs {
-  const float4* in[kStreamMax];
-  float4* out[kStreamMax];
-};
-template <int R, int W>
-__global__ void __launch_bounds__(256) k_stream(StreamPtrs p, size_t n4, float* sink) {
-  float keep = 0.f;
-  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) {
-    float4 v[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = p.in[j][i];
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      s.x += v[j].x;
-      s.y += v[j].y;
-      s.z += v[j].z;
-      s.w += v[j].w;
-    }
-#pragma unroll
-    for (int j = 0; j < W; ++j) p.out[j][i] = s;
-    keep += s.x;
-  }
-  if (keep == 1234.5f) sink[0] = keep;   // keeps the loads of a read-only pattern
-}
-
-int stream_mode(int n, int r, int w) {
-  using Launch = void (*)(const StreamPtrs&, size_t, float*);
-  Launch launch = nullptr;
-  if (r == 24 && w == 0) launch = [](const StreamPtrs& p, size_t n4, float* s) { hipLaunchKernelGGL((k_stream<24, 0>), dim3(1024), dim3(256), 0, nullptr, p, n4, s); };
-  if (r == 22 && w == 1) launch = [](const StreamPtrs& p, size_t n4, float* s) { hipLaunchKernelGGL((k_stream<22, 1>), dim3(1024), dim3(256), 0, nullptr, p, n4, s); };
-  if (r == 24 && w == 4) launch = [](const StreamPtrs& p, size_t n4, float* s) { hipLaunchKernelGGL((k_stream<24, 4>), dim3(1024), dim3(256), 0, nullptr, p, n4, s); };
-  if (r == 21 && w == 1) launch = [](const StreamPtrs& p, size_t n4, float* s) { hipLaunchKernelGGL((k_stream<21, 1>), dim3(1024), dim3(256), 0, nullptr, p, n4, s); };
-  if (launch == nullptr || n <= 0 || n % 2 != 0) {
-    std::fprintf(stderr, "stream: N even, (R, W) one of (24, 0), (22, 1), (24, 4), (21, 1)\n");
-    return 2;
-  }
-  const size_t nn = (size_t)n * n, n4 = nn / 4;
-  StreamPtrs p{};
-  std::vector<void*> bufs;
-  for (int j = 0; j < r + w; ++j) {
-    void* b = nullptr;
-    hip_check(hipMalloc(&b, nn * sizeof(float)), "hipMalloc");
-    hip_check(hipMemset(b, 0, nn * sizeof(float)), "hipMemset");
-    bufs.push_back(b);
-    if (j < r)
-      p.in[j] = static_cast<const float4*>(b);
-    else
-      p.out[j - r] = static_cast<float4*>(b);
-  }
-  float* sink = nullptr;
-  hip_check(hipMalloc(&sink, sizeof(float)), "hipMalloc");
-  launch(p, n4, sink);   // untimed
-  hipEvent_t e0, e1;
-  hip_check(hipEventCreate(&e0), "hipEventCreate");
-  hip_check(hipEventCreate(&e1), "hipEventCreate");
-  constexpr int kReps = 20;
-  hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
-  for (int i = 0; i < kReps; ++i) launch(p, n4, sink);
-  hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
-  hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-  float ms = 0.f;
-  hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-  ms /= kReps;
-  const double bytes = (double)(r + w) * nn * sizeof(float);
-  std::printf("stream n=%d reads=%d writes=%d ms=%.4f bytes=%.0f rate=%.1f GB/s\n", n, r, w, ms, bytes,
-              bytes / (ms * 1e-3) / 1e9);
-  for (void* b : bufs) hip_check(hipFree(b), "hipFree");
-  hip_check(hipFree(sink), "hipFree");
-  return 0;
-}
-
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -155,8 +79,19 @@ int main(int argc, char** argv) {
     return 0;
   }
   if (mode == "stream") {
+    // litho_bench stream N R W: the plain R-read, W-write pattern over N x N floats
     if (argc < 5) return 2;
-    return stream_mode(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]));
+    const int sn = std::atoi(argv[2]), r = std::atoi(argv[3]), w = std::atoi(argv[4]);
+    const size_t nn = (size_t)sn * (size_t)sn;
+    litho_ctx* sctx = nullptr;
+    check(litho_ctx_create(0, &sctx), "litho_ctx_create");
+    double ms = 0.0;
+    check(litho_profile_stream(sctx, nn, r, w, 20, &ms), "litho_profile_stream");
+    const double bytes = (double)(r + w) * nn * sizeof(float);
+    std::printf("stream n=%d reads=%d writes=%d ms=%.4f bytes=%.0f rate=%.1f GB/s\n", sn, r, w, ms, bytes,
+                bytes / (ms * 1e-3) / 1e9);
+    check(litho_ctx_destroy(sctx), "litho_ctx_destroy");
+    return 0;
   }
   if (mode == "pair") {
     // litho_bench pair N M REPEATS: the L-BFGS pair kernel over one N x N tile

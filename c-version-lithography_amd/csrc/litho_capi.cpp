@@ -580,6 +580,48 @@ int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, doubl
   });
 }
 
+int litho_profile_stream(litho_ctx* ctx, size_t n, int reads, int writes, int repeats, double* ms) {
+  return guarded([&] {
+    litho::Context& c = ctx_of(ctx);
+    require(ms != nullptr && repeats > 0, "null argument or non-positive repeats");
+    require(n > 0 && n % 4 == 0, "n must be a positive multiple of 4");
+    require((reads == 24 && writes == 0) || (reads == 22 && writes == 1) || (reads == 24 && writes == 4) ||
+                (reads == 21 && writes == 1),
+            "stream pattern: (reads, writes) must be (24, 0), (22, 1), (24, 4) or (21, 1)");
+    hipStream_t s = c.stream;
+    std::vector<litho::DevMem> bufs((size_t)(reads + writes));
+    std::vector<const float*> in;
+    std::vector<float*> out;
+    for (size_t j = 0; j < bufs.size(); ++j) {
+      bufs[j].alloc(n * sizeof(float));
+      LITHO_HIP(hipMemsetAsync(bufs[j].ptr, 0, bufs[j].bytes, s));
+      if ((int)j < reads)
+        in.push_back(bufs[j].as<float>());
+      else
+        out.push_back(bufs[j].as<float>());
+    }
+    litho::DevMem sink;
+    sink.alloc(sizeof(float));
+    auto launch = [&] {
+      if (!litho::launch_stream_pattern(reads, writes, in.data(), out.data(), n / 4, sink.as<float>(), s))
+        throw litho::Error(LITHO_EINVAL, "stream pattern not supported");
+    };
+    launch();   // untimed: loads the code object
+    hipEvent_t e0, e1;
+    LITHO_HIP(hipEventCreate(&e0));
+    LITHO_HIP(hipEventCreate(&e1));
+    LITHO_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < repeats; ++r) launch();
+    LITHO_HIP(hipEventRecord(e1, s));
+    LITHO_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    LITHO_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / repeats;
+  });
+}
+
 int litho_lbfgs_direction(litho_ctx* ctx, size_t n, int m, int num_points, const float* x_points,
                           const float* g_points, float* d_out, double* gd_out, int* kept_out) {
   return guarded([&] {

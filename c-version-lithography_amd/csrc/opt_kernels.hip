@@ -636,6 +636,56 @@ void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, 
                      points, meta, b, which, n, out_theta, out_mask);
 }
 
+// The yardstick of the L-BFGS vector kernels: a plain kernel that reads R and
+// writes W vectors of n4 float4 with their grid (kOptBlocks x kOptThreads),
+// every load of an element issued before the sum that uses it.
+constexpr int kStreamMax = 32;
+struct StreamPtrs {
+  const float4* in[kStreamMax];
+  float4* out[kStreamMax];
+};
+template <int R, int W>
+__global__ void __launch_bounds__(kOptThreads) k_stream(StreamPtrs p, size_t n4, float* sink) {
+  float keep = 0.f;
+  for (size_t i = blockIdx.x * (size_t)kOptThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kOptThreads) {
+    float4 v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = p.in[j][i];
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      acc.x += v[j].x;
+      acc.y += v[j].y;
+      acc.z += v[j].z;
+      acc.w += v[j].w;
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) p.out[j][i] = acc;
+    keep += acc.x;
+  }
+  if (keep == 1234.5f) sink[0] = keep;   // keeps the loads of a read-only pattern
+}
+
+bool launch_stream_pattern(int reads, int writes, const float* const* in, float* const* out, size_t n4, float* sink,
+                           hipStream_t s) {
+  if (reads < 0 || writes < 0 || reads + writes > kStreamMax) return false;
+  StreamPtrs p{};
+  for (int j = 0; j < reads; ++j) p.in[j] = reinterpret_cast<const float4*>(in[j]);
+  for (int j = 0; j < writes; ++j) p.out[j] = reinterpret_cast<float4*>(out[j]);
+  const dim3 grid(kOptBlocks), blk(kOptThreads);
+  if (reads == 24 && writes == 0)
+    hipLaunchKernelGGL((k_stream<24, 0>), grid, blk, 0, s, p, n4, sink);
+  else if (reads == 22 && writes == 1)
+    hipLaunchKernelGGL((k_stream<22, 1>), grid, blk, 0, s, p, n4, sink);
+  else if (reads == 24 && writes == 4)
+    hipLaunchKernelGGL((k_stream<24, 4>), grid, blk, 0, s, p, n4, sink);
+  else if (reads == 21 && writes == 1)
+    hipLaunchKernelGGL((k_stream<21, 1>), grid, blk, 0, s, p, n4, sink);
+  else
+    return false;
+  return true;
+}
+
 // One block per (v, tile): thread t sums blocks t, t + 256, ... in order, then a
 // fixed shuffle tree and a fixed sum over the four waves.
 __global__ void __launch_bounds__(256)

@@ -181,6 +181,7 @@ SIGNATURES = {
     "litho_initialize_theta": (_I, [_P, _P, _P, C.c_size_t]),
     "litho_profile_ctm_eval": (_I, [_P, _I, _P, _P, _D, _D, _I, _DP]),
     "litho_profile_lbfgs_pair": (_I, [_P, C.c_size_t, _I, _I, _DP]),
+    "litho_profile_stream": (_I, [_P, C.c_size_t, _I, _I, _I, _DP]),
     "litho_lbfgs_direction": (_I, [_P, C.c_size_t, _I, _I, _P, _P, _P, _DP, _IP]),
     "litho_ls_terms": (_I, [_P, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P]),
     "litho_ls_dt": (_D, [_D, _D, _D, _D, _D, _D]),
@@ -299,6 +300,12 @@ class Context:
         """HIP-event time (ms) of one L-BFGS pair-kernel launch over n floats with m pairs."""
         ms = C.c_double(0.0)
         check(self.lib.litho_profile_lbfgs_pair(self.h, n, m, repeats, C.byref(ms)))
+        return ms.value
+
+    def profile_stream(self, n, reads, writes, repeats=20):
+        """HIP-event time (ms) of a plain kernel reading `reads` and writing `writes` vectors of n floats."""
+        ms = C.c_double(0.0)
+        check(self.lib.litho_profile_stream(self.h, n, reads, writes, repeats, C.byref(ms)))
         return ms.value
 
     def lbfgs_direction(self, x_points, g_points, d_out, m=10):

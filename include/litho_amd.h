@@ -266,6 +266,15 @@ int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, cons
  * `repeats` back-to-back launches, by HIP events on the context stream. */
 int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, double* ms);
 
+/* The yardstick of that kernel (no reference counterpart): a plain kernel
+ * that reads `reads` and writes `writes` vectors of n floats (a multiple of
+ * 4) with the L-BFGS vector kernels' grid, all loads of an element in flight
+ * before its sum.  (reads, writes) is one of (24, 0) (the pair kernel's
+ * pattern at m = 10), (22, 1) (the direction kernel's), (24, 4), (21, 1);
+ * anything else is LITHO_EINVAL.  *ms = mean over `repeats` launches, by HIP
+ * events on the context stream. */
+int litho_profile_stream(litho_ctx* ctx, size_t n, int reads, int writes, int repeats, double* ms);
+
 /* The L-BFGS history and inverse-Hessian product of CTM_Optimizer::optimize_lbfgs
  * (CTM_Optimizer.cpp:194-212), i.e. LBFGS++'s BFGSMat::add_correction and
  * apply_Hv, on the device kernels that litho_ctm_optimize runs: for the

@@ -223,6 +223,19 @@ def test_profile_lbfgs_pair(gpu_ctx):
         gpu_ctx.profile_lbfgs_pair(64 * 64, 0, 2)
 
 
+def test_profile_stream(gpu_ctx):
+    """The access-pattern yardstick of the L-BFGS vector kernels runs the
+    patterns it names and rejects the others and unaligned sizes."""
+    print("case profile_stream 64x64", flush=True)
+    la = litho_amd()
+    for r, w in [(24, 0), (22, 1), (24, 4), (21, 1)]:
+        assert gpu_ctx.profile_stream(64 * 64, r, w, 2) > 0.0
+    with pytest.raises(la.LithoError):
+        gpu_ctx.profile_stream(64 * 64, 5, 0, 2)
+    with pytest.raises(la.LithoError):
+        gpu_ctx.profile_stream(64 * 64 + 2, 24, 0, 2)
+
+
 @pytest.mark.parametrize("m,points", [(5, 14), (10, 4), (3, 1)])
 def test_lbfgs_direction_matches_two_loop(gpu_ctx, m, points):
     """The device history update and compact-form direction (the kernels of
