@@ -149,10 +149,11 @@ __device__ __forceinline__ V ldv(const V* base, unsigned byte_off) {
 // G) of the same groups, lanes 0-31 the first group of a wave and lanes
 // 32-63 the second.  So each lane keeps 2 (MB + 1) FP64 accumulators, the
 // point bases of a wave are uniform, and both sides of a group load x, xp, g,
-// gp at the same addresses (the second load hits in cache).  All the point
-// loads of an element are issued before its arithmetic; pairs beyond m load
-// the base and are not accumulated.  Nothing is written: the trial point
-// becomes the base by its index alone.
+// gp at the same addresses (the second load hits in cache).  In the chained
+// sweep all the point loads of an element are issued before its arithmetic
+// (pairs beyond m load the base and are not accumulated); a broken chain
+// loads each held pair's points under its own test.  Nothing is written: the
+// trial point becomes the base by its index alone.
 template <int MB, class V, bool CHAIN>
 __device__ __forceinline__ void pair_sweep(const V* __restrict__ hv, const V* __restrict__ x4,
                                            const V* __restrict__ xp4, const V* __restrict__ g4,
@@ -255,9 +256,10 @@ k_lbfgs_pair(const float* __restrict__ X, const float* __restrict__ G, int point
 
 // d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB held pairs,
 // g = G[base], plus the FP64 partial of g . d.  coef layout per tile: [0] cg,
-// [1+j] cs_j, [1+kMaxHist+j] cy_j.  All the point loads of an element are
-// issued before its arithmetic (pairs beyond m load the base, unused).  The
-// sweep runs from the highest addresses down: the pair kernel has just read
+// [1+j] cs_j, [1+kMaxHist+j] cy_j.  In the chained sweep all the point loads
+// of an element are issued before its arithmetic (pairs beyond m load the
+// base, unused), as in the pair kernel.  The sweep runs from the highest
+// addresses down: the pair kernel has just read
 // the same points upwards, so the lines it read last, still in the
 // last-level cache, are read first.
 template <int MB, class V, bool CHAIN>
