@@ -129,10 +129,27 @@ __device__ __forceinline__ const V* point_ptr(const V* slots, const int* mt, int
 // Load at a byte offset from a wave-uniform base: the offset stays 32 bits (a
 // point holds at most kGenericMaxN^2 floats), so every load of a sweep shares
 // one offset register and takes its base from scalar registers.
-template <class V>
-__device__ __forceinline__ V ldv(const V* base, unsigned byte_off) {
-  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+__device__ __forceinline__ float ld_nt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+  typedef float nf4 __attribute__((ext_vector_type(4)));
+  const nf4 v = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
 }
+template <bool NT = false, class V>
+__device__ __forceinline__ V ldv(const V* base, unsigned byte_off) {
+  const V* p = reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+  if constexpr (NT)
+    return ld_nt(p);
+  else
+    return *p;
+}
+// The pair and direction kernels load the history points non-temporally (the
+// nt bit of global_load): each point is read once per kernel, and over the
+// 24 streams of 16.8 MB of a 2048^2 tile the nt loads read 5.5-5.8 TB/s
+// against 4.9-5.1 TB/s for plain loads.  A 2048^2 L-BFGS solve runs 4-5 %
+// faster with nt loads in both kernels; in the pair kernel alone they gain
+// nothing in the solve.
+constexpr bool kNtPair = true, kNtDir = true;
 
 // L-BFGS pair dots.  For tile b the new pair is s = x - xp, y = g - gp with x,
 // g the trial point and xp, gp the base; the held pairs are s_j = X[end_j] -
@@ -166,10 +183,10 @@ __device__ __forceinline__ void pair_sweep(const V* __restrict__ hv, const V* __
 #pragma unroll
     for (int j = 0; j < MB; ++j) pc[j + 1] = point_ptr(hv, mt, j < m ? mt[kMetaEnd + j] : -1, n4);
     for (unsigned bo = bo0; bo < bend; bo += bstride) {
-      const V xv = ldv(x4, bo), xpv = ldv(xp4, bo), gv = ldv(g4, bo), gpv = ldv(gp4, bo);
+      const V xv = ldv<kNtPair>(x4, bo), xpv = ldv<kNtPair>(xp4, bo), gv = ldv<kNtPair>(g4, bo), gpv = ldv<kNtPair>(gp4, bo);
       V cv[MB + 1];
 #pragma unroll
-      for (int j = 0; j <= MB; ++j) cv[j] = ldv(pc[j], bo);
+      for (int j = 0; j <= MB; ++j) cv[j] = ldv<kNtPair>(pc[j], bo);
       const V y = sub4(gv, gpv);
       const V w = h ? y : sub4(xv, xpv);
 #pragma unroll
@@ -185,14 +202,14 @@ __device__ __forceinline__ void pair_sweep(const V* __restrict__ hv, const V* __
     }
   } else {
     for (unsigned bo = bo0; bo < bend; bo += bstride) {
-      const V xv = ldv(x4, bo), xpv = ldv(xp4, bo), gv = ldv(g4, bo), gpv = ldv(gp4, bo);
+      const V xv = ldv<kNtPair>(x4, bo), xpv = ldv<kNtPair>(xp4, bo), gv = ldv<kNtPair>(g4, bo), gpv = ldv<kNtPair>(gp4, bo);
       const V y = sub4(gv, gpv);
       const V w = h ? y : sub4(xv, xpv);
 #pragma unroll
       for (int j = 0; j < MB; ++j) {
         if (j < m) {
-          const V vj = sub4(ldv(point_ptr(hv, mt, mt[kMetaEnd + j], n4), bo),
-                            ldv(point_ptr(hv, mt, mt[kMetaStart + j], n4), bo));
+          const V vj = sub4(ldv<kNtPair>(point_ptr(hv, mt, mt[kMetaEnd + j], n4), bo),
+                            ldv<kNtPair>(point_ptr(hv, mt, mt[kMetaStart + j], n4), bo));
           acc[2 * j + 0] += dot4(vj, y);
           acc[2 * j + 1] += dot4(vj, gv);
         }
@@ -284,12 +301,12 @@ __device__ __forceinline__ double direction_sweep(const V* __restrict__ X4, cons
     }
     for (unsigned kk = bo0 < bend ? (bend - bo0 + bstride - 1) / bstride : 0; kk-- > 0;) {
       const unsigned bo = bo0 + kk * bstride;
-      const V gv = ldv(gb, bo);
+      const V gv = ldv<kNtDir>(gb, bo);
       V xv[MB + 1], gvv[MB + 1];
 #pragma unroll
       for (int j = 0; j <= MB; ++j) {
-        xv[j] = ldv(cx[j], bo);
-        gvv[j] = ldv(cgp[j], bo);
+        xv[j] = ldv<kNtDir>(cx[j], bo);
+        gvv[j] = ldv<kNtDir>(cgp[j], bo);
       }
       V v = scale4(cg, gv);
 #pragma unroll
@@ -301,14 +318,14 @@ __device__ __forceinline__ double direction_sweep(const V* __restrict__ X4, cons
   } else {
     for (unsigned kk = bo0 < bend ? (bend - bo0 + bstride - 1) / bstride : 0; kk-- > 0;) {
       const unsigned bo = bo0 + kk * bstride;
-      const V gv = ldv(gb, bo);
+      const V gv = ldv<kNtDir>(gb, bo);
       V v = scale4(cg, gv);
 #pragma unroll
       for (int j = 0; j < MB; ++j) {
         if (j < m) {
           const int sp = mt[kMetaStart + j], ep = mt[kMetaEnd + j];
-          const V sj = sub4(ldv(point_ptr(X4, mt, ep, n4), bo), ldv(point_ptr(X4, mt, sp, n4), bo));
-          const V yj = sub4(ldv(point_ptr(G4, mt, ep, n4), bo), ldv(point_ptr(G4, mt, sp, n4), bo));
+          const V sj = sub4(ldv<kNtDir>(point_ptr(X4, mt, ep, n4), bo), ldv<kNtDir>(point_ptr(X4, mt, sp, n4), bo));
+          const V yj = sub4(ldv<kNtDir>(point_ptr(G4, mt, ep, n4), bo), ldv<kNtDir>(point_ptr(G4, mt, sp, n4), bo));
           comb4(v, cs[j], sj, cy[j], yj);
         }
       }
@@ -639,8 +656,9 @@ void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, 
 }
 
 // The yardstick of the L-BFGS vector kernels: a plain kernel that reads R and
-// writes W vectors of n4 float4 with their grid (kOptBlocks x kOptThreads),
-// every load of an element issued before the sum that uses it.
+// writes W vectors of n4 float4 with their grid (kOptBlocks x kOptThreads)
+// and their non-temporal loads, every load of an element issued before the
+// sum that uses it.
 constexpr int kStreamMax = 32;
 struct StreamPtrs {
   const float4* in[kStreamMax];
@@ -652,7 +670,7 @@ __global__ void __launch_bounds__(kOptThreads) k_stream(StreamPtrs p, size_t n4,
   for (size_t i = blockIdx.x * (size_t)kOptThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kOptThreads) {
     float4 v[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = p.in[j][i];
+    for (int j = 0; j < R; ++j) v[j] = ld_nt(p.in[j] + i);
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int j = 0; j < R; ++j) {

@@ -70,7 +70,8 @@ void launch_lbfgs_start(const float* G, int points, const int* flags, int* meta,
 void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, int which, size_t n,
                             float* out_theta, float* out_mask, hipStream_t s);
 // A plain kernel reading `reads` and writing `writes` vectors of n4 float4
-// with the L-BFGS kernels' grid (the rate their access pattern reaches);
+// with the L-BFGS kernels' grid and load policy (the rate their access
+// pattern reaches);
 // (reads, writes) one of (24, 0), (22, 1), (24, 4), (21, 1), else false.
 bool launch_stream_pattern(int reads, int writes, const float* const* in, float* const* out, size_t n4, float* sink,
                            hipStream_t s);

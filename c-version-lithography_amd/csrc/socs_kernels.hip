@@ -43,6 +43,18 @@
 
 namespace litho {
 
+// Non-temporal load (the nt bit of global_load) of a value read once and not
+// again before HBM holds it anew: the row adjoint's V_k loads.  With them a
+// 2048^2 L-BFGS solve runs about 3 % faster; the same policy on the
+// correlation kernels' T_k loads, the final column kernel's theta and d
+// loads, the final row kernel's Y_k loads or the spectrum partials measured
+// slower or no faster.
+__device__ __forceinline__ float2 ld_nt2(const float2* p) {
+  typedef float nf2 __attribute__((ext_vector_type(2)));
+  const nf2 v = __builtin_nontemporal_load(reinterpret_cast<const nf2*>(p));
+  return make_float2(v.x, v.y);
+}
+
 using fft::cconj;
 using fft::cmul;
 using fft::cmulc;
@@ -705,7 +717,7 @@ k_row_adj_pruned(SocsArgs a) {
   float2* yk = a.y + (size_t)b * rows * a.wc;
   RF::forward(
       buf, a.rtw,
-      [&](int l, int x) { return q0 + l < rows ? v[(size_t)(q0 + l) * N + x] : make_float2(0.f, 0.f); },
+      [&](int l, int x) { return q0 + l < rows ? ld_nt2(v + (size_t)(q0 + l) * N + x) : make_float2(0.f, 0.f); },
       [&](int l, int u, float2 z) {
         const int q = q0 + l, j = a.col_of_u[u];
         if (q < rows && j >= 0) yk[(size_t)q * a.wc + j] = cmulc(z, a.h[(size_t)q * a.wc + j]);

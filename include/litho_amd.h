@@ -268,8 +268,8 @@ int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, doubl
 
 /* The yardstick of that kernel (no reference counterpart): a plain kernel
  * that reads `reads` and writes `writes` vectors of n floats (a multiple of
- * 4) with the L-BFGS vector kernels' grid, all loads of an element in flight
- * before its sum.  (reads, writes) is one of (24, 0) (the pair kernel's
+ * 4) with the L-BFGS vector kernels' grid and non-temporal loads, all loads
+ * of an element in flight before its sum.  (reads, writes) is one of (24, 0) (the pair kernel's
  * pattern at m = 10), (22, 1) (the direction kernel's), (24, 4), (21, 1);
  * anything else is LITHO_EINVAL.  *ms = mean over `repeats` launches, by HIP
  * events on the context stream. */
