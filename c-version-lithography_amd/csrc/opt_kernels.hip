@@ -120,6 +120,13 @@ __device__ __forceinline__ bool pairs_chained(const int* mt, int m) {
   for (int j = 1; j < m; ++j) chain = chain && mt[kMetaStart + j] == mt[kMetaEnd + j - 1];
   return chain;
 }
+// Whether the chain of held pairs also ends at the base (m = 0 counts: the
+// chain is the base alone).  Then the last chain point is the base, and the
+// chained sweeps take x_p and g_p from it instead of loading them again.  A
+// rejected newest pair (the base moved without a pair) breaks this.
+__device__ __forceinline__ bool chain_ends_at_base(const int* mt, int m) {
+  return pairs_chained(mt, m) && (m == 0 || mt[kMetaEnd + m - 1] == mt[kMetaBase]);
+}
 // Point p of a tile's slots (p < 0: the base, a valid stand-in for the pairs
 // beyond m, whose loads are issued but not used).
 template <class V>
@@ -164,59 +171,73 @@ constexpr bool kNtPair = true, kNtDir = true;
 // strided over the grid) are taken by two waves: waves 0 and 1 the s side
 // (points of X) of groups 0-1 and 2-3, waves 2 and 3 the y side (points of
 // G) of the same groups, lanes 0-31 the first group of a wave and lanes
-// 32-63 the second.  So each lane keeps 2 (MB + 1) FP64 accumulators, the
-// point bases of a wave are uniform, and both sides of a group load x, xp, g,
-// gp at the same addresses (the second load hits in cache).  In the chained
-// sweep all the point loads of an element are issued before its arithmetic
-// (pairs beyond m load the base and are not accumulated); a broken chain
-// loads each held pair's points under its own test.  Nothing is written: the
-// trial point becomes the base by its index alone.
-template <int MB, class V, bool CHAIN>
-__device__ __forceinline__ void pair_sweep(const V* __restrict__ hv, const V* __restrict__ x4,
-                                           const V* __restrict__ xp4, const V* __restrict__ g4,
-                                           const V* __restrict__ gp4, const int* mt, int m, int h, unsigned bo0,
-                                           unsigned bstride, unsigned bend, size_t n4, double (&acc)[2 * (MB + 1)]) {
-  if constexpr (CHAIN) {
-    // chain points c_0 = start of pair 0, c_{j+1} = end of pair j
-    const V* pc[MB + 1];
-    pc[0] = point_ptr(hv, mt, m > 0 ? mt[kMetaStart] : -1, n4);
+// 32-63 the second.  So each lane keeps 2 (MB + 1) FP64 accumulators and the
+// point bases of a wave are uniform.  In the chained sweep (the chain ends
+// at the base) all the point loads of an element are issued before its
+// arithmetic, pairs beyond m load the base and are not accumulated, and the
+// last chain point is the base: the s side loads x, g, g_p and the X chain,
+// the y side g and the G chain, 26 loads per element group for its 24
+// points.  Otherwise each held pair's points are loaded under its own test,
+// and both sides load x, x_p, g, g_p.  Nothing is written: the trial point
+// becomes the base by its index alone.
+template <int MB, class V, int SIDE>
+__device__ __forceinline__ void pair_sweep_chain(const V* __restrict__ hv, const V* __restrict__ x4,
+                                                 const V* __restrict__ g4, const V* __restrict__ gp4, const int* mt,
+                                                 int m, unsigned bo0, unsigned bstride, unsigned bend, size_t n4,
+                                                 double (&acc)[2 * (MB + 1)]) {
+  // chain points c_0 = start of pair 0, c_{j+1} = end of pair j, c_MB = the base
+  const V* pc[MB + 1];
+  pc[0] = point_ptr(hv, mt, m > 0 ? mt[kMetaStart] : -1, n4);
 #pragma unroll
-    for (int j = 0; j < MB; ++j) pc[j + 1] = point_ptr(hv, mt, j < m ? mt[kMetaEnd + j] : -1, n4);
-    for (unsigned bo = bo0; bo < bend; bo += bstride) {
-      const V xv = ldv<kNtPair>(x4, bo), xpv = ldv<kNtPair>(xp4, bo), gv = ldv<kNtPair>(g4, bo), gpv = ldv<kNtPair>(gp4, bo);
-      V cv[MB + 1];
-#pragma unroll
-      for (int j = 0; j <= MB; ++j) cv[j] = ldv<kNtPair>(pc[j], bo);
-      const V y = sub4(gv, gpv);
-      const V w = h ? y : sub4(xv, xpv);
-#pragma unroll
-      for (int j = 0; j < MB; ++j) {
-        if (j < m) {
-          const V vj = sub4(cv[j + 1], cv[j]);
-          acc[2 * j + 0] += dot4(vj, y);
-          acc[2 * j + 1] += dot4(vj, gv);
-        }
-      }
-      acc[2 * MB + 0] += dot4(w, y);
-      acc[2 * MB + 1] += dot4(w, gv);
+  for (int j = 0; j < MB; ++j) pc[j + 1] = point_ptr(hv, mt, j < m ? mt[kMetaEnd + j] : -1, n4);
+  for (unsigned bo = bo0; bo < bend; bo += bstride) {
+    V xv{}, gpv{};
+    if constexpr (SIDE == 0) {
+      xv = ldv<kNtPair>(x4, bo);
+      gpv = ldv<kNtPair>(gp4, bo);
     }
-  } else {
-    for (unsigned bo = bo0; bo < bend; bo += bstride) {
-      const V xv = ldv<kNtPair>(x4, bo), xpv = ldv<kNtPair>(xp4, bo), gv = ldv<kNtPair>(g4, bo), gpv = ldv<kNtPair>(gp4, bo);
-      const V y = sub4(gv, gpv);
-      const V w = h ? y : sub4(xv, xpv);
+    const V gv = ldv<kNtPair>(g4, bo);
+    V cv[MB + 1];
 #pragma unroll
-      for (int j = 0; j < MB; ++j) {
-        if (j < m) {
-          const V vj = sub4(ldv<kNtPair>(point_ptr(hv, mt, mt[kMetaEnd + j], n4), bo),
-                            ldv<kNtPair>(point_ptr(hv, mt, mt[kMetaStart + j], n4), bo));
-          acc[2 * j + 0] += dot4(vj, y);
-          acc[2 * j + 1] += dot4(vj, gv);
-        }
+    for (int j = 0; j <= MB; ++j) cv[j] = ldv<kNtPair>(pc[j], bo);
+    // y = g - g_p, s = x - x_p, x_p (g_p) being the last X (G) chain point
+    const V y = SIDE == 0 ? sub4(gv, gpv) : sub4(gv, cv[MB]);
+    const V w = SIDE == 0 ? sub4(xv, cv[MB]) : y;
+#pragma unroll
+    for (int j = 0; j < MB; ++j) {
+      if (j < m) {
+        const V vj = sub4(cv[j + 1], cv[j]);
+        acc[2 * j + 0] += dot4(vj, y);
+        acc[2 * j + 1] += dot4(vj, gv);
       }
-      acc[2 * MB + 0] += dot4(w, y);
-      acc[2 * MB + 1] += dot4(w, gv);
     }
+    acc[2 * MB + 0] += dot4(w, y);
+    acc[2 * MB + 1] += dot4(w, gv);
+  }
+}
+
+template <int MB, class V>
+__device__ __forceinline__ void pair_sweep_broken(const V* __restrict__ hv, const V* __restrict__ x4,
+                                                  const V* __restrict__ xp4, const V* __restrict__ g4,
+                                                  const V* __restrict__ gp4, const int* mt, int m, int h,
+                                                  unsigned bo0, unsigned bstride, unsigned bend, size_t n4,
+                                                  double (&acc)[2 * (MB + 1)]) {
+  for (unsigned bo = bo0; bo < bend; bo += bstride) {
+    const V xv = ldv<kNtPair>(x4, bo), xpv = ldv<kNtPair>(xp4, bo), gv = ldv<kNtPair>(g4, bo),
+            gpv = ldv<kNtPair>(gp4, bo);
+    const V y = sub4(gv, gpv);
+    const V w = h ? y : sub4(xv, xpv);
+#pragma unroll
+    for (int j = 0; j < MB; ++j) {
+      if (j < m) {
+        const V vj = sub4(ldv<kNtPair>(point_ptr(hv, mt, mt[kMetaEnd + j], n4), bo),
+                          ldv<kNtPair>(point_ptr(hv, mt, mt[kMetaStart + j], n4), bo));
+        acc[2 * j + 0] += dot4(vj, y);
+        acc[2 * j + 1] += dot4(vj, gv);
+      }
+    }
+    acc[2 * MB + 0] += dot4(w, y);
+    acc[2 * MB + 1] += dot4(w, gv);
   }
 }
 
@@ -247,11 +268,13 @@ k_lbfgs_pair(const float* __restrict__ X, const float* __restrict__ G, int point
   const unsigned bstride = (unsigned)((size_t)gridDim.x * WAVES * 32 * sizeof(V));
   const unsigned bend = (unsigned)(n4 * sizeof(V));
   const V* hv = h ? G4 : X4;                                 // wave-uniform
-  if (pairs_chained(mt, m))
-    pair_sweep<MB, V, true>(hv, X4 + trial, X4 + base, G4 + trial, G4 + base, mt, m, h, bo0, bstride, bend, n4, acc);
-  else
-    pair_sweep<MB, V, false>(hv, X4 + trial, X4 + base, G4 + trial, G4 + base, mt, m, h, bo0, bstride, bend, n4,
+  if (!chain_ends_at_base(mt, m))
+    pair_sweep_broken<MB, V>(hv, X4 + trial, X4 + base, G4 + trial, G4 + base, mt, m, h, bo0, bstride, bend, n4,
                              acc);
+  else if (h == 0)
+    pair_sweep_chain<MB, V, 0>(hv, X4 + trial, G4 + trial, G4 + base, mt, m, bo0, bstride, bend, n4, acc);
+  else
+    pair_sweep_chain<MB, V, 1>(hv, X4 + trial, G4 + trial, G4 + base, mt, m, bo0, bstride, bend, n4, acc);
   // fixed-order reduction: each group over its 32 lanes, then the groups in order
   __shared__ double red[WAVES][2][NA];
 #pragma unroll
@@ -273,9 +296,10 @@ k_lbfgs_pair(const float* __restrict__ X, const float* __restrict__ G, int point
 
 // d = cg * g + sum_j cs_j s_j + sum_j cy_j y_j over the m <= MB held pairs,
 // g = G[base], plus the FP64 partial of g . d.  coef layout per tile: [0] cg,
-// [1+j] cs_j, [1+kMaxHist+j] cy_j.  In the chained sweep all the point loads
-// of an element are issued before its arithmetic (pairs beyond m load the
-// base, unused), as in the pair kernel.  The sweep runs from the highest
+// [1+j] cs_j, [1+kMaxHist+j] cy_j.  In the chained sweep (the chain ends at
+// the base) all the point loads of an element are issued before its
+// arithmetic (pairs beyond m load the base, unused), as in the pair kernel,
+// and g is the last G chain point.  The sweep runs from the highest
 // addresses down: the pair kernel has just read
 // the same points upwards, so the lines it read last, still in the
 // last-level cache, are read first.
@@ -301,13 +325,13 @@ __device__ __forceinline__ double direction_sweep(const V* __restrict__ X4, cons
     }
     for (unsigned kk = bo0 < bend ? (bend - bo0 + bstride - 1) / bstride : 0; kk-- > 0;) {
       const unsigned bo = bo0 + kk * bstride;
-      const V gv = ldv<kNtDir>(gb, bo);
       V xv[MB + 1], gvv[MB + 1];
 #pragma unroll
       for (int j = 0; j <= MB; ++j) {
         xv[j] = ldv<kNtDir>(cx[j], bo);
         gvv[j] = ldv<kNtDir>(cgp[j], bo);
       }
+      const V gv = gvv[MB];   // the base gradient, the last chain point
       V v = scale4(cg, gv);
 #pragma unroll
       for (int j = 0; j < MB; ++j)
@@ -362,7 +386,7 @@ k_lbfgs_direction(const float* __restrict__ X, const float* __restrict__ G, int 
   const unsigned bstride = (unsigned)((size_t)gridDim.x * blockDim.x * sizeof(V));
   const unsigned bend = (unsigned)(n4 * sizeof(V));
   double acc[1];
-  acc[0] = pairs_chained(mt, m)
+  acc[0] = chain_ends_at_base(mt, m)
                ? direction_sweep<MB, V, true>(X4, G4, gb, mt, m, cg, cs, cy, d4, bo0, bstride, bend, n4)
                : direction_sweep<MB, V, false>(X4, G4, gb, mt, m, cg, cs, cy, d4, bo0, bstride, bend, n4);
   block_reduce_store<1>(acc, partials + ((size_t)b * gridDim.x + blockIdx.x));
