@@ -236,12 +236,14 @@ def test_profile_stream(gpu_ctx):
         gpu_ctx.profile_stream(64 * 64 + 2, 24, 0, 2)
 
 
-@pytest.mark.parametrize("m,points", [(5, 14), (10, 4), (3, 1)])
+@pytest.mark.parametrize("m,points", [(5, 14), (10, 4), (4, 3), (3, 1)])
 def test_lbfgs_direction_matches_two_loop(gpu_ctx, m, points):
     """The device history update and compact-form direction (the kernels of
     litho_ctm_optimize) against the oracle's two-loop recursion on the same
     iterates: more pairs than m (the oldest leave), one pair with s.y < 0
-    (rejected), and no pair at all (d = -g)."""
+    (rejected: with 3 points it is the newest, so the held pairs end short
+    of the base and the sweeps load each pair's points), and no pair at all
+    (d = -g)."""
     import torch
     print(f"case lbfgs_direction m={m} points={points}", flush=True)
     n = 64 * 64
