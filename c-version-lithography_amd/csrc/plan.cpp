@@ -199,6 +199,7 @@ void run_eval(Plan& plan, int batch, const EvalRequest& rq) {
   if (rq.adjoint && rq.im.target != nullptr) {
     GradArgs ga = rq.ga;
     ga.partials = plan.grad_partials.as<double>();
+    ga.nt_grad_theta = batch == 1 ? 1 : 0;
     plan.ops->adjoint_tail(a, ga, batch, s);
     launch_reduce_eval(plan.img_partials.as<double>(), plan.img_blocks, plan.grad_partials.as<double>(),
                        plan.col_blocks, red, batch, s, rq.active, rq.extra_partials, rq.extra_blocks, rq.extra_out);
@@ -277,6 +278,7 @@ void profile_eval(Plan& plan, int batch, const EvalRequest& rq, int repeats, dou
   im.partials = plan.img_partials.as<double>();
   GradArgs ga = rq.ga;
   ga.partials = plan.grad_partials.as<double>();
+  ga.nt_grad_theta = batch == 1 ? 1 : 0;   // as run_eval sets it
   // untimed warm-up: the first launch of each kernel loads its code object
   plan.ops->forward_half(a, batch, rq.mode, rq.base, rq.dir, rq.coef, rq.theta_out, rq.mask_out, s);
   plan.ops->image_adjoint(a, im, batch, s, kColumnAll);

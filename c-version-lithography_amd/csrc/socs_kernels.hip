@@ -58,9 +58,10 @@ __device__ __forceinline__ float2 ld_nt2(const float2* p) {
 // Non-temporal store of the trial theta an evaluation writes into its point
 // slot, which the L-BFGS sweeps read non-temporally from HBM later: a 2048^2
 // L-BFGS solve runs about 1 % faster with it, batched tiles the same.  The
-// same store of the theta gradient made the 2048^2 solve a further 1 %
-// faster but the batched 640^2-1152^2 tiles 7 % slower, and on the T_k and
-// V_k stores it measured no faster; those stores stay plain.
+// theta gradient is stored the same way only by evaluations of one tile
+// (GradArgs::nt_grad_theta): 0.5-1 % faster at 2048^2, while in batches of
+// 640^2-1152^2 tiles it cost 7 %.  On the T_k and V_k stores it measured no
+// faster; those stores stay plain.
 __device__ __forceinline__ void st_nt1(float* p, float v) { __builtin_nontemporal_store(v, p); }
 
 using fft::cconj;
@@ -813,7 +814,12 @@ k_final_col_inv(SocsArgs a, GradArgs ga) {
       if (ga.theta != nullptr) {
         float gt = -0.5f * gm * sinf(ga.theta[pt + lg]);
         if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
-        if (ga.grad_theta != nullptr) ga.grad_theta[pt + lg] = gt;
+        if (ga.grad_theta != nullptr) {
+          if (ga.nt_grad_theta)
+            st_nt1(ga.grad_theta + pt + lg, gt);
+          else
+            ga.grad_theta[pt + lg] = gt;
+        }
         nrm += (double)gt * (double)gt;
         if (ga.dir != nullptr) dot += (double)gt * (double)ga.dir[gi];
       }
@@ -1706,7 +1712,12 @@ k_final_col_inv_pruned(SocsArgs a, GradArgs ga) {
       if (ga.theta != nullptr) {
         float gt = -0.5f * gm * sinf(th[s][r]);
         if (rect != nullptr && (y < rect[0] || y >= rect[1] || x < rect[2] || x >= rect[3])) gt = 0.f;
-        if (ga.grad_theta != nullptr) ga.grad_theta[pt + lg] = gt;
+        if (ga.grad_theta != nullptr) {
+          if (ga.nt_grad_theta)
+            st_nt1(ga.grad_theta + pt + lg, gt);
+          else
+            ga.grad_theta[pt + lg] = gt;
+        }
         nrm += (double)gt * (double)gt;
         th[s][r] = gt;
       }

@@ -90,6 +90,7 @@ struct GradArgs {
   const float* dir;               // optional: partial dot grad_theta . dir
   const int* rect;                // optional [b][4] = y0,y1,x0,x1 update window
   double* partials;               // [b][col_blocks][2]: |G_theta|^2, G_theta . dir
+  int nt_grad_theta = 0;          // 1: grad_theta stored non-temporally (evaluations of one tile)
 };
 
 // per-pixel resist, losses and dL/dI shared by the column image kernels
