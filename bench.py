@@ -9,10 +9,10 @@ One step = one full CTM L-BFGS solve; a grad-eval is one objective
 evaluation (line-search trials included).  Inputs are resident in HBM before
 the timed region.  FP32 fields, FP64 loss / norm / L-BFGS scalars.
 
-After the timed solves, a short served-tile run (Poisson requests of 384 /
-640 / 1152 / 2176 px tiles at 70 % of capacity) reports the tile p50 / p99
-latency of continuous batching against the plain one-request-at-a-time path
-(`served_tiles`; `--no-served` skips it).
+Last, with the headline context closed, a short served-tile run (Poisson
+requests of 384 / 640 / 1152 / 2176 px tiles at 70 % of capacity) reports
+the tile p50 / p99 latency of continuous batching against the plain
+one-request-at-a-time path (`served_tiles`; `--no-served` skips it).
 
 The CPU baseline is the FP64 numpy/scipy oracle (the reference's full-FFT
 dataflow, 3K+1 N x N FFTs per grad-eval) timed on bounded samples on this
@@ -289,6 +289,13 @@ def main():
                                                  (st[4] * 1e-3) / HBM_PEAK, 4))},
     }
     plan.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(dk, target, ctx, tg)
+    # The served run's four pools (one stream each) need a hardware queue
+    # each, and HIP gives a process four by default: a fifth stream that has
+    # run work (this context's) makes two pools share one, which doubled the
+    # served p50.  So this context is closed first.
+    ctx.close()
     if not args.no_served:
         # the metric's second half, tile p99 latency vs plain: Poisson requests
         # of 384 / 640 / 1152 / 2176 px tiles (20 GD iterations each) at 70 % of
@@ -301,9 +308,6 @@ def main():
                                                    "identical_masks_alone_vs_mixed")}
         print(f"served: p50 {sv['served']['p50_ms']} ms, p99 {sv['served']['p99_ms']} ms vs plain p99 "
               f"{sv['plain']['p99_ms']} ms", file=sys.stderr, flush=True)
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(dk, target, ctx, tg)
-    ctx.close()
     print(json.dumps(line), flush=True)
 
 
