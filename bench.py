@@ -239,7 +239,9 @@ def main():
     pair_ms = pair_total_ms / pair_launches
     pair_bytes = pair_kernel_bytes(N, HISTORY)
     # the yardstick: a plain kernel with the pair kernel's access pattern
-    # (2m + 4 = 24 vectors read, none written, the same grid), HIP events
+    # (2m + 4 = 24 vectors read, none written, the same grid and loads),
+    # each launch after a read that evicts the last-level cache, as the
+    # sweeps of a solve run after an evaluation; HIP events
     pattern_ms = ctx.profile_stream(N * N, 2 * HISTORY + 4, 0, repeats=20)
     print(f"stages (ms): forward {st[0]:.3f} column image {st[1]:.3f} column adjoint {st[2]:.3f} "
           f"adjoint-tail {st[3]:.3f} eval {st[4]:.3f}; {PAIR_KERNEL} {pair_ms:.4f} (mean of {pair_launches} "

@@ -710,6 +710,21 @@ __global__ void __launch_bounds__(kOptThreads) k_stream(StreamPtrs p, size_t n4,
   if (keep == 1234.5f) sink[0] = keep;   // keeps the loads of a read-only pattern
 }
 
+// Plain loads of n4 float4 (one word of each): a read larger than the
+// last-level cache leaves none of the timed pattern's vectors there, as in a
+// solve, where an evaluation runs between the L-BFGS sweeps.
+__global__ void __launch_bounds__(kOptThreads) k_cache_evict(const float4* __restrict__ buf, size_t n4, float* sink) {
+  float keep = 0.f;
+  for (size_t i = blockIdx.x * (size_t)kOptThreads + threadIdx.x; i < n4; i += (size_t)gridDim.x * kOptThreads)
+    keep += buf[i].x;
+  if (keep == 1234.5f) sink[0] = keep;
+}
+
+void launch_cache_evict(const float* buf, size_t n4, float* sink, hipStream_t s) {
+  hipLaunchKernelGGL(k_cache_evict, dim3(4096), dim3(kOptThreads), 0, s, reinterpret_cast<const float4*>(buf), n4,
+                     sink);
+}
+
 bool launch_stream_pattern(int reads, int writes, const float* const* in, float* const* out, size_t n4, float* sink,
                            hipStream_t s) {
   if (reads < 0 || writes < 0 || reads + writes > kStreamMax) return false;

@@ -73,6 +73,9 @@ void launch_lbfgs_point_out(const float* X, int points, const int* meta, int b, 
 // with the L-BFGS kernels' grid and load policy (the rate their access
 // pattern reaches);
 // (reads, writes) one of (24, 0), (22, 1), (24, 4), (21, 1), else false.
+// Reads n4 float4 of buf (one word each, plain loads) to evict the
+// last-level cache before a timed launch.
+void launch_cache_evict(const float* buf, size_t n4, float* sink, hipStream_t s);
 bool launch_stream_pattern(int reads, int writes, const float* const* in, float* const* out, size_t n4, float* sink,
                            hipStream_t s);
 // out[b * out_stride + v] = sum over blk of partials[(b * nblk + blk) * nv + v] for

@@ -262,8 +262,11 @@ int litho_profile_ctm_eval(litho_plan* plan, int batch, const float* theta, cons
 /* Timing of the L-BFGS pair kernel alone (no reference counterpart; the
  * reference's LBFGS++ forms s, y and its dot products on the host): one tile
  * of n floats (a multiple of 4) with a full history of m chained pairs over
- * its point slots (2 m + 4 vectors read, none written).  *ms = mean over
- * `repeats` back-to-back launches, by HIP events on the context stream. */
+ * its point slots (2 m + 4 vectors read, none written).  Each timed launch
+ * follows a 1 GiB read that evicts the last-level cache, so the points
+ * stream cold, as in a solve (the call allocates that 1 GiB).  *ms = mean
+ * over `repeats` launches, by HIP events around each launch on the context
+ * stream. */
 int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, double* ms);
 
 /* The yardstick of that kernel (no reference counterpart): a plain kernel
@@ -271,8 +274,10 @@ int litho_profile_lbfgs_pair(litho_ctx* ctx, size_t n, int m, int repeats, doubl
  * 4) with the L-BFGS vector kernels' grid and non-temporal loads, all loads
  * of an element in flight before its sum.  (reads, writes) is one of (24, 0) (the pair kernel's
  * pattern at m = 10), (22, 1) (the direction kernel's), (24, 4), (21, 1);
- * anything else is LITHO_EINVAL.  *ms = mean over `repeats` launches, by HIP
- * events on the context stream. */
+ * anything else is LITHO_EINVAL.  Each timed launch follows a 1 GiB read
+ * that evicts the last-level cache, so the vectors stream cold, as in a
+ * solve (the call allocates that 1 GiB).  *ms = mean over `repeats`
+ * launches, by HIP events around each launch on the context stream. */
 int litho_profile_stream(litho_ctx* ctx, size_t n, int reads, int writes, int repeats, double* ms);
 
 /* The L-BFGS history and inverse-Hessian product of CTM_Optimizer::optimize_lbfgs
