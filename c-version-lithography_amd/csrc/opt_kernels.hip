@@ -133,15 +133,16 @@ template <class V>
 __device__ __forceinline__ const V* point_ptr(const V* slots, const int* mt, int p, size_t n4) {
   return slots + (size_t)(p < 0 ? mt[kMetaBase] : p) * n4;
 }
-// Load at a byte offset from a wave-uniform base: the offset stays 32 bits (a
-// point holds at most kGenericMaxN^2 floats), so every load of a sweep shares
-// one offset register and takes its base from scalar registers.
+// Non-temporal loads (the nt bit of global_load).
 __device__ __forceinline__ float ld_nt(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float4 ld_nt(const float4* p) {
   typedef float nf4 __attribute__((ext_vector_type(4)));
   const nf4 v = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(p));
   return make_float4(v.x, v.y, v.z, v.w);
 }
+// Load at a byte offset from a wave-uniform base: the offset stays 32 bits (a
+// point holds at most kGenericMaxN^2 floats), so every load of a sweep shares
+// one offset register and takes its base from scalar registers.
 template <bool NT = false, class V>
 __device__ __forceinline__ V ldv(const V* base, unsigned byte_off) {
   const V* p = reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
@@ -154,8 +155,8 @@ __device__ __forceinline__ V ldv(const V* base, unsigned byte_off) {
 // nt bit of global_load): each point is read once per kernel, and over the
 // 24 streams of 16.8 MB of a 2048^2 tile the nt loads read 5.5-5.8 TB/s
 // against 4.9-5.1 TB/s for plain loads.  A 2048^2 L-BFGS solve runs 4-5 %
-// faster with nt loads in both kernels; in the pair kernel alone they gain
-// nothing in the solve.
+// faster with nt loads in both kernels; with them in the pair kernel only,
+// it was no faster.
 constexpr bool kNtPair = true, kNtDir = true;
 
 // L-BFGS pair dots.  For tile b the new pair is s = x - xp, y = g - gp with x,
